@@ -1,0 +1,142 @@
+/*
+ * tnad.h -- C ABI of libtnad: CTMRG and TRG renormalisation steps of
+ * TensorNetworkAD.jl and their reverse-mode adjoint, as hand-written HIP
+ * kernels for AMD Instinct MI355X (gfx950), one GPU per context.
+ *
+ * Conventions
+ *  - Arrays are column-major (Julia memory order), FP64, passed as plain
+ *    pointers.  Every array argument may be a host pointer or a device
+ *    pointer on the context's GPU; the library detects which.
+ *  - Index names and orders follow the reference: the bulk (double layer)
+ *    tensor a[j,k,l,m] is D x D x D x D; the corner C is chi x chi; the edge
+ *    T[i,j,k] is chi x D x chi; the iPEPS bulk is d x d x d x d x s.
+ *  - Calls are synchronous on return.  A context owns one HIP stream and is
+ *    not thread safe; distinct contexts are independent.
+ *  - Return value: TNAD_OK (0) or a negative TNAD_ERR_* code; the message is
+ *    available from tnad_last_error().  Non-convergence is not an error: it is
+ *    reported through *iters == maxit, as the reference's fixedpoint returns
+ *    silently at maxit (src/fixedpoint.jl:33).
+ *
+ * The reference (TensorNetworkAD.jl) has no FFI; the functions below replace
+ * the Julia methods cited next to each declaration (paths relative to the
+ * reference repository).  The Julia-side ccall bindings and ChainRulesCore
+ * rrules that attach them are listed in INTEGRATION.md.
+ */
+#ifndef TNAD_H
+#define TNAD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TNAD_ABI_VERSION 1
+
+typedef struct tnad_ctx tnad_ctx;
+typedef struct tnad_tape tnad_tape;
+
+enum {
+  TNAD_OK = 0,
+  TNAD_ERR_ARG = -1,         /* invalid argument / null pointer */
+  TNAD_ERR_SHAPE = -2,       /* DimensionMismatch in the reference (src/ipeps.jl:19-20) */
+  TNAD_ERR_HIP = -3,         /* HIP runtime error */
+  TNAD_ERR_NAN = -4,         /* NaN detected in singular values / residuals */
+  TNAD_ERR_ALLOC = -5,       /* device allocation failed */
+  TNAD_ERR_UNSUPPORTED = -6  /* size outside what the kernels support */
+};
+
+enum { TNAD_PREC_FP64 = 0, TNAD_PREC_FP32 = 1, TNAD_PREC_BF16 = 2 };
+
+/* Run options of the CTMRG fixed point.  tnad_opts_default() gives the
+ * reference semantics: FP64 contractions, cold-started eigensolver every step,
+ * full unrolled reverse sweep. */
+typedef struct tnad_opts {
+  int precision;       /* contraction precision of the low-precision phase (TNAD_PREC_*) */
+  double switch_tol;   /* precision != FP64: switch to FP64 once |vals - oldvals| <= switch_tol */
+  int bwd_precision;   /* contraction precision of the reverse sweep */
+  int eig_block;       /* subspace block size k for chi*D > 192 (0 = 1.5 chi, capped at 192) */
+  int eig_max_iters;   /* subspace iterations per step, cold start */
+  int eig_warm;        /* 1: start each step's eigensolver from the previous step's Ritz vectors */
+  int eig_warm_iters;  /* subspace iterations per step, warm start */
+  double eig_tol;      /* relative Ritz residual required of the kept chi vectors */
+  int bwd_last;        /* reverse sweep over the last bwd_last steps only (0 = all steps) */
+} tnad_opts;
+
+/* Per-call statistics of the last tnad_ctmrg / tnad_energy call. */
+typedef struct tnad_stats {
+  int iters;            /* CTMRG steps taken */
+  int low_prec_iters;   /* of which at the low contraction precision */
+  int eig_iters_total;  /* subspace iterations summed over steps */
+  int eig_k;            /* eigensolver block size (n when the full spectrum is computed) */
+  double eig_resid_max; /* largest relative Ritz residual over the steps */
+  int bwd_steps;        /* steps covered by the last reverse sweep */
+} tnad_stats;
+
+int tnad_version(void);
+void tnad_opts_default(tnad_opts* opts);
+
+/* context: one GPU, one stream */
+int tnad_create(int device, tnad_ctx** ctx);
+int tnad_destroy(tnad_ctx* ctx);
+const char* tnad_last_error(const tnad_ctx* ctx);
+int tnad_get_stats(const tnad_ctx* ctx, tnad_stats* out);
+
+/* indexperm_symmetrize (src/ipeps.jl:32-39): bulk d x d x d x d x s -> out (same shape) */
+int tnad_indexperm_symmetrize(tnad_ctx* ctx, const double* bulk, int d, int s, double* out);
+
+/* double layer (src/variationalipeps.jl:30-34): ap (D,D,D,D,s,s) and a = tr_phys ap (D,D,D,D), D = d^2.
+ * ap may be NULL. */
+int tnad_double_layer(tnad_ctx* ctx, const double* bulk, int d, int s, double* ap, double* a);
+
+/* _initializect_square(bulk, Val(:raw), chi) (src/ctmrg.jl:74-86) */
+int tnad_init_raw(tnad_ctx* ctx, const double* a, int D, int chi, double* C, double* T);
+
+/* svd(cpmat) on the symmetric n x n matrix A (src/ctmrg.jl:136): the k eigenpairs of largest |lambda|,
+ * sorted by |lambda| descending (n <= 192: all n, exactly).  evals: k, evecs: n x k. Returns k via *k_out. */
+int tnad_symeig(tnad_ctx* ctx, const double* A, int n, int want, int k, double* evals, double* evecs,
+                int* k_out);
+
+/* svd_back (src/trg.jl:72-106) for the CTMRG use: A symmetric n x n, cotangent G (n x chi) of the first chi
+ * left singular vectors, dS = dV = nothing; returns Abar + Abar^T (the cotangent of cpmat before
+ * cpmat += cpmat', src/ctmrg.jl:135). */
+int tnad_svd_back_sym(tnad_ctx* ctx, const double* A, int n, int chi, int k, const double* G, double* Abar);
+
+/* ctmrgstep (src/ctmrg.jl:126-153): one step; vals has chi*D entries (uncomputed tail = 0). */
+int tnad_ctmrgstep(tnad_ctx* ctx, const double* a, int D, int chi, const double* C, const double* T,
+                   const tnad_opts* opts, double* Cout, double* Tout, double* vals);
+
+/* ctmrg (src/ctmrg.jl:110-117 with fixedpoint/StopFunction, src/fixedpoint.jl:11-41).
+ * C0/T0: initial environment (e.g. from tnad_init_raw).  If tape != NULL a checkpoint of every step is
+ * kept on the device for tnad_ctmrg_back; free it with tnad_tape_free. */
+int tnad_ctmrg(tnad_ctx* ctx, const double* a, int D, int chi, const double* C0, const double* T0, double tol,
+               int maxit, const tnad_opts* opts, double* C, double* T, double* vals, int* iters,
+               tnad_tape** tape);
+
+/* reverse sweep of tnad_ctmrg (what Zygote computes through the unrolled fixedpoint loop):
+ * cotangents of the final C, T -> cotangent of a.  The initial environment gets no gradient
+ * (@nograd _initializect_square, src/autodiff.jl:5). */
+int tnad_ctmrg_back(tnad_ctx* ctx, tnad_tape* tape, const double* Cbar, const double* Tbar, double* abar);
+int tnad_tape_free(tnad_tape* tape);
+
+/* expectationvalue (src/variationalipeps.jl:49-56) */
+int tnad_expectationvalue(tnad_ctx* ctx, const double* h, int s, const double* ap, int D, int chi,
+                          const double* C, const double* T, double* e);
+
+/* energy (src/variationalipeps.jl:28-40) and, if grad != NULL, its gradient with respect to the
+ * (unsymmetrised) bulk -- one boundary crossing per objective + gradient evaluation. */
+int tnad_energy(tnad_ctx* ctx, const double* h, const double* bulk, int d, int s, int chi, double tol,
+                int maxit, const tnad_opts* opts, double* e, double* grad, int* iters);
+
+/* trg (src/trg.jl:13-30): ln Z per site of the rank-4 tensor a (q x q x q x q); if abar != NULL also
+ * d lnZ / d a (q^4 entries). */
+int tnad_trg(tnad_ctx* ctx, const double* a, int q, int chi, int niter, double tol, double* lnz,
+             double* abar);
+
+/* trg_svd (src/trg.jl:33-44) on a (d1,d2,d3,d4) tensor: u (d1,d2,k), v (k,d3,d4); *k_out = kept count.
+ * u and v must hold min(dmax, d1*d2, d3*d4) columns/rows. */
+int tnad_trg_svd(tnad_ctx* ctx, const double* t, int d1, int d2, int d3, int d4, int dmax, double tol,
+                 double* u, double* v, int* k_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* TNAD_H */

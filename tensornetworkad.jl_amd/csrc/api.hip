@@ -1,0 +1,529 @@
+// extern "C" entry points of libtnad (include/tnad.h): context and memory
+// management, the CTMRG fixed point with the reference's StopFunction
+// semantics, checkpoint tapes for the reverse sweep, and the fused
+// energy + gradient call.
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <memory>
+
+#include "ctmrg.h"
+#include "trg.h"
+
+namespace tnad {
+
+// ---------------------------------------------------------------------------
+// memory: a size-keyed cache of device blocks per context
+// ---------------------------------------------------------------------------
+void* Ctx::alloc(size_t bytes) {
+  if (bytes == 0) bytes = 8;
+  bytes = (bytes + 255) & ~size_t(255);
+  auto it = free_blocks.find(bytes);
+  if (it != free_blocks.end()) {
+    void* p = it->second;
+    free_blocks.erase(it);
+    cached_bytes -= bytes;
+    return p;
+  }
+  void* p = nullptr;
+  hipError_t e = hipMalloc(&p, bytes);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    trim();
+    e = hipMalloc(&p, bytes);
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      throw Error(TNAD_ERR_ALLOC, "hipMalloc of " + std::to_string(bytes) + " bytes failed");
+    }
+  }
+  return p;
+}
+void Ctx::release(void* p, size_t bytes) {
+  if (!p) return;
+  if (bytes == 0) bytes = 8;
+  bytes = (bytes + 255) & ~size_t(255);
+  free_blocks.emplace(bytes, p);
+  cached_bytes += bytes;
+  if (cached_bytes > (size_t(32) << 30)) trim();
+}
+void Ctx::trim() {
+  // blocks may still be referenced by queued work
+  (void)hipStreamSynchronize(stream);
+  for (auto& kv : free_blocks) (void)hipFree(kv.second);
+  free_blocks.clear();
+  cached_bytes = 0;
+}
+Ctx::~Ctx() {
+  if (stream) (void)hipStreamSynchronize(stream);
+  for (auto& kv : free_blocks) (void)hipFree(kv.second);
+  if (stream) (void)hipStreamDestroy(stream);
+}
+
+DevBuf::DevBuf(Ctx* c, size_t b) : ptr(c->alloc(b)), bytes(b), owner(c) {}
+DevBuf& DevBuf::operator=(DevBuf&& o) noexcept {
+  if (this != &o) {
+    if (ptr && owner) owner->release(ptr, bytes);
+    ptr = o.ptr; bytes = o.bytes; owner = o.owner;
+    o.ptr = nullptr; o.bytes = 0; o.owner = nullptr;
+  }
+  return *this;
+}
+DevBuf::~DevBuf() {
+  // blocks return to the cache; work queued on the context's single stream
+  // that still reads them is ordered before any later reuse.
+  if (ptr && owner) owner->release(ptr, bytes);
+}
+
+void copy_in(Ctx* c, void* dst_dev, const void* src_any, size_t bytes) {
+  if (!src_any) throw Error(TNAD_ERR_ARG, "null input pointer");
+  TNAD_HIP_TRY(hipMemcpyAsync(dst_dev, src_any, bytes, hipMemcpyDefault, c->stream));
+}
+void copy_out(Ctx* c, void* dst_any, const void* src_dev, size_t bytes) {
+  if (!dst_any) throw Error(TNAD_ERR_ARG, "null output pointer");
+  TNAD_HIP_TRY(hipMemcpyAsync(dst_any, src_dev, bytes, hipMemcpyDefault, c->stream));
+}
+
+}  // namespace tnad
+
+using namespace tnad;
+
+struct tnad_ctx {
+  Ctx c;
+  tnad_stats stats{};
+};
+
+struct tnad_tape {
+  tnad_ctx* ctx = nullptr;
+  StepDims d;
+  std::vector<std::unique_ptr<StepTape>> steps;
+  DevBuf a64;
+  PBuf ap[3];          // a at each contraction precision (lazily filled)
+  bool have[3] = {false, false, false};
+  tnad_opts opts{};
+};
+
+namespace {
+
+template <typename F>
+int guarded(tnad_ctx* ctx, F&& f) {
+  if (!ctx) return TNAD_ERR_ARG;
+  try {
+    TNAD_HIP_TRY(hipSetDevice(ctx->c.device));
+    f();
+    ctx->c.sync();
+    ctx->c.last_error.clear();
+    return TNAD_OK;
+  } catch (const Error& e) {
+    ctx->c.last_error = e.msg;
+    (void)hipGetLastError();
+    return e.code;
+  } catch (const std::exception& e) {
+    ctx->c.last_error = e.what();
+    return TNAD_ERR_ARG;
+  }
+}
+
+Prec to_prec_enum(int p) {
+  if (p == TNAD_PREC_FP64) return Prec::F64;
+  if (p == TNAD_PREC_FP32) return Prec::F32;
+  if (p == TNAD_PREC_BF16) return Prec::BF16;
+  throw Error(TNAD_ERR_ARG, "unknown precision");
+}
+
+void check_dims(int D, int chi) {
+  if (D < 1 || chi < 1) throw Error(TNAD_ERR_SHAPE, "bond dimensions must be positive");
+  if ((long long)D * chi > 8192) throw Error(TNAD_ERR_UNSUPPORTED, "chi*D above 8192 is not supported");
+}
+
+const PBuf& ap_at(Ctx* c, tnad_tape& t, Prec p) {
+  int i = (int)p;
+  if (!t.have[i]) {
+    size_t nA = (size_t)t.d.D * t.d.D * t.d.D * t.d.D;
+    t.ap[i] = PBuf(c, p, nA);
+    to_prec(c, t.a64.as<double>(), t.ap[i]);
+    t.have[i] = true;
+  }
+  return t.ap[i];
+}
+
+// The fixed point (src/ctmrg.jl:110-117, src/fixedpoint.jl:11-41) on device buffers.
+// `t` always carries a and its precision copies; its `steps` are filled only when keep_steps.
+void run_ctmrg(tnad_ctx* ctx, tnad_tape& t, const double* C0, const double* T0, double tol, int maxit,
+               bool keep_steps, double* C_out, double* T_out, std::vector<double>& vals_out, int& iters) {
+  Ctx* c = &ctx->c;
+  const tnad_opts& o = t.opts;
+  const StepDims d = t.d;
+  const int chi = d.chi, D = d.D, n = d.n;
+  const size_t nC = (size_t)chi * chi, nT = (size_t)chi * D * chi;
+  DevBuf Ca(c, nC * 8), Ta(c, nT * 8), Cb(c, nC * 8), Tb(c, nT * 8);
+  k_copy(C0, Ca.as<double>(), nC, c->stream);
+  k_copy(T0, Ta.as<double>(), nT, c->stream);
+  const Prec low = to_prec_enum(o.precision);
+  const bool lowp = low != Prec::F64;
+  bool switched = !lowp;
+  StepOpts so;
+  so.eig_block = o.eig_block;
+  so.eig_max_iters = o.eig_max_iters > 0 ? o.eig_max_iters : 60;
+  so.eig_warm_iters = o.eig_warm_iters > 0 ? o.eig_warm_iters : 3;
+  so.eig_tol = o.eig_tol > 0 ? o.eig_tol : 1e-13;
+  const int k = step_k(d, so);
+  DevBuf Xprev;            // warm start block when no tape is kept
+  const double* X0 = nullptr;
+  std::vector<double> oldvals(n, std::numeric_limits<double>::infinity());
+  std::vector<double> vals = oldvals;
+  int counter = 0;
+  iters = 0;
+  tnad_stats& st = ctx->stats;
+  st = tnad_stats{};
+  st.eig_k = k;
+  while (true) {
+    // StopFunction (src/fixedpoint.jl:31-41), evaluated on every state including the first
+    counter += 1;
+    if (counter > maxit) break;
+    double diff2 = 0.0;
+    for (int i = 0; i < n; ++i) {
+      double dv = vals[i] - oldvals[i];
+      diff2 += dv * dv;
+    }
+    double diff = std::sqrt(diff2);   // NaN for the first state (Inf - Inf), Inf for the second
+    if (diff <= tol) break;
+    if (!switched && diff <= o.switch_tol) switched = true;
+    oldvals = vals;
+    so.prec = switched ? Prec::F64 : low;
+    const PBuf& ap = ap_at(c, t, so.prec);
+    StepTape* tp = nullptr;
+    if (keep_steps) {
+      t.steps.emplace_back(new StepTape(c, d, so.prec, k));
+      tp = t.steps.back().get();
+    }
+    StepState ss;
+    double* X_keep = nullptr;
+    const bool warm = o.eig_warm && n > JACOBI_MAX;
+    if (warm && !tp) {
+      if (!Xprev.ptr) Xprev = DevBuf(c, (size_t)n * k * 8);
+      X_keep = Xprev.as<double>();
+    }
+    ctmrg_step(c, d, so, t.a64.as<double>(), ap, Ca.as<double>(), Ta.as<double>(), warm ? X0 : nullptr, X_keep,
+               Cb.as<double>(), Tb.as<double>(), vals, tp, ss);
+    if (warm) X0 = tp ? tp->X.as<double>() : Xprev.as<double>();
+    std::swap(Ca, Cb);
+    std::swap(Ta, Tb);
+    iters += 1;
+    st.iters = iters;
+    if (so.prec != Prec::F64) st.low_prec_iters += 1;
+    st.eig_iters_total += ss.eig_iters;
+    st.eig_resid_max = std::max(st.eig_resid_max, ss.eig_resid);
+  }
+  k_copy(Ca.as<double>(), C_out, nC, c->stream);
+  k_copy(Ta.as<double>(), T_out, nT, c->stream);
+  vals_out = vals;
+}
+
+}  // namespace
+
+extern "C" {
+
+int tnad_version(void) { return TNAD_ABI_VERSION; }
+
+void tnad_opts_default(tnad_opts* o) {
+  if (!o) return;
+  o->precision = TNAD_PREC_FP64;
+  o->switch_tol = 0.0;
+  o->bwd_precision = TNAD_PREC_FP64;
+  o->eig_block = 0;
+  o->eig_max_iters = 60;
+  o->eig_warm = 0;
+  o->eig_warm_iters = 3;
+  o->eig_tol = 1e-13;
+  o->bwd_last = 0;
+}
+
+int tnad_create(int device, tnad_ctx** out) {
+  if (!out) return TNAD_ERR_ARG;
+  *out = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || device < 0 || device >= n) return TNAD_ERR_HIP;
+  if (hipSetDevice(device) != hipSuccess) return TNAD_ERR_HIP;
+  tnad_ctx* ctx = new tnad_ctx();
+  ctx->c.device = device;
+  if (hipStreamCreateWithFlags(&ctx->c.stream, hipStreamNonBlocking) != hipSuccess) {
+    delete ctx;
+    return TNAD_ERR_HIP;
+  }
+  *out = ctx;
+  return TNAD_OK;
+}
+
+int tnad_destroy(tnad_ctx* ctx) {
+  if (!ctx) return TNAD_ERR_ARG;
+  (void)hipSetDevice(ctx->c.device);
+  delete ctx;
+  return TNAD_OK;
+}
+
+const char* tnad_last_error(const tnad_ctx* ctx) { return ctx ? ctx->c.last_error.c_str() : "null context"; }
+
+int tnad_get_stats(const tnad_ctx* ctx, tnad_stats* out) {
+  if (!ctx || !out) return TNAD_ERR_ARG;
+  *out = ctx->stats;
+  return TNAD_OK;
+}
+
+int tnad_indexperm_symmetrize(tnad_ctx* ctx, const double* bulk, int d, int s, double* out) {
+  return guarded(ctx, [&] {
+    if (d < 1 || s < 1) throw Error(TNAD_ERR_SHAPE, "size of tensor error, should be (d, d, d, d, s)");
+    Ctx* c = &ctx->c;
+    size_t N = (size_t)d * d * d * d * s;
+    DevBuf x(c, N * 8), y(c, N * 8);
+    copy_in(c, x.ptr, bulk, N * 8);
+    k_symmetrize_bulk(x.as<double>(), d, s, y.as<double>(), c->stream);
+    copy_out(c, out, y.ptr, N * 8);
+  });
+}
+
+int tnad_double_layer(tnad_ctx* ctx, const double* bulk, int d, int s, double* ap, double* a) {
+  return guarded(ctx, [&] {
+    if (d < 1 || s < 1) throw Error(TNAD_ERR_SHAPE, "size of tensor error, should be (d, d, d, d, s)");
+    Ctx* c = &ctx->c;
+    size_t N = (size_t)d * d * d * d * s, D4 = (size_t)d * d * d * d * d * d * d * d;
+    DevBuf x(c, N * 8), bap(c, D4 * s * s * 8), ba(c, D4 * 8);
+    copy_in(c, x.ptr, bulk, N * 8);
+    k_double_layer(x.as<double>(), d, s, ap ? bap.as<double>() : nullptr, ba.as<double>(), c->stream);
+    if (ap) copy_out(c, ap, bap.ptr, D4 * s * s * 8);
+    if (a) copy_out(c, a, ba.ptr, D4 * 8);
+  });
+}
+
+int tnad_init_raw(tnad_ctx* ctx, const double* a, int D, int chi, double* C, double* T) {
+  return guarded(ctx, [&] {
+    check_dims(D, chi);
+    Ctx* c = &ctx->c;
+    size_t nA = (size_t)D * D * D * D, nC = (size_t)chi * chi, nT = (size_t)chi * D * chi;
+    DevBuf ba(c, nA * 8), bC(c, nC * 8), bT(c, nT * 8);
+    copy_in(c, ba.ptr, a, nA * 8);
+    k_init_raw(ba.as<double>(), D, chi, bC.as<double>(), bT.as<double>(), c->stream);
+    copy_out(c, C, bC.ptr, nC * 8);
+    copy_out(c, T, bT.ptr, nT * 8);
+  });
+}
+
+int tnad_symeig(tnad_ctx* ctx, const double* A, int n, int want, int k, double* evals, double* evecs, int* k_out) {
+  return guarded(ctx, [&] {
+    if (n < 1 || want < 1 || want > n) throw Error(TNAD_ERR_ARG, "symeig: bad sizes");
+    Ctx* c = &ctx->c;
+    int kk = (n <= JACOBI_MAX) ? n : std::min(std::max(k, want), std::min(n, JACOBI_MAX));
+    DevBuf bA(c, (size_t)n * n * 8), X(c, (size_t)n * kk * 8), AX(c, (size_t)n * kk * 8), th(c, (size_t)kk * 8);
+    copy_in(c, bA.ptr, A, (size_t)n * n * 8);
+    int kr = topk_eig(c, bA.as<double>(), n, want, kk, nullptr, 200, 1e-13, X.as<double>(), AX.as<double>(),
+                      th.as<double>(), nullptr);
+    if (evals) copy_out(c, evals, th.ptr, (size_t)kr * 8);
+    if (evecs) copy_out(c, evecs, X.ptr, (size_t)n * kr * 8);
+    if (k_out) *k_out = kr;
+  });
+}
+
+int tnad_svd_back_sym(tnad_ctx* ctx, const double* A, int n, int chi, int k, const double* G, double* Abar) {
+  return guarded(ctx, [&] {
+    if (n < 1 || chi < 1 || chi > n) throw Error(TNAD_ERR_ARG, "svd_back: bad sizes");
+    Ctx* c = &ctx->c;
+    StepDims d;
+    d.chi = chi; d.D = n / chi; d.n = n;
+    int kk = (n <= JACOBI_MAX) ? n : std::min(std::max(k, chi), std::min(n, JACOBI_MAX));
+    StepTape tp(c, d, Prec::F64, kk);
+    copy_in(c, tp.M.ptr, A, (size_t)n * n * 8);
+    tp.k = topk_eig(c, tp.M.as<double>(), n, chi, kk, nullptr, 200, 1e-13, tp.X.as<double>(), tp.AX.as<double>(),
+                    tp.theta.as<double>(), nullptr);
+    DevBuf bG(c, (size_t)n * chi * 8), zero(c, (size_t)chi * chi * 8), out(c, (size_t)n * n * 8);
+    copy_in(c, bG.ptr, G, (size_t)n * chi * 8);
+    k_fill(zero.as<double>(), (size_t)chi * chi, 0.0, c->stream);
+    eig_adjoint(c, d, tp, bG.as<double>(), zero.as<double>(), out.as<double>(), nullptr);
+    copy_out(c, Abar, out.ptr, (size_t)n * n * 8);
+  });
+}
+
+int tnad_ctmrgstep(tnad_ctx* ctx, const double* a, int D, int chi, const double* C, const double* T,
+                   const tnad_opts* opts, double* Cout, double* Tout, double* vals) {
+  return guarded(ctx, [&] {
+    check_dims(D, chi);
+    Ctx* c = &ctx->c;
+    tnad_opts o;
+    tnad_opts_default(&o);
+    if (opts) o = *opts;
+    StepDims d;
+    d.chi = chi; d.D = D; d.n = chi * D;
+    size_t nA = (size_t)D * D * D * D, nC = (size_t)chi * chi, nT = (size_t)chi * D * chi;
+    DevBuf ba(c, nA * 8), bC(c, nC * 8), bT(c, nT * 8), oC(c, nC * 8), oT(c, nT * 8);
+    copy_in(c, ba.ptr, a, nA * 8);
+    copy_in(c, bC.ptr, C, nC * 8);
+    copy_in(c, bT.ptr, T, nT * 8);
+    StepOpts so;
+    so.prec = to_prec_enum(o.precision);
+    so.eig_block = o.eig_block;
+    so.eig_max_iters = o.eig_max_iters > 0 ? o.eig_max_iters : 60;
+    so.eig_tol = o.eig_tol > 0 ? o.eig_tol : 1e-13;
+    PBuf ap(c, so.prec, nA);
+    to_prec(c, ba.as<double>(), ap);
+    std::vector<double> v;
+    StepState ss;
+    ctmrg_step(c, d, so, ba.as<double>(), ap, bC.as<double>(), bT.as<double>(), nullptr, nullptr,
+               oC.as<double>(), oT.as<double>(), v, nullptr, ss);
+    copy_out(c, Cout, oC.ptr, nC * 8);
+    copy_out(c, Tout, oT.ptr, nT * 8);
+    if (vals) std::memcpy(vals, v.data(), v.size() * 8);
+  });
+}
+
+int tnad_ctmrg(tnad_ctx* ctx, const double* a, int D, int chi, const double* C0, const double* T0, double tol,
+               int maxit, const tnad_opts* opts, double* C, double* T, double* vals, int* iters,
+               tnad_tape** tape_out) {
+  std::unique_ptr<tnad_tape> t(new tnad_tape());
+  int rc = guarded(ctx, [&] {
+    check_dims(D, chi);
+    if (maxit < 0) throw Error(TNAD_ERR_ARG, "maxit must be >= 0");
+    Ctx* c = &ctx->c;
+    t->ctx = ctx;
+    tnad_opts_default(&t->opts);
+    if (opts) t->opts = *opts;
+    t->d.chi = chi; t->d.D = D; t->d.n = chi * D;
+    size_t nA = (size_t)D * D * D * D, nC = (size_t)chi * chi, nT = (size_t)chi * D * chi;
+    t->a64 = DevBuf(c, nA * 8);
+    copy_in(c, t->a64.ptr, a, nA * 8);
+    DevBuf bC0(c, nC * 8), bT0(c, nT * 8), oC(c, nC * 8), oT(c, nT * 8);
+    copy_in(c, bC0.ptr, C0, nC * 8);
+    copy_in(c, bT0.ptr, T0, nT * 8);
+    std::vector<double> v;
+    int it = 0;
+    run_ctmrg(ctx, *t, bC0.as<double>(), bT0.as<double>(), tol, maxit, tape_out != nullptr, oC.as<double>(),
+              oT.as<double>(), v, it);
+    copy_out(c, C, oC.ptr, nC * 8);
+    copy_out(c, T, oT.ptr, nT * 8);
+    c->sync();
+    if (vals) std::memcpy(vals, v.data(), v.size() * 8);
+    if (iters) *iters = it;
+  });
+  if (rc == TNAD_OK && tape_out) *tape_out = t.release();
+  else if (tape_out) *tape_out = nullptr;
+  return rc;
+}
+
+static void ctmrg_back_impl(tnad_ctx* ctx, tnad_tape* t, const double* Cbar_dev, const double* Tbar_dev,
+                            double* abar_dev) {
+  Ctx* c = &ctx->c;
+  const StepDims d = t->d;
+  const int chi = d.chi, D = d.D;
+  size_t nA = (size_t)D * D * D * D, nC = (size_t)chi * chi, nT = (size_t)chi * D * chi;
+  DevBuf Cb(c, nC * 8), Tb(c, nT * 8), Cb2(c, nC * 8), Tb2(c, nT * 8);
+  k_copy(Cbar_dev, Cb.as<double>(), nC, c->stream);
+  k_copy(Tbar_dev, Tb.as<double>(), nT, c->stream);
+  k_fill(abar_dev, nA, 0.0, c->stream);
+  const int ns = (int)t->steps.size();
+  const int first = (t->opts.bwd_last > 0 && t->opts.bwd_last < ns) ? ns - t->opts.bwd_last : 0;
+  const Prec bp = to_prec_enum(t->opts.bwd_precision);
+  for (int i = ns - 1; i >= first; --i) {
+    const PBuf& ap = ap_at(c, *t, bp);
+    ctmrg_step_back(c, d, bp, *t->steps[i], t->a64.as<double>(), ap, Cb.as<double>(), Tb.as<double>(),
+                    Cb2.as<double>(), Tb2.as<double>(), abar_dev);
+    std::swap(Cb, Cb2);
+    std::swap(Tb, Tb2);
+  }
+  ctx->stats.bwd_steps = ns - first;
+}
+
+int tnad_ctmrg_back(tnad_ctx* ctx, tnad_tape* t, const double* Cbar, const double* Tbar, double* abar) {
+  return guarded(ctx, [&] {
+    if (!t || t->ctx != ctx) throw Error(TNAD_ERR_ARG, "tape does not belong to this context");
+    Ctx* c = &ctx->c;
+    const StepDims d = t->d;
+    size_t nA = (size_t)d.D * d.D * d.D * d.D, nC = (size_t)d.chi * d.chi, nT = (size_t)d.chi * d.D * d.chi;
+    DevBuf bC(c, nC * 8), bT(c, nT * 8), out(c, nA * 8);
+    copy_in(c, bC.ptr, Cbar, nC * 8);
+    copy_in(c, bT.ptr, Tbar, nT * 8);
+    ctmrg_back_impl(ctx, t, bC.as<double>(), bT.as<double>(), out.as<double>());
+    copy_out(c, abar, out.ptr, nA * 8);
+  });
+}
+
+int tnad_tape_free(tnad_tape* t) {
+  if (!t) return TNAD_ERR_ARG;
+  if (t->ctx) (void)hipSetDevice(t->ctx->c.device);
+  delete t;
+  return TNAD_OK;
+}
+
+int tnad_expectationvalue(tnad_ctx* ctx, const double* h, int s, const double* ap, int D, int chi, const double* C,
+                          const double* T, double* e) {
+  return guarded(ctx, [&] {
+    check_dims(D, chi);
+    if (s < 1) throw Error(TNAD_ERR_SHAPE, "physical dimension must be positive");
+    Ctx* c = &ctx->c;
+    size_t nap = (size_t)D * D * D * D * s * s, nC = (size_t)chi * chi, nT = (size_t)chi * D * chi;
+    DevBuf bh(c, (size_t)s * s * s * s * 8), bap(c, nap * 8), bC(c, nC * 8), bT(c, nT * 8);
+    copy_in(c, bh.ptr, h, (size_t)s * s * s * s * 8);
+    copy_in(c, bap.ptr, ap, nap * 8);
+    copy_in(c, bC.ptr, C, nC * 8);
+    copy_in(c, bT.ptr, T, nT * 8);
+    EvCache ev(c, chi, D, s);
+    expectation_fwd(c, chi, D, s, bh.as<double>(), bap.as<double>(), bC.as<double>(), bT.as<double>(), ev, e);
+  });
+}
+
+int tnad_energy(tnad_ctx* ctx, const double* h, const double* bulk, int d, int s, int chi, double tol, int maxit,
+                const tnad_opts* opts, double* e, double* grad, int* iters) {
+  return guarded(ctx, [&] {
+    if (d < 1 || s < 1) throw Error(TNAD_ERR_SHAPE, "size of tensor error, should be (d, d, d, d, s)");
+    const int D = d * d;
+    check_dims(D, chi);
+    if (maxit < 0) throw Error(TNAD_ERR_ARG, "maxit must be >= 0");
+    Ctx* c = &ctx->c;
+    hipStream_t st = c->stream;
+    const size_t N = (size_t)d * d * d * d * s, nA = (size_t)D * D * D * D, nap = nA * s * s;
+    const size_t nC = (size_t)chi * chi, nT = (size_t)chi * D * chi;
+    DevBuf bh(c, (size_t)s * s * s * s * 8), bx(c, N * 8), bs(c, N * 8), bap(c, nap * 8);
+    copy_in(c, bh.ptr, h, (size_t)s * s * s * s * 8);
+    copy_in(c, bx.ptr, bulk, N * 8);
+    // ipeps = indexperm_symmetrize(ipeps); ap, a = double layer    (src/variationalipeps.jl:29-34)
+    k_symmetrize_bulk(bx.as<double>(), d, s, bs.as<double>(), st);
+    tnad_tape t;
+    t.ctx = ctx;
+    tnad_opts_default(&t.opts);
+    if (opts) t.opts = *opts;
+    t.d.chi = chi; t.d.D = D; t.d.n = chi * D;
+    t.a64 = DevBuf(c, nA * 8);
+    k_double_layer(bs.as<double>(), d, s, bap.as<double>(), t.a64.as<double>(), st);
+    // rt = SquareCTMRGRuntime(a, Val(:raw), chi); rt = ctmrg(rt; tol, maxit)   (src/variationalipeps.jl:36-37)
+    DevBuf C0(c, nC * 8), T0(c, nT * 8), C(c, nC * 8), T(c, nT * 8);
+    k_init_raw(t.a64.as<double>(), D, chi, C0.as<double>(), T0.as<double>(), st);
+    std::vector<double> v;
+    int it = 0;
+    run_ctmrg(ctx, t, C0.as<double>(), T0.as<double>(), tol, maxit, grad != nullptr, C.as<double>(), T.as<double>(),
+              v, it);
+    // e = expectationvalue(h, ap, rt)                                       (src/variationalipeps.jl:38)
+    EvCache ev(c, chi, D, s);
+    double eh = 0.0;
+    expectation_fwd(c, chi, D, s, bh.as<double>(), bap.as<double>(), C.as<double>(), T.as<double>(), ev, &eh);
+    if (e) *e = eh;
+    if (iters) *iters = it;
+    if (grad) {
+      DevBuf apbar(c, nap * 8), Cbar(c, nC * 8), Tbar(c, nT * 8), abar(c, nA * 8), bsbar(c, N * 8), g(c, N * 8);
+      k_fill(Cbar.as<double>(), nC, 0.0, st);
+      k_fill(Tbar.as<double>(), nT, 0.0, st);
+      expectation_back(c, chi, D, s, bh.as<double>(), bap.as<double>(), C.as<double>(), T.as<double>(), ev,
+                       apbar.as<double>(), Cbar.as<double>(), Tbar.as<double>());
+      ctmrg_back_impl(ctx, &t, Cbar.as<double>(), Tbar.as<double>(), abar.as<double>());
+      k_double_layer_back(bs.as<double>(), d, s, apbar.as<double>(), abar.as<double>(), bsbar.as<double>(), st);
+      k_symmetrize_bulk_back2(bsbar.as<double>(), bx.as<double>(), d, s, g.as<double>(), st);
+      copy_out(c, grad, g.ptr, N * 8);
+    }
+  });
+}
+
+int tnad_trg(tnad_ctx* ctx, const double* a, int q, int chi, int niter, double tol, double* lnz, double* abar) {
+  return guarded(ctx, [&] { trg_run(&ctx->c, a, q, chi, niter, tol, lnz, abar); });
+}
+
+int tnad_trg_svd(tnad_ctx* ctx, const double* t, int d1, int d2, int d3, int d4, int dmax, double tol, double* u,
+                 double* v, int* k_out) {
+  return guarded(ctx, [&] { trg_svd_run(&ctx->c, t, d1, d2, d3, d4, dmax, tol, u, v, k_out); });
+}
+
+}  // extern "C"

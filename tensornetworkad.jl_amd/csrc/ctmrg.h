@@ -1,0 +1,109 @@
+// Host-side orchestration types of the CTMRG step, its reverse sweep and the
+// energy chain (ctmrg.hip).
+#pragma once
+#include <vector>
+
+#include "tc_build.h"
+#include "tnad_internal.h"
+
+namespace tnad {
+
+// buffer at a contraction precision (double for F64, float for F32 / BF16)
+struct PBuf {
+  Prec p = Prec::F64;
+  size_t n = 0;
+  DevBuf b;
+  PBuf() = default;
+  PBuf(Ctx* c, Prec p_, size_t n_);
+  PBuf(PBuf&&) = default;
+  PBuf& operator=(PBuf&&) = default;
+  void* ptr() const { return b.ptr; }
+};
+void to_prec(Ctx* c, const double* src, PBuf& dst);
+void from_prec(Ctx* c, const PBuf& src, double* dst);
+
+// one contraction launch from einsum-style views
+void tc(Ctx* c, Prec p, const TView& A, const void* a, const TView& B, const void* b, const TView& C, void* o,
+        double alpha = 1.0, double beta = 0.0);
+
+struct StepDims {
+  int chi = 0, D = 0, n = 0;
+};
+
+struct StepOpts {
+  Prec prec = Prec::F64;      // contraction precision of this step
+  int eig_block = 0;          // subspace block size k (n > 192); 0 = automatic
+  int eig_max_iters = 60;     // cold start
+  int eig_warm_iters = 3;     // warm start (previous step's Ritz vectors)
+  double eig_tol = 1e-13;     // relative Ritz residual of the kept vectors
+};
+
+struct StepState {
+  int k = 0;
+  int eig_iters = 0;
+  double eig_resid = 0.0;
+};
+
+struct EigStats {
+  int iters = 0, k = 0;
+  double resid = 0.0;
+};
+struct EigBackStats {
+  int neumann_terms = 0;
+};
+
+struct EigWork {
+  DevBuf Y, Xr, Yr, H, W, Vw, G, Rinv, nrm, res, info, sweeps;
+  EigWork(Ctx* c, int n, int k);
+};
+
+// Top-k eigenpairs (by |lambda|) of the symmetric n x n FP64 matrix A.
+// n <= 192: full Jacobi (k = n).  Otherwise block subspace iteration with
+// Rayleigh-Ritz and sCholQR3 orthonormalisation, optionally warm-started from X0.
+// Outputs X (n x k), AX = A X, theta (k).  Returns k.
+int topk_eig(Ctx* c, const double* A, int n, int want, int k, const double* X0, int maxit, double tol,
+             double* X, double* AX, double* theta, EigStats* st);
+
+inline int step_k(const StepDims& d, const StepOpts& o) {
+  if (d.n <= JACOBI_MAX) return d.n;
+  int k = o.eig_block > 0 ? o.eig_block : d.chi + d.chi / 2;
+  if (k < d.chi) k = d.chi;
+  if (k > JACOBI_MAX) k = JACOBI_MAX;
+  if (k > d.n) k = d.n;
+  return k;
+}
+
+// checkpoint of one step for the reverse sweep
+struct StepTape {
+  Prec prec;
+  int k;
+  DevBuf C, T;          // step inputs (FP64)
+  DevBuf X, AX, theta;  // eigenpairs (FP64): X = [z | extra], AX = A X
+  DevBuf M;             // cpmat + cpmat^T (FP64)
+  PBuf X2, Y2;          // contraction intermediates reused by the adjoint
+  DevBuf C2, T2, nrm;   // symmetrised, unnormalised outputs and their norms
+  StepTape(Ctx* c, const StepDims& d, Prec p, int k);
+};
+
+// X0: warm-start block (n x k) or null.  X_keep: if non-null, receives this step's
+// Ritz vectors (n x k) for the next step's warm start.
+void ctmrg_step(Ctx* c, const StepDims& d, const StepOpts& o, const double* a64, const PBuf& ap,
+                const double* C, const double* T, const double* X0, double* X_keep, double* Cout, double* Tout,
+                std::vector<double>& vals, StepTape* tape, StepState& state);
+
+void eig_adjoint(Ctx* c, const StepDims& d, const StepTape& tp, const double* G, const double* C1bar,
+                 double* cpbar, EigBackStats* st);
+
+void ctmrg_step_back(Ctx* c, const StepDims& d, Prec p, const StepTape& tp, const double* a64, const PBuf& ap,
+                     const double* C3bar, const double* T3bar, double* Cbar, double* Tbar, double* abar);
+
+struct EvCache {
+  DevBuf apn, E1, E2, E3, E4, l, tr, scal;
+  EvCache(Ctx* c, int chi, int D, int ph);
+};
+void expectation_fwd(Ctx* c, int chi, int D, int ph, const double* h, const double* ap, const double* C,
+                     const double* T, EvCache& ev, double* e_host);
+void expectation_back(Ctx* c, int chi, int D, int ph, const double* h, const double* ap, const double* C,
+                      const double* T, EvCache& ev, double* apbar, double* Cbar, double* Tbar);
+
+}  // namespace tnad

@@ -1,0 +1,720 @@
+// CTMRG renormalisation step, its reverse sweep, the top-k FP64 eigensolver
+// and the energy chain, orchestrated on one HIP stream.
+//
+// Reference: src/ctmrg.jl:110-153 (ctmrg, ctmrgstep), src/fixedpoint.jl:11-41
+// (fixedpoint, StopFunction), src/trg.jl:72-106 (svd_back), src/autodiff.jl:23-29
+// (norm adjoint), src/variationalipeps.jl:28-56 (energy, expectationvalue).
+//
+// Notation: chi = environment bond, D = double-layer bond (Dd = d^2 in the
+// reference), n = chi*D.  Every tensor is column-major with the reference's
+// index order; contractions are issued as the reference's einsum strings
+// through contract_desc (tc_build.cpp).
+//
+// The n x n "corner matrix" cpmat + cpmat^T is symmetric, so its SVD is
+// obtained from an eigendecomposition: s = |lambda| sorted descending,
+// U = eigenvectors, V = U sign(lambda).
+#include "ctmrg.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+namespace tnad {
+
+static inline size_t esize(Prec p) { return p == Prec::F64 ? 8 : 4; }
+
+// --------------------------------------------------------------------------
+// small kernels private to this file
+// --------------------------------------------------------------------------
+namespace {
+
+constexpr int NT = 256;
+
+__global__ void resid_norms_kernel(const double* __restrict__ X, const double* __restrict__ Y,
+                                   const double* __restrict__ theta, int n, double* __restrict__ out) {
+  __shared__ double red[4];
+  const int j = blockIdx.x;
+  const double th = theta[j];
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    double r = Y[i + (long long)j * n] - th * X[i + (long long)j * n];
+    acc += r * r;
+  }
+  for (int o = 32; o > 0; o >>= 1) acc += __shfl_down(acc, o);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) out[j] = sqrt(red[0] + red[1] + red[2] + red[3]);
+}
+
+// make the largest-|.| entry of every column positive (first occurrence wins);
+// the same sign is applied to the companion block Y (A*X) when given.
+__global__ void canon_sign_kernel(double* __restrict__ X, double* __restrict__ Y, int n) {
+  __shared__ double bv[4];
+  __shared__ int bi[4];
+  const int j = blockIdx.x;
+  double best = -1.0;
+  int idx = 0x7fffffff;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    double v = fabs(X[i + (long long)j * n]);
+    if (v > best || (v == best && i < idx)) { best = v; idx = i; }
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    double ob = __shfl_down(best, o);
+    int oi = __shfl_down(idx, o);
+    if (ob > best || (ob == best && oi < idx)) { best = ob; idx = oi; }
+  }
+  if ((threadIdx.x & 63) == 0) { bv[threadIdx.x >> 6] = best; bi[threadIdx.x >> 6] = idx; }
+  __syncthreads();
+  __shared__ double sgn;
+  if (threadIdx.x == 0) {
+    double b = bv[0];
+    int ii = bi[0];
+    for (int w = 1; w < 4; ++w)
+      if (bv[w] > b || (bv[w] == b && bi[w] < ii)) { b = bv[w]; ii = bi[w]; }
+    sgn = X[ii + (long long)j * n] < 0 ? -1.0 : 1.0;
+  }
+  __syncthreads();
+  if (sgn < 0) {
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+      X[i + (long long)j * n] = -X[i + (long long)j * n];
+      if (Y) Y[i + (long long)j * n] = -Y[i + (long long)j * n];
+    }
+  }
+}
+
+// coefficient block of the eigen adjoint (see eig_adjoint below).
+//   rows 0..chi-1   : Kt/2 with Kt = R11 Sigma + Sigma R11^T + C1bar,
+//                     R11_ij = (F_ij g_ij + F_ji g_ji) s_j
+//   rows chi..k-1   : F_ij g_ij (lambda_i + lambda_j)
+// F_ij = (s_j^2 - s_i^2) / ((s_j^2 - s_i^2)^2 + eta),  s = |lambda|.
+__global__ void eigback_coef_kernel(const double* __restrict__ g, const double* __restrict__ lam,
+                                    const double* __restrict__ C1bar, int k, int chi, double eta,
+                                    double* __restrict__ coef) {
+  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= k * chi) return;
+  int i = e % k, j = e / k;
+  double li = lam[i], lj = lam[j];
+  double si2 = li * li, sj2 = lj * lj;
+  double Fij = (sj2 - si2) / ((sj2 - si2) * (sj2 - si2) + eta);
+  double gij = g[i + (long long)j * k];
+  double v;
+  if (i < chi) {
+    double Fji = -Fij;
+    double gji = g[j + (long long)i * k];
+    double sj = fabs(lj), si = fabs(li);
+    double sigj = lj < 0 ? -1.0 : 1.0, sigi = li < 0 ? -1.0 : 1.0;
+    double R_ij = (Fij * gij + Fji * gji) * sj;   // R11[i,j]
+    double R_ji = (Fji * gji + Fij * gij) * si;   // R11[j,i]
+    double Kt = R_ij * sigj + sigi * R_ji + C1bar[i + (long long)j * chi];
+    v = 0.5 * Kt;
+  } else {
+    v = Fij * gij * (li + lj);
+  }
+  coef[e] = v;
+}
+
+// X_r[:, j] += scale_j^(m+1) * V[:, j]   with scale_j = 1 / s_j^2 (0 for masked columns)
+__global__ void neumann_acc_kernel(const double* __restrict__ V, double* __restrict__ Xr,
+                                   const double* __restrict__ inv_s2, int n, int chi, int power) {
+  size_t tot = (size_t)n * chi;
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < tot; e += (size_t)gridDim.x * blockDim.x) {
+    int j = e / n;
+    double f = inv_s2[j];
+    double p = f;
+    for (int q = 0; q < power; ++q) p *= f;
+    Xr[e] += p * V[e];
+  }
+}
+
+// W[:, j] += lambda_j * Xr[:, j] + AXr[:, j]
+__global__ void add_lam_cols_kernel(double* __restrict__ W, const double* __restrict__ Xr,
+                                    const double* __restrict__ AXr, const double* __restrict__ lam, int n,
+                                    int chi) {
+  size_t tot = (size_t)n * chi;
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < tot; e += (size_t)gridDim.x * blockDim.x) {
+    int j = e / n;
+    W[e] += lam[j] * Xr[e] + AXr[e];
+  }
+}
+
+// lbar[abc, x, y] += coef * tr[abc] * delta_xy
+__global__ void add_trace_diag_kernel(double* __restrict__ lbar, const double* __restrict__ tr,
+                                      const double* __restrict__ coef_dev, double coef_host, int m, int s) {
+  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= m * s) return;
+  int abc = e % m, x = e / m;
+  double c = coef_dev ? *coef_dev : coef_host;
+  lbar[abc + (size_t)m * (x + s * x)] += c * tr[abc];
+}
+
+// tr[abc] = sum_x l[abc, x, x]
+__global__ void trace_last2_kernel(const double* __restrict__ l, int m, int s, double* __restrict__ tr) {
+  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= m) return;
+  double acc = 0.0;
+  for (int x = 0; x < s; ++x) acc += l[e + (size_t)m * (x + s * x)];
+  tr[e] = acc;
+}
+
+// out = a / b  and the energy-adjoint scalars (ebar = 1/nn, nnbar = -e/nn^2), all on device
+__global__ void energy_scalars_kernel(const double* e, const double* nn, double* out) {
+  out[0] = e[0] / nn[0];
+  out[1] = 1.0 / nn[0];
+  out[2] = -e[0] / (nn[0] * nn[0]);
+}
+
+// norm adjoint: out = ybar/|x| + nbar * x / (|x| + eps(0f0)),  nbar = -<x,ybar>/|x|^2
+__global__ void norm_back_kernel(const double* __restrict__ x, const double* __restrict__ ybar,
+                                 const double* __restrict__ sumsq, const double* __restrict__ dot,
+                                 double* __restrict__ out, size_t n) {
+  const double nrm = sqrt(*sumsq);
+  const double nbar = -(*dot) / (nrm * nrm);
+  const double eps0 = 1.401298464324817e-45;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    out[i] = ybar[i] / nrm + nbar * x[i] / (nrm + eps0);
+}
+
+__global__ void sqrt_inplace_kernel(double* x) { x[0] = sqrt(x[0]); }
+
+inline unsigned nb(size_t n) {
+  size_t b = (n + NT - 1) / NT;
+  if (b > 65535ull * 8) b = 65535ull * 8;
+  return (unsigned)(b ? b : 1);
+}
+
+}  // namespace
+
+// --------------------------------------------------------------------------
+// typed buffer helpers
+// --------------------------------------------------------------------------
+PBuf::PBuf(Ctx* c, Prec p_, size_t n_) : p(p_), n(n_), b(c, n_ * esize(p_)) {}
+
+void to_prec(Ctx* c, const double* src, PBuf& dst) {
+  if (dst.p == Prec::F64) k_copy(src, dst.b.as<double>(), dst.n, c->stream);
+  else k_cast_from<float>(src, dst.b.as<float>(), dst.n, c->stream);
+}
+void from_prec(Ctx* c, const PBuf& src, double* dst) {
+  if (src.p == Prec::F64) k_copy(src.b.as<double>(), dst, src.n, c->stream);
+  else k_cast<float>(src.b.as<float>(), dst, src.n, c->stream);
+}
+
+void tc(Ctx* c, Prec p, const TView& A, const void* a, const TView& B, const void* b, const TView& C, void* o,
+        double alpha, double beta) {
+  tc_launch(p, contract_desc(A, B, C, alpha, beta), a, b, o, c->stream);
+}
+
+// ==========================================================================
+// Eigensolver: top-k eigenpairs of a symmetric n x n FP64 matrix by |lambda|
+// ==========================================================================
+static void orthonormalize(Ctx* c, double* Y, int n, int k, double* X, EigWork& w, unsigned seed) {
+  hipStream_t s = c->stream;
+  // column scaling, random refill of (numerically) null columns
+  k_col_norms(Y, n, k, w.nrm.as<double>(), s);
+  k_random_fill_cols(Y, n, k, w.nrm.as<double>(), 1e-13, seed, s);
+  k_col_norms(Y, n, k, w.nrm.as<double>(), s);
+  k_scale_cols(Y, n, k, w.nrm.as<double>(), 1, s);
+  // shifted CholeskyQR followed by two plain CholeskyQR passes (sCholQR3)
+  const double u = 1.1102230246251565e-16;
+  const double shift = 11.0 * ((double)n * k + (double)k * (k + 1)) * u * (double)k;
+  double* src = Y;
+  double* dst = X;
+  for (int pass = 0; pass < 3; ++pass) {
+    tc_launch(Prec::F64, gemm_desc('T', 'N', k, k, n, n, n, k), src, src, w.G.as<double>(), s);
+    chol_inv(w.G.as<double>(), k, pass == 0 ? shift : 0.0, w.Rinv.as<double>(), w.info.as<int>(), s);
+    tc_launch(Prec::F64, gemm_desc('N', 'N', n, k, k, n, k, n), src, w.Rinv.as<double>(), dst, s);
+    std::swap(src, dst);
+  }
+  // three passes alternate Y -> X -> Y -> X: the orthonormal block ends in X (Y is scratch)
+}
+
+EigWork::EigWork(Ctx* c, int n, int k)
+    : Y(c, (size_t)n * k * 8), Xr(c, (size_t)n * k * 8), Yr(c, (size_t)n * k * 8), H(c, (size_t)k * k * 8),
+      W(c, (size_t)k * k * 8), Vw(c, (size_t)k * k * 8), G(c, (size_t)k * k * 8), Rinv(c, (size_t)k * k * 8),
+      nrm(c, (size_t)k * 8), res(c, (size_t)k * 8), info(c, 16), sweeps(c, 16) {}
+
+int topk_eig(Ctx* c, const double* A, int n, int want, int k, const double* X0, int maxit, double tol,
+             double* X, double* AX, double* theta, EigStats* st) {
+  hipStream_t s = c->stream;
+  if (n <= JACOBI_MAX) {
+    // full spectrum: exact reference semantics (all n singular values / vectors)
+    DevBuf Vw(c, (size_t)n * n * 8);
+    jacobi_eig_ws(A, n, n, theta, X, Vw.as<double>(), 60, nullptr, s);
+    tc_launch(Prec::F64, gemm_desc('N', 'N', n, n, n, n, n, n), A, X, AX, s);
+    if (st) { st->iters = 1; st->k = n; st->resid = 0.0; }
+    return n;
+  }
+  k = std::min(std::max(k, want), std::min(n, JACOBI_MAX));
+  EigWork w(c, n, k);
+  // starting block
+  if (X0) {
+    k_copy(X0, w.Y.as<double>(), (size_t)n * k, s);
+  } else {
+    k_random_matrix(w.Y.as<double>(), (size_t)n * k, 0x1234567u, s);
+  }
+  orthonormalize(c, w.Y.as<double>(), n, k, X, w, 0x9e3779b9u);
+  std::vector<double> hres(k), hth(k);
+  int it = 0;
+  double rel = 0.0;
+  for (it = 1; it <= maxit; ++it) {
+    // Y = A X ; H = X^T Y ; H = W diag(theta) W^T ; Xr = X W ; Yr = Y W
+    tc_launch(Prec::F64, gemm_desc('N', 'N', n, k, n, n, n, n), A, X, w.Y.as<double>(), s);
+    tc_launch(Prec::F64, gemm_desc('T', 'N', k, k, n, n, n, k), X, w.Y.as<double>(), w.H.as<double>(), s);
+    jacobi_eig_ws(w.H.as<double>(), k, k, theta, w.W.as<double>(), w.Vw.as<double>(), 60, nullptr, s);
+    tc_launch(Prec::F64, gemm_desc('N', 'N', n, k, k, n, k, n), X, w.W.as<double>(), w.Xr.as<double>(), s);
+    tc_launch(Prec::F64, gemm_desc('N', 'N', n, k, k, n, k, n), w.Y.as<double>(), w.W.as<double>(),
+              w.Yr.as<double>(), s);
+    hipLaunchKernelGGL(resid_norms_kernel, dim3(want), dim3(NT), 0, s, w.Xr.as<double>(), w.Yr.as<double>(),
+                       theta, n, w.res.as<double>());
+    TNAD_HIP_TRY(hipGetLastError());
+    TNAD_HIP_TRY(hipMemcpyAsync(hres.data(), w.res.ptr, want * 8, hipMemcpyDeviceToHost, s));
+    TNAD_HIP_TRY(hipMemcpyAsync(hth.data(), theta, 8, hipMemcpyDeviceToHost, s));
+    c->sync();
+    double mx = 0.0;
+    for (int j = 0; j < want; ++j) mx = std::max(mx, hres[j]);
+    rel = mx / std::max(std::fabs(hth[0]), 1e-300);
+    if (!(rel == rel)) throw Error(TNAD_ERR_NAN, "eigensolver: NaN in Ritz residual");
+    if (rel <= tol || it == maxit) break;
+    orthonormalize(c, w.Yr.as<double>(), n, k, X, w, 0x51ed2701u + it);
+  }
+  k_copy(w.Xr.as<double>(), X, (size_t)n * k, s);
+  k_copy(w.Yr.as<double>(), AX, (size_t)n * k, s);
+  hipLaunchKernelGGL(canon_sign_kernel, dim3(k), dim3(NT), 0, s, X, AX, n);
+  TNAD_HIP_TRY(hipGetLastError());
+  if (st) { st->iters = std::min(it, maxit); st->k = k; st->resid = rel; }
+  return k;
+}
+
+// ==========================================================================
+// CTMRG step (src/ctmrg.jl:126-153)
+// ==========================================================================
+StepTape::StepTape(Ctx* c, const StepDims& d, Prec p, int k)
+    : prec(p), k(k), C(c, (size_t)d.chi * d.chi * 8), T(c, (size_t)d.chi * d.D * d.chi * 8),
+      X(c, (size_t)d.n * k * 8), AX(c, (size_t)d.n * k * 8), theta(c, (size_t)k * 8),
+      M(c, (size_t)d.n * d.n * 8), X2(c, p, (size_t)d.n * d.n), Y2(c, p, (size_t)d.n * d.n),
+      C2(c, (size_t)d.chi * d.chi * 8), T2(c, (size_t)d.chi * d.D * d.chi * 8), nrm(c, 4 * 8) {}
+
+void ctmrg_step(Ctx* c, const StepDims& d, const StepOpts& o, const double* a64, const PBuf& ap,
+                const double* C, const double* T, const double* X0, double* X_keep, double* Cout, double* Tout,
+                std::vector<double>& vals, StepTape* tape, StepState& state) {
+  hipStream_t s = c->stream;
+  const int chi = d.chi, D = d.D, n = d.n;
+  const long long lc = chi, lD = D;
+  const Prec p = o.prec;
+  const size_t es = esize(p);
+  // operands at contraction precision
+  PBuf Cp(c, p, (size_t)chi * chi), Tp(c, p, (size_t)chi * D * chi);
+  to_prec(c, C, Cp);
+  to_prec(c, T, Tp);
+  PBuf X1(c, p, (size_t)n * chi);
+  PBuf X2tmp;
+  PBuf* X2 = tape ? &tape->X2 : nullptr;
+  if (!X2) { X2tmp = PBuf(c, p, (size_t)n * n); X2 = &X2tmp; }
+  PBuf cp(c, p, (size_t)n * n);
+  // cp = ein"ad,iba,dcl,jkcb -> ijlk"(C, T, T, a)          (src/ctmrg.jl:130)
+  tc(c, p, cm("iba", {lc, lD, lc}), Tp.ptr(), cm("ad", {lc, lc}), Cp.ptr(), cm("ibd", {lc, lD, lc}), X1.ptr());
+  tc(c, p, cm("ibd", {lc, lD, lc}), X1.ptr(), cm("dcl", {lc, lD, lc}), Tp.ptr(), cm("ibcl", {lc, lD, lD, lc}),
+     X2->ptr());
+  tc(c, p, cm("ibcl", {lc, lD, lD, lc}), X2->ptr(), cm("jkcb", {lD, lD, lD, lD}), ap.ptr(),
+     cm("ijlk", {lc, lD, lc, lD}), cp.ptr());
+  // cpmat = reshape(cp, n, n); cpmat += cpmat'                (src/ctmrg.jl:134-135)
+  DevBuf Mtmp;
+  double* M = tape ? tape->M.as<double>() : nullptr;
+  if (!M) { Mtmp = DevBuf(c, (size_t)n * n * 8); M = Mtmp.as<double>(); }
+  if (p == Prec::F64) {
+    k_sym_add(cp.b.as<double>(), n, M, s);
+  } else {
+    DevBuf cp64(c, (size_t)n * n * 8);
+    k_cast<float>(cp.b.as<float>(), cp64.as<double>(), (size_t)n * n, s);
+    k_sym_add(cp64.as<double>(), n, M, s);
+  }
+  // u, s, v = svd(cpmat) -> top-k eigenpairs by |lambda|        (src/ctmrg.jl:136)
+  const int kreq = (n <= JACOBI_MAX) ? n : std::min(std::max(o.eig_block, chi), JACOBI_MAX);
+  DevBuf Xtmp, AXtmp, thtmp;
+  double *X = nullptr, *AX = nullptr, *theta = nullptr;
+  if (tape) { X = tape->X.as<double>(); AX = tape->AX.as<double>(); theta = tape->theta.as<double>(); }
+  else {
+    Xtmp = DevBuf(c, (size_t)n * kreq * 8); AXtmp = DevBuf(c, (size_t)n * kreq * 8); thtmp = DevBuf(c, kreq * 8);
+    X = Xtmp.as<double>(); AX = AXtmp.as<double>(); theta = thtmp.as<double>();
+  }
+  EigStats est;
+  const bool warm = X0 != nullptr && n > JACOBI_MAX;
+  int k = topk_eig(c, M, n, chi, kreq, warm ? X0 : nullptr, warm ? o.eig_warm_iters : o.eig_max_iters,
+                   o.eig_tol, X, AX, theta, &est);
+  state.k = k;
+  if (X_keep) k_copy(X, X_keep, (size_t)n * k, s);
+  state.eig_iters = est.iters;
+  state.eig_resid = est.resid;
+  // z = reshape(u[:, 1:chi], chi, D, chi)                        (src/ctmrg.jl:137)
+  const double* z = X;   // first chi columns
+  // C' + C'^T = z^T (cpmat + cpmat^T) z  (src/ctmrg.jl:139,145), FP64
+  double* C2 = tape ? tape->C2.as<double>() : nullptr;
+  DevBuf C2tmp;
+  if (!C2) { C2tmp = DevBuf(c, (size_t)chi * chi * 8); C2 = C2tmp.as<double>(); }
+  tc_launch(Prec::F64, gemm_desc('T', 'N', chi, chi, n, n, n, chi), z, AX, C2, s);
+  // T' = ein"abjcd,abi,dck -> ijk"(tp, z, z), tp = ein"iam,jkla -> ijklm"(T, a)  (src/ctmrg.jl:131,140)
+  // evaluated as ((T z) a) z without materialising tp.
+  PBuf zp(c, p, (size_t)n * chi);
+  to_prec(c, z, zp);
+  PBuf Y1(c, p, (size_t)n * n);
+  PBuf Y2tmp;
+  PBuf* Y2 = tape ? &tape->Y2 : nullptr;
+  if (!Y2) { Y2tmp = PBuf(c, p, (size_t)n * n); Y2 = &Y2tmp; }
+  PBuf T1p(c, p, (size_t)chi * D * chi);
+  tc(c, p, cm("aed", {lc, lD, lc}), Tp.ptr(), cm("abx", {lc, lD, lc}), zp.ptr(), cm("edbx", {lD, lc, lD, lc}),
+     Y1.ptr());
+  tc(c, p, cm("edbx", {lD, lc, lD, lc}), Y1.ptr(), cm("bjce", {lD, lD, lD, lD}), ap.ptr(),
+     cm("dcxj", {lc, lD, lc, lD}), Y2->ptr());
+  tc(c, p, cm("dcxj", {lc, lD, lc, lD}), Y2->ptr(), cm("dcy", {lc, lD, lc}), zp.ptr(), cm("xjy", {lc, lD, lc}),
+     T1p.ptr());
+  DevBuf T1(c, (size_t)chi * D * chi * 8);
+  from_prec(c, T1p, T1.as<double>());
+  double* T2 = tape ? tape->T2.as<double>() : nullptr;
+  DevBuf T2tmp;
+  if (!T2) { T2tmp = DevBuf(c, (size_t)chi * D * chi * 8); T2 = T2tmp.as<double>(); }
+  // edge += ein"ijk -> kji"(conj(edge))                          (src/ctmrg.jl:146)
+  k_edge_sym(T1.as<double>(), chi, D, T2, s);
+  // corner /= norm(corner); edge /= norm(edge)                   (src/ctmrg.jl:149-150)
+  DevBuf red(c, 300 * 8), nr(c, 4 * 8);
+  double* nrm = tape ? tape->nrm.as<double>() : nr.as<double>();
+  k_sumsq(C2, (size_t)chi * chi, red.as<double>(), s);
+  k_copy(red.as<double>(), nrm + 0, 1, s);
+  k_sumsq(T2, (size_t)chi * D * chi, red.as<double>(), s);
+  k_copy(red.as<double>(), nrm + 1, 1, s);
+  hipLaunchKernelGGL(sqrt_inplace_kernel, dim3(1), dim3(1), 0, s, nrm + 0);
+  hipLaunchKernelGGL(sqrt_inplace_kernel, dim3(1), dim3(1), 0, s, nrm + 1);
+  k_copy(C2, Cout, (size_t)chi * chi, s);
+  k_scale_by_dev(Cout, (size_t)chi * chi, nrm + 0, s);
+  k_copy(T2, Tout, (size_t)chi * D * chi, s);
+  k_scale_by_dev(Tout, (size_t)chi * D * chi, nrm + 1, s);
+  if (tape) {
+    k_copy(C, tape->C.as<double>(), (size_t)chi * chi, s);
+    k_copy(T, tape->T.as<double>(), (size_t)chi * D * chi, s);
+  }
+  // vals = s ./ s[1]  (computed part of the spectrum; the uncomputed tail is 0)   (src/ctmrg.jl:142)
+  std::vector<double> th(k);
+  TNAD_HIP_TRY(hipMemcpyAsync(th.data(), theta, (size_t)k * 8, hipMemcpyDeviceToHost, s));
+  c->sync();
+  vals.assign(n, 0.0);
+  double s0 = std::fabs(th[0]);
+  for (int i = 0; i < k; ++i) vals[i] = std::fabs(th[i]) / s0;
+  for (int i = 0; i < k; ++i)
+    if (!(vals[i] == vals[i])) throw Error(TNAD_ERR_NAN, "ctmrgstep: NaN in singular values");
+  (void)es;
+}
+
+// ==========================================================================
+// Eigen adjoint: cotangent of z = U[:, 1:chi] (plus the C' chain's C1bar)
+// -> cotangent of cpmat, i.e. svd_back (src/trg.jl:72-106) with dU nonzero in
+// the first chi columns, dS = dV = nothing, followed by cpmat += cpmat'.
+//
+//   cpbar = W U1^T + U1 W^T,  W = U [Kt/2 ; F.g.(l_i + l_j)] + (Xr Lambda1 + A Xr)
+//
+// where the last term is the contribution of the part of the spectrum the
+// truncated eigensolver did not compute (indices >= k): x_j = P (s_j^2 - A^2)^-1 P G_j
+// is summed as a Neumann series in A^2 / s_j^2 on the complement of the k
+// computed eigenvectors.  For n <= 192 the whole spectrum is known and the
+// formula is exactly the reference's.
+// ==========================================================================
+void eig_adjoint(Ctx* c, const StepDims& d, const StepTape& tp, const double* G, const double* C1bar,
+                 double* cpbar, EigBackStats* st) {
+  hipStream_t s = c->stream;
+  const int chi = d.chi, n = d.n, k = tp.k;
+  const double* X = tp.X.as<double>();
+  const double* lam = tp.theta.as<double>();
+  DevBuf g(c, (size_t)k * chi * 8), coef(c, (size_t)k * chi * 8), W(c, (size_t)n * chi * 8);
+  // g = U^T G  (k x chi)
+  tc_launch(Prec::F64, gemm_desc('T', 'N', k, chi, n, n, n, k), X, G, g.as<double>(), s);
+  hipLaunchKernelGGL(eigback_coef_kernel, dim3((k * chi + NT - 1) / NT), dim3(NT), 0, s, g.as<double>(), lam,
+                     C1bar, k, chi, 1e-40, coef.as<double>());
+  TNAD_HIP_TRY(hipGetLastError());
+  // W = U coef  (n x k . k x chi)
+  tc_launch(Prec::F64, gemm_desc('N', 'N', n, chi, k, n, k, n), X, coef.as<double>(), W.as<double>(), s);
+  int terms = 0;
+  if (k < n) {
+    // remainder of the spectrum via the Neumann series
+    std::vector<double> hl(k);
+    TNAD_HIP_TRY(hipMemcpyAsync(hl.data(), lam, (size_t)k * 8, hipMemcpyDeviceToHost, s));
+    c->sync();
+    const double s_tail = std::fabs(hl[k - 1]);   // upper bound of the uncomputed |lambda|
+    const double s_top = std::fabs(hl[0]);
+    std::vector<double> inv_s2(chi, 0.0);
+    double rho_max = 0.0;
+    for (int j = 0; j < chi; ++j) {
+      double sj = std::fabs(hl[j]);
+      double rho = (s_tail * s_tail) / (sj * sj);
+      if (sj > 1e-7 * s_top && rho < 0.5) {   // well separated from the uncomputed tail
+        inv_s2[j] = 1.0 / (sj * sj);
+        rho_max = std::max(rho_max, rho);
+      }
+    }
+    if (rho_max > 0.0 || inv_s2[0] != 0.0) {
+      terms = (rho_max > 0.0) ? (int)std::ceil(std::log(1e-17) / std::log(rho_max)) : 1;
+      terms = std::max(1, std::min(terms, 80));
+      DevBuf is2(c, (size_t)chi * 8), V(c, (size_t)n * chi * 8), V2(c, (size_t)n * chi * 8),
+          Xr(c, (size_t)n * chi * 8), pj(c, (size_t)k * chi * 8);
+      TNAD_HIP_TRY(hipMemcpyAsync(is2.ptr, inv_s2.data(), (size_t)chi * 8, hipMemcpyHostToDevice, s));
+      // V0 = P G = G - U (U^T G)
+      k_copy(G, V.as<double>(), (size_t)n * chi, s);
+      tc_launch(Prec::F64, gemm_desc('N', 'N', n, chi, k, n, k, n, -1.0, 1.0), X, g.as<double>(), V.as<double>(), s);
+      k_fill(Xr.as<double>(), (size_t)n * chi, 0.0, s);
+      for (int m = 0; m < terms; ++m) {
+        hipLaunchKernelGGL(neumann_acc_kernel, dim3(nb((size_t)n * chi)), dim3(NT), 0, s, V.as<double>(),
+                           Xr.as<double>(), is2.as<double>(), n, chi, m);
+        if (m + 1 == terms) break;
+        // V <- P A (A V)
+        tc_launch(Prec::F64, gemm_desc('N', 'N', n, chi, n, n, n, n), tp.M.as<double>(), V.as<double>(),
+                  V2.as<double>(), s);
+        tc_launch(Prec::F64, gemm_desc('N', 'N', n, chi, n, n, n, n), tp.M.as<double>(), V2.as<double>(),
+                  V.as<double>(), s);
+        tc_launch(Prec::F64, gemm_desc('T', 'N', k, chi, n, n, n, k), X, V.as<double>(), pj.as<double>(), s);
+        tc_launch(Prec::F64, gemm_desc('N', 'N', n, chi, k, n, k, n, -1.0, 1.0), X, pj.as<double>(),
+                  V.as<double>(), s);
+      }
+      // W += Xr Lambda1 + A Xr
+      tc_launch(Prec::F64, gemm_desc('N', 'N', n, chi, n, n, n, n), tp.M.as<double>(), Xr.as<double>(),
+                V2.as<double>(), s);
+      hipLaunchKernelGGL(add_lam_cols_kernel, dim3(nb((size_t)n * chi)), dim3(NT), 0, s, W.as<double>(),
+                         Xr.as<double>(), V2.as<double>(), lam, n, chi);
+      TNAD_HIP_TRY(hipGetLastError());
+    }
+  }
+  // cpbar = W U1^T + U1 W^T
+  tc_launch(Prec::F64, gemm_desc('N', 'T', n, n, chi, n, n, n), W.as<double>(), X, cpbar, s);
+  tc_launch(Prec::F64, gemm_desc('N', 'T', n, n, chi, n, n, n, 1.0, 1.0), X, W.as<double>(), cpbar, s);
+  if (st) st->neumann_terms = terms;
+}
+
+// ==========================================================================
+// Reverse sweep of one ctmrgstep.  In: cotangents of the step outputs
+// (C3bar, T3bar).  Out: cotangents of the step inputs (Cbar, Tbar) and the
+// bulk contribution added into abar.
+// ==========================================================================
+void ctmrg_step_back(Ctx* c, const StepDims& d, Prec p, const StepTape& tp, const double* a64, const PBuf& ap,
+                     const double* C3bar, const double* T3bar, double* Cbar, double* Tbar, double* abar) {
+  hipStream_t s = c->stream;
+  const int chi = d.chi, D = d.D, n = d.n;
+  const long long lc = chi, lD = D;
+  const size_t nC = (size_t)chi * chi, nT = (size_t)chi * D * chi, nA = (size_t)D * D * D * D;
+  DevBuf red(c, 300 * 8), dotb(c, 300 * 8);
+  // --- normalisation adjoints (src/autodiff.jl:23-29) --------------------
+  DevBuf C2bar(c, nC * 8), T2bar(c, nT * 8), C1bar(c, nC * 8), T1bar(c, nT * 8);
+  {
+    DevBuf ss(c, 2 * 8), dd(c, 2 * 8);
+    k_sumsq(tp.C2.as<double>(), nC, red.as<double>(), s);
+    k_copy(red.as<double>(), ss.as<double>() + 0, 1, s);
+    k_dot(tp.C2.as<double>(), C3bar, nC, red.as<double>(), s);
+    k_copy(red.as<double>(), dd.as<double>() + 0, 1, s);
+    k_sumsq(tp.T2.as<double>(), nT, red.as<double>(), s);
+    k_copy(red.as<double>(), ss.as<double>() + 1, 1, s);
+    k_dot(tp.T2.as<double>(), T3bar, nT, red.as<double>(), s);
+    k_copy(red.as<double>(), dd.as<double>() + 1, 1, s);
+    hipLaunchKernelGGL(norm_back_kernel, dim3(nb(nC)), dim3(NT), 0, s, tp.C2.as<double>(), C3bar,
+                       ss.as<double>() + 0, dd.as<double>() + 0, C2bar.as<double>(), nC);
+    hipLaunchKernelGGL(norm_back_kernel, dim3(nb(nT)), dim3(NT), 0, s, tp.T2.as<double>(), T3bar,
+                       ss.as<double>() + 1, dd.as<double>() + 1, T2bar.as<double>(), nT);
+    TNAD_HIP_TRY(hipGetLastError());
+  }
+  // C1bar = C2bar + C2bar' ; T1bar = T2bar + permutedims(T2bar, (3,2,1))
+  k_sym_add(C2bar.as<double>(), chi, C1bar.as<double>(), s);
+  k_edge_sym(T2bar.as<double>(), chi, D, T1bar.as<double>(), s);
+
+  // --- T' chain adjoint (tp never materialised) ----------------------------
+  PBuf T1b(c, p, nT), zp(c, p, (size_t)n * chi), Tp(c, p, nT), Cp(c, p, nC);
+  to_prec(c, T1bar.as<double>(), T1b);
+  to_prec(c, tp.X.as<double>(), zp);   // first n*chi entries = z
+  to_prec(c, tp.T.as<double>(), Tp);
+  to_prec(c, tp.C.as<double>(), Cp);
+  PBuf P1(c, p, (size_t)n * n), Q1(c, p, (size_t)n * n), S1(c, p, (size_t)n * n);
+  PBuf Tbp(c, p, nT), abp(c, p, nA), zbp(c, p, (size_t)n * chi);
+  // P1[a,b,j,y] = sum_x z[a,b,x] T1bar[x,j,y]
+  tc(c, p, cm("abx", {lc, lD, lc}), zp.ptr(), cm("xjy", {lc, lD, lc}), T1b.ptr(), cm("abjy", {lc, lD, lD, lc}),
+     P1.ptr());
+  // Q1[a,e,c,y] = sum_{b,j} P1[a,b,j,y] a[b,j,c,e] ; Tbar[a,e,d] = sum_{c,y} Q1[a,e,c,y] z[d,c,y]
+  tc(c, p, cm("abjy", {lc, lD, lD, lc}), P1.ptr(), cm("bjce", {lD, lD, lD, lD}), ap.ptr(),
+     cm("aecy", {lc, lD, lD, lc}), Q1.ptr());
+  tc(c, p, cm("aecy", {lc, lD, lD, lc}), Q1.ptr(), cm("dcy", {lc, lD, lc}), zp.ptr(), cm("aed", {lc, lD, lc}),
+     Tbp.ptr());
+  // S1[a,e,c,y] = sum_d T[a,e,d] z[d,c,y] ; abar[b,j,c,e] = sum_{a,y} P1[a,b,j,y] S1[a,e,c,y]
+  tc(c, p, cm("aed", {lc, lD, lc}), Tp.ptr(), cm("dcy", {lc, lD, lc}), zp.ptr(), cm("aecy", {lc, lD, lD, lc}),
+     S1.ptr());
+  tc(c, p, cm("abjy", {lc, lD, lD, lc}), P1.ptr(), cm("aecy", {lc, lD, lD, lc}), S1.ptr(),
+     cm("bjce", {lD, lD, lD, lD}), abp.ptr());
+  // U1[a,b,j,y] = sum_{e,c} S1[a,e,c,y] a[b,j,c,e] (reuses P1) ; zbar[a,b,x] = sum_{j,y} U1 T1bar[x,j,y]
+  tc(c, p, cm("aecy", {lc, lD, lD, lc}), S1.ptr(), cm("bjce", {lD, lD, lD, lD}), ap.ptr(),
+     cm("abjy", {lc, lD, lD, lc}), P1.ptr());
+  tc(c, p, cm("abjy", {lc, lD, lD, lc}), P1.ptr(), cm("xjy", {lc, lD, lc}), T1b.ptr(), cm("abx", {lc, lD, lc}),
+     zbp.ptr());
+  // zbar[d,c,y] += sum_{x,j} Y2[d,c,x,j] T1bar[x,j,y]
+  tc(c, p, cm("dcxj", {lc, lD, lc, lD}), tp.Y2.ptr(), cm("xjy", {lc, lD, lc}), T1b.ptr(), cm("dcy", {lc, lD, lc}),
+     zbp.ptr(), 1.0, 1.0);
+  DevBuf zbar(c, (size_t)n * chi * 8);
+  from_prec(c, zbp, zbar.as<double>());
+  DevBuf a_acc(c, nA * 8);
+  from_prec(c, abp, a_acc.as<double>());
+  k_axpby(1.0, a_acc.as<double>(), 1.0, abar, nA, s);
+  DevBuf Tacc(c, nT * 8);
+  from_prec(c, Tbp, Tacc.as<double>());
+
+  // --- C' chain: zbar += (cpmat + cpmat') z C1bar  (FP64; A z kept from the forward)
+  tc_launch(Prec::F64, gemm_desc('N', 'N', n, chi, chi, n, chi, n, 1.0, 1.0), tp.AX.as<double>(),
+            C1bar.as<double>(), zbar.as<double>(), s);
+
+  // --- eigen adjoint -> cpbar (n x n, FP64) ---------------------------------
+  DevBuf cpbar(c, (size_t)n * n * 8);
+  eig_adjoint(c, d, tp, zbar.as<double>(), C1bar.as<double>(), cpbar.as<double>(), nullptr);
+
+  // --- cp chain adjoint -----------------------------------------------------
+  PBuf cpb(c, p, (size_t)n * n), X2b(c, p, (size_t)n * n), X1(c, p, (size_t)n * chi), X1b(c, p, (size_t)n * chi);
+  PBuf Cbp(c, p, nC), Tbp2(c, p, nT);
+  to_prec(c, cpbar.as<double>(), cpb);
+  // X1 = T C (recomputed)
+  tc(c, p, cm("iba", {lc, lD, lc}), Tp.ptr(), cm("ad", {lc, lc}), Cp.ptr(), cm("ibd", {lc, lD, lc}), X1.ptr());
+  // X2bar[i,b,c,l] = sum_{j,k} cpbar[i,j,l,k] a[j,k,c,b]
+  tc(c, p, cm("ijlk", {lc, lD, lc, lD}), cpb.ptr(), cm("jkcb", {lD, lD, lD, lD}), ap.ptr(),
+     cm("ibcl", {lc, lD, lD, lc}), X2b.ptr());
+  // abar[j,k,c,b] += sum_{i,l} cpbar[i,j,l,k] X2[i,b,c,l]
+  tc(c, p, cm("ijlk", {lc, lD, lc, lD}), cpb.ptr(), cm("ibcl", {lc, lD, lD, lc}), tp.X2.ptr(),
+     cm("jkcb", {lD, lD, lD, lD}), abp.ptr());
+  from_prec(c, abp, a_acc.as<double>());
+  k_axpby(1.0, a_acc.as<double>(), 1.0, abar, nA, s);
+  // X1bar[i,b,d] = sum_{c,l} X2bar[i,b,c,l] T[d,c,l] ; Tbar[d,c,l] += sum_{i,b} X1[i,b,d] X2bar[i,b,c,l]
+  tc(c, p, cm("ibcl", {lc, lD, lD, lc}), X2b.ptr(), cm("dcl", {lc, lD, lc}), Tp.ptr(), cm("ibd", {lc, lD, lc}),
+     X1b.ptr());
+  tc(c, p, cm("ibd", {lc, lD, lc}), X1.ptr(), cm("ibcl", {lc, lD, lD, lc}), X2b.ptr(), cm("dcl", {lc, lD, lc}),
+     Tbp2.ptr());
+  // Cbar[a,d] = sum_{i,b} T[i,b,a] X1bar[i,b,d] ; Tbar[i,b,a] += sum_d X1bar[i,b,d] C[a,d]
+  tc(c, p, cm("iba", {lc, lD, lc}), Tp.ptr(), cm("ibd", {lc, lD, lc}), X1b.ptr(), cm("ad", {lc, lc}), Cbp.ptr());
+  tc(c, p, cm("ibd", {lc, lD, lc}), X1b.ptr(), cm("ad", {lc, lc}), Cp.ptr(), cm("iba", {lc, lD, lc}), Tbp2.ptr(),
+     1.0, 1.0);
+  from_prec(c, Cbp, Cbar);
+  from_prec(c, Tbp2, Tbar);
+  k_axpby(1.0, Tacc.as<double>(), 1.0, Tbar, nT, s);
+  (void)a64;
+}
+
+// ==========================================================================
+// expectationvalue (src/variationalipeps.jl:49-56), FP64, with reverse sweep
+// l = ein"ab,ica,bde,cjfdlm,eg,gfk -> ijklm"(C, T, T, ap, C, T)
+// ==========================================================================
+EvCache::EvCache(Ctx* c, int chi, int D, int ph)
+    : apn(c, (size_t)D * D * D * D * ph * ph * 8), E1(c, (size_t)chi * D * chi * 8),
+      E2(c, (size_t)chi * D * D * chi * 8), E3(c, (size_t)chi * D * chi * 8),
+      E4(c, (size_t)chi * D * chi * D * ph * ph * 8), l(c, (size_t)chi * D * chi * ph * ph * 8),
+      tr(c, (size_t)chi * D * chi * 8), scal(c, 8 * 8) {}
+
+void expectation_fwd(Ctx* c, int chi, int D, int ph, const double* h, const double* ap, const double* C,
+                     const double* T, EvCache& ev, double* e_host) {
+  hipStream_t s = c->stream;
+  const long long lc = chi, lD = D, ls = ph;
+  const long long n = lc * lD;
+  const size_t nap = (size_t)D * D * D * D * ph * ph;
+  DevBuf red(c, 300 * 8);
+  // ap /= norm(ap)
+  k_sumsq(ap, nap, red.as<double>(), s);
+  k_copy(red.as<double>(), ev.scal.as<double>() + 3, 1, s);   // |ap|^2
+  k_copy(ap, ev.apn.as<double>(), nap, s);
+  k_copy(red.as<double>(), ev.scal.as<double>() + 4, 1, s);
+  hipLaunchKernelGGL(sqrt_inplace_kernel, dim3(1), dim3(1), 0, s, ev.scal.as<double>() + 4);
+  k_scale_by_dev(ev.apn.as<double>(), nap, ev.scal.as<double>() + 4, s);
+  const Prec F = Prec::F64;
+  double* E1 = ev.E1.as<double>();
+  double* E2 = ev.E2.as<double>();
+  double* E3 = ev.E3.as<double>();
+  double* E4 = ev.E4.as<double>();
+  double* l = ev.l.as<double>();
+  tc(c, F, cm("ica", {lc, lD, lc}), T, cm("ab", {lc, lc}), C, cm("icb", {lc, lD, lc}), E1);
+  tc(c, F, cm("icb", {lc, lD, lc}), E1, cm("bde", {lc, lD, lc}), T, cm("icde", {lc, lD, lD, lc}), E2);
+  tc(c, F, cm("eg", {lc, lc}), C, cm("gfk", {lc, lD, lc}), T, cm("efk", {lc, lD, lc}), E3);
+  // E4[i,j,e,f,l,m] = sum_{c,d} E2[i,c,d,e] ap[c,j,f,d,l,m]  (one launch per physical pair (l,m))
+  const long long e4s[4] = {1, lc, n, n * lc};
+  for (int lm = 0; lm < ph * ph; ++lm) {
+    tc(c, F, cm("icde", {lc, lD, lD, lc}), E2, cm("cjfd", {lD, lD, lD, lD}), ev.apn.as<double>() + (size_t)lm * D * D * D * D,
+       with_strides("ijef", {lc, lD, lc, lD}, {e4s[0], e4s[1], e4s[2], e4s[3]}), E4 + (size_t)lm * n * n);
+  }
+  // l[i,j,k,l,m] = sum_{e,f} E4[i,j,e,f,l,m] E3[e,f,k]
+  tc(c, F, with_strides("ijeflm", {lc, lD, lc, lD, ls, ls}, {1, lc, n, n * lc, n * n, n * n * ls}), E4,
+     cm("efk", {lc, lD, lc}), E3, cm("ijklm", {lc, lD, lc, ls, ls}), l);
+  // e = ein"abcij,abckl,ijkl -> "(l, l, h) ; n = ein"ijkaa,ijkbb -> "(l, l)
+  DevBuf Lh(c, (size_t)n * chi * ph * ph * 8);
+  tc(c, F, cm("abcij", {lc, lD, lc, ls, ls}), l, cm("ijkl", {ls, ls, ls, ls}), h, cm("abckl", {lc, lD, lc, ls, ls}),
+     Lh.as<double>());
+  const size_t nl = (size_t)n * chi * ph * ph, m3 = (size_t)n * chi;
+  k_dot(Lh.as<double>(), l, nl, red.as<double>(), s);
+  k_copy(red.as<double>(), ev.scal.as<double>() + 0, 1, s);   // e
+  hipLaunchKernelGGL(trace_last2_kernel, dim3((m3 + NT - 1) / NT), dim3(NT), 0, s, l, (int)m3, ph, ev.tr.as<double>());
+  k_sumsq(ev.tr.as<double>(), m3, red.as<double>(), s);
+  k_copy(red.as<double>(), ev.scal.as<double>() + 1, 1, s);   // nn
+  // scal[5] = e/nn, scal[6] = 1/nn, scal[7] = -e/nn^2
+  hipLaunchKernelGGL(energy_scalars_kernel, dim3(1), dim3(1), 0, s, ev.scal.as<double>() + 0,
+                     ev.scal.as<double>() + 1, ev.scal.as<double>() + 5);
+  TNAD_HIP_TRY(hipGetLastError());
+  if (e_host) {
+    TNAD_HIP_TRY(hipMemcpyAsync(e_host, ev.scal.as<double>() + 5, 8, hipMemcpyDeviceToHost, s));
+    c->sync();
+  }
+}
+
+void expectation_back(Ctx* c, int chi, int D, int ph, const double* h, const double* ap, const double* C,
+                      const double* T, EvCache& ev, double* apbar, double* Cbar, double* Tbar) {
+  hipStream_t s = c->stream;
+  const long long lc = chi, lD = D, ls = ph;
+  const long long n = lc * lD;
+  const size_t nap = (size_t)D * D * D * D * ph * ph;
+  const size_t nl = (size_t)n * chi * ph * ph, m3 = (size_t)n * chi;
+  const Prec F = Prec::F64;
+  double* l = ev.l.as<double>();
+  // lbar = ebar (l h^T + l h) + 2 nnbar tr (x) delta
+  DevBuf lbar(c, nl * 8), tmp(c, nl * 8), red(c, 300 * 8);
+  tc(c, F, cm("abcij", {lc, lD, lc, ls, ls}), l, cm("ijkl", {ls, ls, ls, ls}), h, cm("abckl", {lc, lD, lc, ls, ls}),
+     lbar.as<double>());
+  tc(c, F, cm("abckl", {lc, lD, lc, ls, ls}), l, cm("ijkl", {ls, ls, ls, ls}), h, cm("abcij", {lc, lD, lc, ls, ls}),
+     tmp.as<double>());
+  k_axpby(1.0, tmp.as<double>(), 1.0, lbar.as<double>(), nl, s);
+  // scale by ebar (device scalar): lbar *= 1/nn  -> use scale_by_dev with nn
+  k_scale_by_dev(lbar.as<double>(), nl, ev.scal.as<double>() + 1, s);
+  {
+    // 2 * nnbar = -2 e / nn^2
+    DevBuf cf(c, 8);
+    std::vector<double> sc(8);
+    TNAD_HIP_TRY(hipMemcpyAsync(sc.data(), ev.scal.ptr, 8 * 8, hipMemcpyDeviceToHost, s));
+    c->sync();
+    double coef = 2.0 * sc[7];
+    hipLaunchKernelGGL(add_trace_diag_kernel, dim3((m3 * ph + NT - 1) / NT), dim3(NT), 0, s, lbar.as<double>(),
+                       ev.tr.as<double>(), (const double*)nullptr, coef, (int)m3, ph);
+    TNAD_HIP_TRY(hipGetLastError());
+  }
+  double* E1 = ev.E1.as<double>();
+  double* E2 = ev.E2.as<double>();
+  double* E3 = ev.E3.as<double>();
+  double* E4 = ev.E4.as<double>();
+  const TView vE4 = with_strides("ijeflm", {lc, lD, lc, lD, ls, ls}, {1, lc, n, n * lc, n * n, n * n * ls});
+  DevBuf E4b(c, (size_t)n * n * ph * ph * 8), E3b(c, (size_t)n * chi * 8), E2b(c, (size_t)n * n * 8),
+      E1b(c, (size_t)n * chi * 8), apnb(c, nap * 8);
+  tc(c, F, cm("ijklm", {lc, lD, lc, ls, ls}), lbar.as<double>(), cm("efk", {lc, lD, lc}), E3, vE4, E4b.as<double>());
+  tc(c, F, vE4, E4, cm("ijklm", {lc, lD, lc, ls, ls}), lbar.as<double>(), cm("efk", {lc, lD, lc}), E3b.as<double>());
+  const TView vE4s = with_strides("ijef", {lc, lD, lc, lD}, {1, lc, n, n * lc});
+  for (int lm = 0; lm < ph * ph; ++lm) {
+    const size_t oa = (size_t)lm * D * D * D * D, oe = (size_t)lm * n * n;
+    tc(c, F, vE4s, E4b.as<double>() + oe, cm("cjfd", {lD, lD, lD, lD}), ev.apn.as<double>() + oa,
+       cm("icde", {lc, lD, lD, lc}), E2b.as<double>(), 1.0, lm == 0 ? 0.0 : 1.0);
+    tc(c, F, cm("icde", {lc, lD, lD, lc}), E2, vE4s, E4b.as<double>() + oe, cm("cjfd", {lD, lD, lD, lD}),
+       apnb.as<double>() + oa);
+  }
+  tc(c, F, cm("icde", {lc, lD, lD, lc}), E2b.as<double>(), cm("bde", {lc, lD, lc}), T, cm("icb", {lc, lD, lc}),
+     E1b.as<double>());
+  tc(c, F, cm("icb", {lc, lD, lc}), E1, cm("icde", {lc, lD, lD, lc}), E2b.as<double>(), cm("bde", {lc, lD, lc}),
+     Tbar, 1.0, 1.0);
+  tc(c, F, cm("icb", {lc, lD, lc}), E1b.as<double>(), cm("ab", {lc, lc}), C, cm("ica", {lc, lD, lc}), Tbar, 1.0, 1.0);
+  tc(c, F, cm("ica", {lc, lD, lc}), T, cm("icb", {lc, lD, lc}), E1b.as<double>(), cm("ab", {lc, lc}), Cbar, 1.0, 1.0);
+  tc(c, F, cm("efk", {lc, lD, lc}), E3b.as<double>(), cm("gfk", {lc, lD, lc}), T, cm("eg", {lc, lc}), Cbar, 1.0, 1.0);
+  tc(c, F, cm("eg", {lc, lc}), C, cm("efk", {lc, lD, lc}), E3b.as<double>(), cm("gfk", {lc, lD, lc}), Tbar, 1.0, 1.0);
+  // ap /= norm(ap) adjoint
+  DevBuf dd(c, 8);
+  k_dot(ap, apnb.as<double>(), nap, red.as<double>(), s);
+  k_copy(red.as<double>(), dd.as<double>(), 1, s);
+  hipLaunchKernelGGL(norm_back_kernel, dim3(nb(nap)), dim3(NT), 0, s, ap, apnb.as<double>(),
+                     ev.scal.as<double>() + 3, dd.as<double>(), apbar, nap);
+  TNAD_HIP_TRY(hipGetLastError());
+}
+
+}  // namespace tnad

@@ -1,0 +1,332 @@
+// Generic strided tensor contraction on CDNA4 matrix cores.
+//
+//   C[m,n] = alpha * sum_k A[m,k] * B[k,n] + beta * C[m,n]
+//
+// with every index split into two sub-indices carrying arbitrary element
+// strides (TcDesc).  This lets each einsum of the CTMRG step, its adjoint and
+// the energy chain run as one launch straight from the tensors' natural
+// column-major layout: the permutations live in the tile loaders' address
+// arithmetic instead of separate transpose passes.
+//
+// Tiling: 64x64 output tile per 256-thread workgroup (4 waves in 2x2), each
+// wave owns a 32x32 sub-tile made of 2x2 MFMA 16x16 blocks.  Operands are
+// staged through LDS with a register prefetch of the next K-tile.
+//   F64 : v_mfma_f64_16x16x4f64       (BK = 16)
+//   F32 : v_mfma_f32_16x16x4f32       (BK = 16)
+//   BF16: v_mfma_f32_16x16x16bf16_1k  (BK = 32, fp32 in HBM, bf16 in LDS,
+//                                      fp32 accumulation)
+#include "tnad_internal.h"
+
+namespace tnad {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+typedef short s4 __attribute__((ext_vector_type(4)));
+
+TcDesc gemm_desc(char ta, char tb, int M, int N, int K, long long lda, long long ldb, long long ldc,
+                 double alpha, double beta) {
+  TcDesc d;
+  d.M0 = M; d.N0 = N; d.K0 = K;
+  if (ta == 'N') { d.am0 = 1; d.ak0 = lda; } else { d.am0 = lda; d.ak0 = 1; }
+  if (tb == 'N') { d.bk0 = 1; d.bn0 = ldb; } else { d.bk0 = ldb; d.bn0 = 1; }
+  d.cm0 = 1; d.cn0 = ldc;
+  d.alpha = alpha; d.beta = beta;
+  return d;
+}
+
+namespace {
+
+constexpr int BM = 64, BN = 64, NT = 256;
+
+__device__ __forceinline__ long long off2(int i, int I0, long long s0, long long s1) {
+  int i0 = i % I0;
+  int i1 = i / I0;
+  return (long long)i0 * s0 + (long long)i1 * s1;
+}
+
+template <typename T> struct MfmaOp;
+template <> struct MfmaOp<double> {
+  typedef d4 acc_t;
+  static __device__ __forceinline__ acc_t run(double a, double b, acc_t c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+};
+template <> struct MfmaOp<float> {
+  typedef f4 acc_t;
+  static __device__ __forceinline__ acc_t run(float a, float b, acc_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+};
+
+// ---------------------------------------------------------------------------
+// F64 / F32 kernel (k-major LDS tiles, one element per lane per MFMA)
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(NT) void tc_kernel(TcDesc d, const T* __restrict__ A,
+                                                const T* __restrict__ B, T* __restrict__ C) {
+  constexpr int BK = 16;
+  constexpr int PAD = 16;
+  __shared__ T As[BK][BM + PAD];
+  __shared__ T Bs[BK][BN + PAD];
+
+  const int M = d.M0 * d.M1, N = d.N0 * d.N1, K = d.K0 * d.K1;
+  const int tile_m = blockIdx.x * BM, tile_n = blockIdx.y * BN;
+  const int bz = blockIdx.z;
+  A += (long long)bz * d.abat;
+  B += (long long)bz * d.bbat;
+  C += (long long)bz * d.cbat;
+
+  const int t = threadIdx.x;
+  const int lane = t & 63, wave = t >> 6;
+  const int wm = wave & 1, wn = wave >> 1;
+
+  // A loader mapping
+  const bool a_mfast = (d.am0 == 1 && d.M0 > 1) || d.ak0 != 1;
+  int a_mi[4], a_ki[4];
+  long long a_moff[4];
+  bool a_mok[4];
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    if (a_mfast) { a_mi[r] = t & 63; a_ki[r] = (t >> 6) + 4 * r; }
+    else         { a_ki[r] = t & 15; a_mi[r] = (t >> 4) + 16 * r; }
+    int m = tile_m + a_mi[r];
+    a_mok[r] = m < M;
+    a_moff[r] = a_mok[r] ? off2(m, d.M0, d.am0, d.am1) : 0;
+  }
+  // B loader mapping
+  const bool b_nfast = (d.bn0 == 1 && d.N0 > 1) || d.bk0 != 1;
+  int b_ni[4], b_ki[4];
+  long long b_noff[4];
+  bool b_nok[4];
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    if (b_nfast) { b_ni[r] = t & 63; b_ki[r] = (t >> 6) + 4 * r; }
+    else         { b_ki[r] = t & 15; b_ni[r] = (t >> 4) + 16 * r; }
+    int n = tile_n + b_ni[r];
+    b_nok[r] = n < N;
+    b_noff[r] = b_nok[r] ? off2(n, d.N0, d.bn0, d.bn1) : 0;
+  }
+
+  T ra[4], rb[4];
+  auto gload = [&](int k0) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      int k = k0 + a_ki[r];
+      T v = T(0);
+      if (a_mok[r] && k < K) v = A[a_moff[r] + off2(k, d.K0, d.ak0, d.ak1)];
+      ra[r] = v;
+      int kb = k0 + b_ki[r];
+      T w = T(0);
+      if (b_nok[r] && kb < K) w = B[b_noff[r] + off2(kb, d.K0, d.bk0, d.bk1)];
+      rb[r] = w;
+    }
+  };
+
+  typedef typename MfmaOp<T>::acc_t acc_t;
+  acc_t acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
+
+  gload(0);
+  for (int k0 = 0; k0 < K; k0 += BK) {
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      As[a_ki[r]][a_mi[r]] = ra[r];
+      Bs[b_ki[r]][b_ni[r]] = rb[r];
+    }
+    __syncthreads();
+    if (k0 + BK < K) gload(k0 + BK);
+#pragma unroll
+    for (int kk = 0; kk < BK; kk += 4) {
+      T af[2], bf[2];
+#pragma unroll
+      for (int i = 0; i < 2; ++i) af[i] = As[kk + (lane >> 4)][wm * 32 + i * 16 + (lane & 15)];
+#pragma unroll
+      for (int j = 0; j < 2; ++j) bf[j] = Bs[kk + (lane >> 4)][wn * 32 + j * 16 + (lane & 15)];
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) acc[i][j] = MfmaOp<T>::run(af[i], bf[j], acc[i][j]);
+    }
+  }
+
+  // epilogue: lane holds rows 4*(lane/16)+r, column lane%16 of each 16x16 block
+  const T alpha = (T)d.alpha, beta = (T)d.beta;
+#pragma unroll
+  for (int j = 0; j < 2; ++j) {
+    int n = tile_n + wn * 32 + j * 16 + (lane & 15);
+    if (n >= N) continue;
+    long long noff = off2(n, d.N0, d.cn0, d.cn1);
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        int m = tile_m + wm * 32 + i * 16 + 4 * (lane >> 4) + r;
+        if (m >= M) continue;
+        long long o = noff + off2(m, d.M0, d.cm0, d.cm1);
+        T v = alpha * (T)acc[i][j][r];
+        if (d.beta != 0.0) v += beta * C[o];
+        C[o] = v;
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// BF16 kernel: fp32 operands in HBM rounded to bf16 while staging, m-major LDS
+// tiles so each lane fetches its 4 consecutive-k bf16 values in one 8-byte read.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ unsigned short f2bf(float x) {
+  unsigned int u = __float_as_uint(x);
+  // round to nearest even (NaN stays NaN: quiet bit kept by the shift)
+  unsigned int lsb = (u >> 16) & 1u;
+  u += 0x7fffu + lsb;
+  return (unsigned short)(u >> 16);
+}
+
+__global__ __launch_bounds__(NT) void tc_kernel_bf16(TcDesc d, const float* __restrict__ A,
+                                                     const float* __restrict__ B,
+                                                     float* __restrict__ C) {
+  constexpr int BK = 32;
+  constexpr int KP = BK + 4;   // row pitch in bf16 elements (8-byte aligned rows)
+  __shared__ unsigned short As[BM][KP];
+  __shared__ unsigned short Bs[BN][KP];
+
+  const int M = d.M0 * d.M1, N = d.N0 * d.N1, K = d.K0 * d.K1;
+  const int tile_m = blockIdx.x * BM, tile_n = blockIdx.y * BN;
+  const int bz = blockIdx.z;
+  A += (long long)bz * d.abat;
+  B += (long long)bz * d.bbat;
+  C += (long long)bz * d.cbat;
+
+  const int t = threadIdx.x;
+  const int lane = t & 63, wave = t >> 6;
+  const int wm = wave & 1, wn = wave >> 1;
+
+  // each thread stages 8 elements of A and 8 of B per K-tile (64x32 tiles)
+  const bool a_mfast = (d.am0 == 1 && d.M0 > 1) || d.ak0 != 1;
+  int a_mi[8], a_ki[8];
+  long long a_moff[8];
+  bool a_mok[8];
+#pragma unroll
+  for (int r = 0; r < 8; ++r) {
+    if (a_mfast) { a_mi[r] = t & 63; a_ki[r] = (t >> 6) + 4 * r; }
+    else         { a_ki[r] = t & 31; a_mi[r] = (t >> 5) + 8 * r; }
+    int m = tile_m + a_mi[r];
+    a_mok[r] = m < M;
+    a_moff[r] = a_mok[r] ? off2(m, d.M0, d.am0, d.am1) : 0;
+  }
+  const bool b_nfast = (d.bn0 == 1 && d.N0 > 1) || d.bk0 != 1;
+  int b_ni[8], b_ki[8];
+  long long b_noff[8];
+  bool b_nok[8];
+#pragma unroll
+  for (int r = 0; r < 8; ++r) {
+    if (b_nfast) { b_ni[r] = t & 63; b_ki[r] = (t >> 6) + 4 * r; }
+    else         { b_ki[r] = t & 31; b_ni[r] = (t >> 5) + 8 * r; }
+    int n = tile_n + b_ni[r];
+    b_nok[r] = n < N;
+    b_noff[r] = b_nok[r] ? off2(n, d.N0, d.bn0, d.bn1) : 0;
+  }
+  float ra[8], rb[8];
+  auto gload = [&](int k0) {
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+      int k = k0 + a_ki[r];
+      float v = 0.f;
+      if (a_mok[r] && k < K) v = A[a_moff[r] + off2(k, d.K0, d.ak0, d.ak1)];
+      ra[r] = v;
+      int kb = k0 + b_ki[r];
+      float w = 0.f;
+      if (b_nok[r] && kb < K) w = B[b_noff[r] + off2(kb, d.K0, d.bk0, d.bk1)];
+      rb[r] = w;
+    }
+  };
+
+  f4 acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) acc[i][j] = f4{0, 0, 0, 0};
+
+  gload(0);
+  for (int k0 = 0; k0 < K; k0 += BK) {
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+      As[a_mi[r]][a_ki[r]] = f2bf(ra[r]);
+      Bs[b_ni[r]][b_ki[r]] = f2bf(rb[r]);
+    }
+    __syncthreads();
+    if (k0 + BK < K) gload(k0 + BK);
+#pragma unroll
+    for (int kk = 0; kk < BK; kk += 16) {
+      s4 af[2], bf[2];
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+        af[i] = *reinterpret_cast<const s4*>(&As[wm * 32 + i * 16 + (lane & 15)][kk + 4 * (lane >> 4)]);
+#pragma unroll
+      for (int j = 0; j < 2; ++j)
+        bf[j] = *reinterpret_cast<const s4*>(&Bs[wn * 32 + j * 16 + (lane & 15)][kk + 4 * (lane >> 4)]);
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(af[i], bf[j], acc[i][j], 0, 0, 0);
+    }
+  }
+  const float alpha = (float)d.alpha, beta = (float)d.beta;
+#pragma unroll
+  for (int j = 0; j < 2; ++j) {
+    int n = tile_n + wn * 32 + j * 16 + (lane & 15);
+    if (n >= N) continue;
+    long long noff = off2(n, d.N0, d.cn0, d.cn1);
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        int m = tile_m + wm * 32 + i * 16 + 4 * (lane >> 4) + r;
+        if (m >= M) continue;
+        long long o = noff + off2(m, d.M0, d.cm0, d.cm1);
+        float v = alpha * acc[i][j][r];
+        if (d.beta != 0.0) v += beta * C[o];
+        C[o] = v;
+      }
+    }
+  }
+}
+
+}  // namespace
+
+void tc_launch(Prec p, const TcDesc& d, const void* A, const void* B, void* C, hipStream_t s) {
+  const long long M = (long long)d.M0 * d.M1, N = (long long)d.N0 * d.N1;
+  if (M <= 0 || N <= 0 || d.batch <= 0) return;
+  if ((long long)d.K0 * d.K1 <= 0) {
+    throw Error(TNAD_ERR_ARG, "tc_launch: empty contraction index");
+  }
+  if (d.M0 <= 0 || d.M1 <= 0 || d.N0 <= 0 || d.N1 <= 0 || d.K0 <= 0 || d.K1 <= 0 ||
+      M > (1LL << 30) || N > (1LL << 30) || d.batch > 65535)
+    throw Error(TNAD_ERR_ARG, "tc_launch: extents out of range");
+  dim3 grid((unsigned)((M + BM - 1) / BM), (unsigned)((N + BN - 1) / BN), (unsigned)d.batch);
+  if (grid.y > 65535) throw Error(TNAD_ERR_ARG, "tc_launch: N too large");
+  switch (p) {
+    case Prec::F64:
+      hipLaunchKernelGGL(tc_kernel<double>, grid, dim3(NT), 0, s, d, (const double*)A,
+                         (const double*)B, (double*)C);
+      break;
+    case Prec::F32:
+      hipLaunchKernelGGL(tc_kernel<float>, grid, dim3(NT), 0, s, d, (const float*)A,
+                         (const float*)B, (float*)C);
+      break;
+    case Prec::BF16:
+      hipLaunchKernelGGL(tc_kernel_bf16, grid, dim3(NT), 0, s, d, (const float*)A,
+                         (const float*)B, (float*)C);
+      break;
+  }
+  TNAD_HIP_TRY(hipGetLastError());
+}
+
+}  // namespace tnad

@@ -1,0 +1,137 @@
+// Internal declarations shared by the HIP translation units of libtnad.
+// Layout convention everywhere: column-major (first index fastest), the
+// memory order of Julia arrays, so buffers cross the C ABI without transposes.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstddef>
+#include <cstdint>
+#include <string>
+#include <vector>
+#include <map>
+
+#include "tnad.h"
+
+#define TNAD_HIP_TRY(expr)                                                          \
+  do {                                                                              \
+    hipError_t _e = (expr);                                                         \
+    if (_e != hipSuccess) {                                                         \
+      throw tnad::Error(TNAD_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(_e)); \
+    }                                                                               \
+  } while (0)
+
+namespace tnad {
+
+struct Error {
+  int code;
+  std::string msg;
+  Error(int c, std::string m) : code(c), msg(std::move(m)) {}
+};
+
+// ---------------------------------------------------------------------------
+// Device buffer with RAII; allocations go through the context's cache.
+// ---------------------------------------------------------------------------
+struct Ctx;
+
+struct DevBuf {
+  void* ptr = nullptr;
+  size_t bytes = 0;
+  Ctx* owner = nullptr;
+  DevBuf() = default;
+  DevBuf(Ctx* c, size_t b);
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  DevBuf(DevBuf&& o) noexcept { *this = std::move(o); }
+  DevBuf& operator=(DevBuf&& o) noexcept;
+  ~DevBuf();
+  template <typename T> T* as() const { return reinterpret_cast<T*>(ptr); }
+};
+
+// ---------------------------------------------------------------------------
+// Generic strided contraction  C[m,n] = alpha * sum_k A[m,k] B[k,n] + beta * C[m,n]
+// Each of m, n, k is a pair of sub-indices (x0 fastest) with arbitrary element
+// strides, so every einsum of the hot path is one launch with no permute pass.
+// ---------------------------------------------------------------------------
+struct TcDesc {
+  int M0 = 1, M1 = 1, N0 = 1, N1 = 1, K0 = 1, K1 = 1;
+  long long am0 = 0, am1 = 0, ak0 = 0, ak1 = 0;
+  long long bk0 = 0, bk1 = 0, bn0 = 0, bn1 = 0;
+  long long cm0 = 0, cm1 = 0, cn0 = 0, cn1 = 0;
+  int batch = 1;
+  long long abat = 0, bbat = 0, cbat = 0;
+  double alpha = 1.0, beta = 0.0;
+};
+
+// Plain column-major GEMM helper: C(MxN) = alpha op(A) op(B) + beta C
+TcDesc gemm_desc(char ta, char tb, int M, int N, int K, long long lda, long long ldb, long long ldc,
+                 double alpha = 1.0, double beta = 0.0);
+
+enum class Prec { F64 = TNAD_PREC_FP64, F32 = TNAD_PREC_FP32, BF16 = TNAD_PREC_BF16 };
+
+// A, B, C in double (F64) or float (F32 / BF16: BF16 rounds operands on the way into LDS).
+void tc_launch(Prec p, const TcDesc& d, const void* A, const void* B, void* C, hipStream_t s);
+
+// ---------------------------------------------------------------------------
+// Small dense kernels (eig.hip)
+// ---------------------------------------------------------------------------
+constexpr int JACOBI_MAX = 192;   // packed FP64 upper triangle of 192x192 fits in LDS
+
+// Full symmetric eigendecomposition of an m x m matrix (m <= JACOBI_MAX).
+// Output eigenpairs sorted by |lambda| descending; V column-major (ld = m).
+// Vwork: m*m doubles of scratch.  sweeps_out: device int (nullable).
+void jacobi_eig_ws(const double* H, int ldh, int m, double* evals, double* V, double* Vwork,
+                   int max_sweeps, int* sweeps_out, hipStream_t s);
+
+// Cholesky of the m x m SPD matrix G (+ shift on the diagonal), returns Linv^T
+// (upper triangular, column-major) so that X = Y * Rinv with Rinv = L^-T.
+void chol_inv(const double* G, int m, double shift, double* Rinv, int* info, hipStream_t s);
+
+// ---------------------------------------------------------------------------
+// Elementwise / reduction kernels (elem.hip)
+// ---------------------------------------------------------------------------
+void k_sym_add(const double* X, int n, double* Y, hipStream_t s);              // Y = X + X^T
+template <typename T> void k_cast(const T* src, double* dst, size_t n, hipStream_t s);
+template <typename T> void k_cast_from(const double* src, T* dst, size_t n, hipStream_t s);
+void k_fill(double* x, size_t n, double v, hipStream_t s);
+void k_copy(const double* x, double* y, size_t n, hipStream_t s);
+void k_axpby(double a, const double* x, double b, double* y, size_t n, hipStream_t s); // y = a x + b y
+void k_scale_by_dev(double* x, size_t n, const double* nrm, hipStream_t s);   // x /= *nrm
+void k_sumsq(const double* x, size_t n, double* out, hipStream_t s);           // *out = sum x^2 (atomic-free 2 pass)
+void k_dot(const double* x, const double* y, size_t n, double* out, hipStream_t s);
+void k_edge_sym(const double* T1, int chi, int D, double* T2, hipStream_t s);  // T2 = T1 + T1^{kji}
+void k_col_norms(const double* X, int n, int k, double* nrm, hipStream_t s);
+void k_scale_cols(double* X, int n, int k, const double* scale, int invert, hipStream_t s);
+// refill (with hash-random entries) every column whose norm is <= rel * max norm
+void k_random_fill_cols(double* X, int n, int k, const double* nrm, double rel, unsigned seed,
+                        hipStream_t s);
+void k_random_matrix(double* X, size_t n, unsigned seed, hipStream_t s);
+void k_init_raw(const double* a, int D, int chi, double* C, double* T, hipStream_t s);
+void k_symmetrize_bulk(const double* x, int d, int ph, double* y, hipStream_t s);
+void k_symmetrize_bulk_back2(const double* g, const double* x_orig, int d, int ph, double* out,
+                             hipStream_t s);
+void k_double_layer(const double* bulk, int d, int ph, double* ap, double* a, hipStream_t s);
+void k_double_layer_back(const double* bulk, int d, int ph, const double* apbar, const double* abar,
+                         double* bbar, hipStream_t s);
+
+// ---------------------------------------------------------------------------
+// Context
+// ---------------------------------------------------------------------------
+struct Ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string last_error;
+  std::multimap<size_t, void*> free_blocks;   // size -> ptr cache
+  size_t cached_bytes = 0;
+  void* alloc(size_t bytes);
+  void release(void* p, size_t bytes);
+  void trim();
+  DevBuf buf(size_t bytes) { return DevBuf(this, bytes); }
+  template <typename T> DevBuf bufn(size_t n) { return DevBuf(this, n * sizeof(T)); }
+  void sync() { TNAD_HIP_TRY(hipStreamSynchronize(stream)); }
+  ~Ctx();
+};
+
+// host <-> device helpers that accept either host or device pointers
+void copy_in(Ctx* c, void* dst_dev, const void* src_any, size_t bytes);
+void copy_out(Ctx* c, void* dst_any, const void* src_dev, size_t bytes);
+
+}  // namespace tnad

@@ -1,0 +1,232 @@
+"""Pins of the CPU oracle (oracle/tnad_oracle.py) against the reference's own
+golden values, closed forms, finite differences and invariants.  These run on
+the CPU only; they establish that the oracle the GPU is later compared against
+restates the reference correctly."""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import tnad_oracle as O
+
+
+# --- TRG: golden scalars held by the reference -----------------------------------------
+def test_trg_tensorgrad_golden():
+    # test/trg.jl:12-18 (value agreed with the PyTorch tensorgrad project)
+    assert O.trg(O.model_tensor_ising(0.4), 5, 5) == pytest.approx(0.8919788686747141, rel=1e-13, abs=0)
+
+
+def test_trg_readme_golden_value_and_gradient():
+    # README.md:58-60 and README.md:67-70 (chi=20, niter=20, beta=0.5)
+    lnz, g = O.trg_and_grad(0.5, 20, 20)
+    assert lnz == pytest.approx(1.0257933734351765, rel=1e-13, abs=0)
+    assert g == pytest.approx(1.7455677143228514, rel=1e-12, abs=0)
+
+
+def test_trg_userguide_gradient():
+    # docs/src/userguide.md:25-28
+    _, g = O.trg_and_grad(0.5, 5, 5)
+    assert g == pytest.approx(1.7502426939979507, rel=1e-13, abs=0)
+
+
+def test_trg_gradient_matches_num_grad():
+    # test/trg.jl:19
+    f = lambda b: O.trg(O.model_tensor_ising(b), 5, 5)
+    _, g = O.trg_and_grad(0.4, 5, 5)
+    assert O.num_grad(f, 0.4, 1e-6) == pytest.approx(g, rel=1e-7)
+
+
+def test_trg_onsager_closed_form():
+    # independent pin: Onsager's exact free energy (chi=20 TRG is within 1e-6 of it)
+    assert O.onsager_lnz(0.5) == pytest.approx(1.0257928126949176, abs=5e-14)
+    assert O.onsager_lnz(0.4) == pytest.approx(0.87936382077494819, abs=5e-14)
+    lnz, g = O.trg_and_grad(0.5, 20, 20)
+    assert abs(lnz - O.onsager_lnz(0.5)) < 1e-6
+    dexact = O.num_grad(O.onsager_lnz, 0.5, 1e-4)
+    assert abs(g - dexact) < 1e-5
+
+
+def test_trg_svd_reconstruction():
+    # test/trg.jl:6-10: at tol=0 the split reproduces the tensor
+    rng = np.random.default_rng(0)
+    t = rng.standard_normal((10, 10, 10, 10))
+    u, v = O.trg_svd(t, 100, 0)
+    assert np.allclose(np.einsum("ija,akl->ijkl", u, v), t, atol=1e-10)
+
+
+def test_trg_svd_keep_count_off_by_one():
+    # src/trg.jl:37: k = min(#{s > tol} + 1, dmax, len(s))
+    rng = np.random.default_rng(1)
+    a = rng.standard_normal((4, 2))
+    b = rng.standard_normal((2, 4))
+    t = np.reshape(a @ b, (2, 2, 2, 2), order="F")      # rank 2 of 4
+    u, v = O.trg_svd(t, 10, 1e-8)
+    assert u.shape[2] == 3
+    u, v = O.trg_svd(t, 2, 1e-8)
+    assert u.shape[2] == 2
+
+
+def test_tensorfromclassical_matches_model_tensor():
+    # test/exampletensors.jl:6-9
+    b = 0.37
+    assert np.allclose(O.model_tensor_ising(b), O.tensorfromclassical(np.array([[b, -b], [-b, b]])))
+
+
+def test_dmodel_tensor_is_derivative():
+    b, h = 0.45, 1e-6
+    fd = (O.model_tensor_ising(b + h / 2) - O.model_tensor_ising(b - h / 2)) / h
+    assert np.allclose(O.dmodel_tensor_ising(b), fd, atol=1e-8)
+
+
+# --- svd_back: central FD on small random matrices (test/svd.jl pattern) ---------------
+@pytest.mark.parametrize("shape", [(5, 5), (6, 6)])
+def test_svd_back_matches_fd(shape):
+    rng = np.random.default_rng(3)
+    A = rng.standard_normal(shape)
+    H = rng.standard_normal((shape[0], shape[0]))
+    H = H + H.T
+    k = 2
+
+    def loss(M):
+        U, s, Vh = np.linalg.svd(M)
+        # gauge-invariant use of the first k left vectors and all singular values
+        P = U[:, :k] @ U[:, :k].T
+        return np.sum(P * H) + np.sum(s[:k] ** 2)
+
+    U, s, Vh = np.linalg.svd(A)
+    dU = np.zeros_like(U)
+    dU[:, :k] = 2 * H @ U[:, :k]          # d/dU of tr(U_k U_k^T H)
+    dS = np.zeros_like(s)
+    dS[:k] = 2 * s[:k]
+    g = O.svd_back(U, s, Vh.T, dU, dS, None)
+    gn = O.num_grad(loss, A, 1e-6)
+    assert np.allclose(g, gn, atol=1e-6)
+
+
+def test_svd_back_all_nothing_returns_none():
+    # src/trg.jl:73
+    U = np.eye(3)
+    assert O.svd_back(U, np.ones(3), U, None, None, None) is None
+
+
+# --- models -----------------------------------------------------------------------
+def test_heisenberg_matches_readme_print():
+    # README.md:83-100 prints h[:, :, k, l] blocks of hamiltonian(Heisenberg())
+    h = O.hamiltonian_heisenberg()
+    assert np.allclose(h[:, :, 0, 0], [[-0.5, 0.0], [0.0, 0.5]])
+    assert np.allclose(h[:, :, 1, 0], [[0.0, 0.0], [-1.0, 0.0]])
+    assert np.allclose(h[:, :, 0, 1], [[0.0, -1.0], [0.0, 0.0]])
+    assert np.allclose(h[:, :, 1, 1], [[0.5, 0.0], [0.0, -0.5]])
+
+
+def test_symmetrize_is_d4_invariant_and_normalised():
+    rng = np.random.default_rng(4)
+    x = O.indexperm_symmetrize(rng.random((2, 2, 2, 2, 2)))
+    assert np.linalg.norm(x) == pytest.approx(1.0)
+    for p in [(0, 3, 2, 1, 4), (2, 1, 0, 3, 4), (1, 0, 3, 2, 4), (3, 2, 1, 0, 4), (1, 2, 3, 0, 4)]:
+        assert np.allclose(np.transpose(x, p), x)
+
+
+def test_symmetrize_dimension_mismatch():
+    # test/ctmrg.jl:13
+    with pytest.raises(ValueError):
+        O.indexperm_symmetrize(np.zeros((3, 3, 4, 3, 2)))
+
+
+# --- CTMRG / energy -------------------------------------------------------------------
+def _rand_bulk(seed, d=2, s=2, dist="normal"):
+    return O.symmetrized_random_bulk(d, s, seed, dist)
+
+
+def test_ctmrgstep_invariants():
+    b = O.indexperm_symmetrize(_rand_bulk(0))
+    ap, a = O.double_layer(b)
+    C, T = O.init_raw(a, 6)
+    for _ in range(4):
+        C, T, vals = O.ctmrgstep(a, C, T)
+        assert np.allclose(C, C.T)
+        assert np.allclose(T, np.transpose(T, (2, 1, 0)))
+        assert np.linalg.norm(C) == pytest.approx(1.0)
+        assert np.linalg.norm(T) == pytest.approx(1.0)
+        assert vals[0] == 1.0 and np.all(np.diff(vals) <= 1e-15)
+
+
+def test_stopfunction_semantics():
+    # src/fixedpoint.jl:31-41: maxit bounds the steps exactly; the first two states never stop
+    b = O.indexperm_symmetrize(_rand_bulk(1))
+    ap, a = O.double_layer(b)
+    C, T = O.init_raw(a, 4)
+    assert O.ctmrg(a, C, T, tol=1e9, maxit=100).iters == 2
+    assert O.ctmrg(a, C, T, tol=0.0, maxit=7).iters == 7
+    assert O.ctmrg(a, C, T, tol=0.0, maxit=0).iters == 0
+
+
+def test_product_state_energies():
+    # test/variationalipeps.jl:14-25
+    h = O.diaglocalhamiltonian([1.0, -1.0])
+    rng = np.random.default_rng(5)
+    a = np.zeros((2, 2, 2, 2, 2)) + 1e-12 * rng.standard_normal((2, 2, 2, 2, 2))
+    a[0, 0, 0, 0, 1] = rng.standard_normal()
+    assert O.energy(h, a, 4, 1e-12, 100) / 2 == pytest.approx(-1.0)
+    a = np.zeros((2, 2, 2, 2, 2)) + 1e-12 * rng.standard_normal((2, 2, 2, 2, 2))
+    a[0, 0, 0, 0, 0] = rng.standard_normal()
+    assert O.energy(h, a, 10, 0.0, 300) / 2 == pytest.approx(1.0)
+    a = np.zeros((2, 2, 2, 2, 2)) + 1e-12 * rng.standard_normal((2, 2, 2, 2, 2))
+    a[0, 0, 0, 0, 1] = a[0, 0, 0, 0, 0] = rng.standard_normal()
+    assert abs(O.energy(h, a, 10, 0.0, 300)) < 1e-9
+
+
+def test_energy_bounded_noninteracting():
+    # test/variationalipeps.jl:10-12
+    h = O.diaglocalhamiltonian([1.0, -1.0])
+    rng = np.random.default_rng(6)
+    for _ in range(5):
+        e = O.energy(h, rng.random((3, 3, 3, 3, 2)), 5, 0.0, 10) / 2
+        assert -1 < e < 1
+
+
+def test_energy_gauge_and_scale_invariance():
+    h = O.hamiltonian_heisenberg()
+    b = _rand_bulk(7)
+    e0 = O.energy(h, b, 6, 1e-12, 100)
+    assert O.energy(h, 3.7 * b, 6, 1e-12, 100) == pytest.approx(e0, abs=1e-12)
+    assert O.energy(h, -b, 6, 1e-12, 100) == pytest.approx(e0, abs=1e-12)
+
+
+@pytest.mark.parametrize("seed,chi,maxit", [(8, 4, 0), (8, 4, 1), (8, 4, 5), (9, 6, 3), (10, 8, 6)])
+def test_energy_gradient_matches_num_grad_frozen_init(seed, chi, maxit):
+    # the reverse sweep is exact for the function Zygote differentiates: the :raw
+    # initial environment is @nograd (src/autodiff.jl:5), so FD holds it fixed.
+    # chi=6, 8 exceed the rank of the first corner matrix (null-space columns kept).
+    h = O.hamiltonian_heisenberg()
+    b = _rand_bulk(seed)
+    env0 = O.raw_env_of(b, chi)
+    e, g, it = O.energy_and_grad(h, b, chi, 0.0, maxit)
+    gn = O.num_grad(lambda x: O.energy_frozen_init(h, x, chi, 0.0, maxit, env0), b, 1e-5)
+    assert np.max(np.abs(g - gn)) < 2e-8 * max(1.0, np.max(np.abs(g)))
+
+
+def test_energy_gradient_approaches_num_grad_at_convergence():
+    # test/variationalipeps.jl:121-134: with many iterations the init's influence is gone
+    # and the gradient matches FD of the full energy (atol 1e-3 in the reference).
+    h = O.hamiltonian_heisenberg()
+    b = _rand_bulk(8)
+    e, g, it = O.energy_and_grad(h, b, 4, 0.0, 60)
+    gn = O.num_grad(lambda x: O.energy(h, x, 4, 0.0, 60), b, 1e-5)
+    assert np.max(np.abs(g - gn)) < 1e-8
+
+
+def test_ctmrg_yang_magnetisation():
+    # test/ctmrg.jl:35-42 (random init from NumPy's generator)
+    rng = np.random.default_rng(5)
+    assert O.magnetisation(0.0, 2, rng) == pytest.approx(O.magofbeta(0.0), abs=1e-6)
+    assert O.magnetisation(1.0, 2, rng) == pytest.approx(O.magofbeta(1.0), rel=1e-6)
+    assert O.magnetisation(0.2, 10, rng) == pytest.approx(O.magofbeta(0.2), abs=1e-4)
+    assert O.magnetisation(0.6, 4, rng) == pytest.approx(O.magofbeta(0.6), rel=1e-6)
+    assert O.magnetisation(0.8, 2, rng) == pytest.approx(O.magofbeta(0.8), rel=1e-6)
+
+
+def test_num_grad_convention():
+    # src/autodiff.jl:39-42 doctest
+    assert O.num_grad(lambda x: x * x, 3.0) == pytest.approx(6.0)
+    assert np.allclose(O.num_grad(np.trace, np.random.default_rng(0).random((2, 2))), np.eye(2))
