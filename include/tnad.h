@@ -136,6 +136,20 @@ int tnad_trg(tnad_ctx* ctx, const double* a, int q, int chi, int niter, double t
 int tnad_trg_svd(tnad_ctx* ctx, const double* t, int d1, int d2, int d3, int d4, int dmax, double tol,
                  double* u, double* v, int* k_out);
 
+/* One strided contraction C = A * B given as an einsum over single-character index names
+ * (e.g. "ibcl", "jkcb" -> "ijlk"; the contraction kernel behind every ein"..." call site of the
+ * step).  Tensors are contiguous column-major FP64 with the listed extents; precision selects the
+ * matrix-core path (FP64, FP32, or BF16 operands with FP32 accumulation). */
+int tnad_contract(tnad_ctx* ctx, int precision, const char* ia, const long long* da, const double* A,
+                  const char* ib, const long long* db, const double* B, const char* ic, const long long* dc,
+                  double alpha, double beta, double* Cout);   /* Cout = alpha A*B + beta Cout */
+
+/* Average device time of one launch of the contraction described as in tnad_contract, measured with
+ * HIP events recorded on the context's stream around `reps` back-to-back launches on device-resident
+ * operands (no transfers inside the timed region).  For roofline measurement. */
+int tnad_time_contract(tnad_ctx* ctx, int precision, const char* ia, const long long* da, const char* ib,
+                       const long long* db, const char* ic, const long long* dc, int reps, double* avg_ms);
+
 #ifdef __cplusplus
 }
 #endif

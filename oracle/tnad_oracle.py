@@ -396,10 +396,21 @@ def ctmrg(a, C0, T0, tol, maxit, keep=False):
     return res
 
 
+def _half_plaquette(C, T, apn):
+    """l = ein"ab,ica,bde,cjfdlm,eg,gfk -> ijklm"(C, T, T, apn, C, T) (src/variationalipeps.jl:52),
+    evaluated pairwise: E1 = T C, E2 = E1 T, E3 = C T, E4 = E2 apn, l = E4 E3."""
+    E1 = ein("ica,ab->icb", T, C)
+    E2 = ein("icb,bde->icde", E1, T)
+    E3 = ein("eg,gfk->efk", C, T)
+    E4 = ein("icde,cjfdlm->iejflm", E2, apn)
+    l = ein("iejflm,efk->ijklm", E4, E3)
+    return l, (E1, E2, E3, E4)
+
+
 def expectationvalue(h, ap, C, T):
     """src/variationalipeps.jl:49-56."""
     apn = ap / np.linalg.norm(ap)
-    l = ein("ab,ica,bde,cjfdlm,eg,gfk->ijklm", C, T, T, apn, C, T)
+    l, _ = _half_plaquette(C, T, apn)
     e = ein("abcij,abckl,ijkl->", l, l, h)
     nn = ein("ijkaa,ijkbb->", l, l)
     return e / nn
@@ -409,7 +420,7 @@ def expectationvalue_back(h, ap, C, T, Ebar=1.0):
     """Reverse sweep of expectationvalue.  Returns (apbar, Cbar, Tbar)."""
     nrm = np.linalg.norm(ap)
     apn = ap / nrm
-    l = ein("ab,ica,bde,cjfdlm,eg,gfk->ijklm", C, T, T, apn, C, T)
+    l, (E1, E2, E3, E4) = _half_plaquette(C, T, apn)
     e = ein("abcij,abckl,ijkl->", l, l, h)
     tr = ein("ijkaa->ijk", l)
     nn = np.sum(tr * tr)
@@ -418,16 +429,22 @@ def expectationvalue_back(h, ap, C, T, Ebar=1.0):
     lbar = ebar * (ein("abckl,ijkl->abcij", l, h) + ein("abcij,ijkl->abckl", l, h))
     s = l.shape[3]
     lbar = lbar + 2 * nnbar * ein("ijk,ab->ijkab", tr, np.eye(s))
-    spec_in = ["ab", "ica", "bde", "cjfdlm", "eg", "gfk"]
-    ops = [C, T, T, apn, C, T]
-    grads = []
-    for p in range(6):
-        others = [spec_in[q] for q in range(6) if q != p]
-        oops = [ops[q] for q in range(6) if q != p]
-        grads.append(ein(",".join(others + ["ijklm"]) + "->" + spec_in[p], *oops, lbar))
-    Cbar = grads[0] + grads[4]
-    Tbar = grads[1] + grads[2] + grads[5]
-    apbar = _norm_back(ap, grads[3])
+    # l = E4 E3
+    E4bar = ein("ijklm,efk->iejflm", lbar, E3)
+    E3bar = ein("iejflm,ijklm->efk", E4, lbar)
+    # E4 = E2 apn
+    E2bar = ein("iejflm,cjfdlm->icde", E4bar, apn)
+    apnbar = ein("icde,iejflm->cjfdlm", E2, E4bar)
+    # E3 = C T
+    Cbar = ein("efk,gfk->eg", E3bar, T)
+    Tbar = ein("eg,efk->gfk", C, E3bar)
+    # E2 = E1 T
+    E1bar = ein("icde,bde->icb", E2bar, T)
+    Tbar = Tbar + ein("icb,icde->bde", E1, E2bar)
+    # E1 = T C
+    Tbar = Tbar + ein("icb,ab->ica", E1bar, C)
+    Cbar = Cbar + ein("ica,icb->ab", T, E1bar)
+    apbar = _norm_back(ap, apnbar)
     return apbar, Cbar, Tbar
 
 

@@ -23,6 +23,7 @@ void* Ctx::alloc(size_t bytes) {
     void* p = it->second;
     free_blocks.erase(it);
     cached_bytes -= bytes;
+    live[(uintptr_t)p] = bytes;
     return p;
   }
   void* p = nullptr;
@@ -36,12 +37,24 @@ void* Ctx::alloc(size_t bytes) {
       throw Error(TNAD_ERR_ALLOC, "hipMalloc of " + std::to_string(bytes) + " bytes failed");
     }
   }
+  live[(uintptr_t)p] = bytes;
   return p;
+}
+void Ctx::check_range(const void* p, size_t bytes, const char* what) const {
+  const uintptr_t a = (uintptr_t)p;
+  auto it = live.upper_bound(a);
+  if (it == live.begin()) throw Error(TNAD_ERR_ARG, std::string(what) + ": operand is not a live device block");
+  --it;
+  if (a + bytes > it->first + it->second)
+    throw Error(TNAD_ERR_ARG, std::string(what) + ": operand extent " + std::to_string(bytes) +
+                                  " bytes exceeds its device block (" + std::to_string(it->first + it->second - a) +
+                                  " bytes available)");
 }
 void Ctx::release(void* p, size_t bytes) {
   if (!p) return;
   if (bytes == 0) bytes = 8;
   bytes = (bytes + 255) & ~size_t(255);
+  live.erase((uintptr_t)p);
   free_blocks.emplace(bytes, p);
   cached_bytes += bytes;
   if (cached_bytes > (size_t(32) << 30)) trim();
@@ -524,6 +537,83 @@ int tnad_trg(tnad_ctx* ctx, const double* a, int q, int chi, int niter, double t
 int tnad_trg_svd(tnad_ctx* ctx, const double* t, int d1, int d2, int d3, int d4, int dmax, double tol, double* u,
                  double* v, int* k_out) {
   return guarded(ctx, [&] { trg_svd_run(&ctx->c, t, d1, d2, d3, d4, dmax, tol, u, v, k_out); });
+}
+
+int tnad_contract(tnad_ctx* ctx, int precision, const char* ia, const long long* da, const double* A, const char* ib,
+                  const long long* db, const double* B, const char* ic, const long long* dc, double alpha, double beta,
+                  double* Cout) {
+  return guarded(ctx, [&] {
+    if (!ia || !ib || !ic || !da || !db || !dc) throw Error(TNAD_ERR_ARG, "contract: null argument");
+    Ctx* c = &ctx->c;
+    auto view = [](const char* idx, const long long* d) {
+      std::string s(idx);
+      std::vector<long long> dims(d, d + s.size());
+      for (long long x : dims)
+        if (x < 1) throw Error(TNAD_ERR_SHAPE, "contract: extents must be positive");
+      return cm(s, dims);
+    };
+    TView va = view(ia, da), vb = view(ib, db), vc = view(ic, dc);
+    auto count = [](const TView& v) {
+      size_t n = 1;
+      for (long long x : v.dim) n *= (size_t)x;
+      return n;
+    };
+    const size_t na = count(va), nb = count(vb), ncount = count(vc);
+    const Prec p = to_prec_enum(precision);
+    TcDesc desc = contract_desc(va, vb, vc, alpha, beta);
+    DevBuf a64(c, na * 8), b64(c, nb * 8), c64(c, ncount * 8);
+    copy_in(c, a64.ptr, A, na * 8);
+    copy_in(c, b64.ptr, B, nb * 8);
+    PBuf ap(c, p, na), bp(c, p, nb), cp(c, p, ncount);
+    to_prec(c, a64.as<double>(), ap);
+    to_prec(c, b64.as<double>(), bp);
+    if (beta != 0.0) {
+      copy_in(c, c64.ptr, Cout, ncount * 8);
+      to_prec(c, c64.as<double>(), cp);
+    }
+    tc_launch(c, p, desc, ap.ptr(), bp.ptr(), cp.ptr(), c->stream);
+    from_prec(c, cp, c64.as<double>());
+    copy_out(c, Cout, c64.ptr, ncount * 8);
+  });
+}
+
+int tnad_time_contract(tnad_ctx* ctx, int precision, const char* ia, const long long* da, const char* ib,
+                       const long long* db, const char* ic, const long long* dc, int reps, double* avg_ms) {
+  return guarded(ctx, [&] {
+    if (!ia || !ib || !ic || !da || !db || !dc || !avg_ms || reps < 1) throw Error(TNAD_ERR_ARG, "time_contract: bad argument");
+    Ctx* c = &ctx->c;
+    auto view = [](const char* idx, const long long* d) {
+      std::string s(idx);
+      return cm(s, std::vector<long long>(d, d + s.size()));
+    };
+    TView va = view(ia, da), vb = view(ib, db), vc = view(ic, dc);
+    auto count = [](const TView& v) {
+      size_t n = 1;
+      for (long long x : v.dim) n *= (size_t)x;
+      return n;
+    };
+    const Prec p = to_prec_enum(precision);
+    TcDesc desc = contract_desc(va, vb, vc);
+    DevBuf a64(c, count(va) * 8), b64(c, count(vb) * 8);
+    k_random_matrix(a64.as<double>(), count(va), 11u, c->stream);
+    k_random_matrix(b64.as<double>(), count(vb), 13u, c->stream);
+    PBuf ap(c, p, count(va)), bp(c, p, count(vb)), cp(c, p, count(vc));
+    to_prec(c, a64.as<double>(), ap);
+    to_prec(c, b64.as<double>(), bp);
+    tc_launch(c, p, desc, ap.ptr(), bp.ptr(), cp.ptr(), c->stream);   // warm-up launch
+    hipEvent_t e0, e1;
+    TNAD_HIP_TRY(hipEventCreate(&e0));
+    TNAD_HIP_TRY(hipEventCreate(&e1));
+    TNAD_HIP_TRY(hipEventRecord(e0, c->stream));
+    for (int r = 0; r < reps; ++r) tc_launch(c, p, desc, ap.ptr(), bp.ptr(), cp.ptr(), c->stream);
+    TNAD_HIP_TRY(hipEventRecord(e1, c->stream));
+    TNAD_HIP_TRY(hipEventSynchronize(e1));
+    float ms = 0.f;
+    TNAD_HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *avg_ms = (double)ms / reps;
+  });
 }
 
 }  // extern "C"

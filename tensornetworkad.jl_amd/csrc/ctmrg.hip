@@ -17,6 +17,8 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 
 namespace tnad {
@@ -198,9 +200,38 @@ void from_prec(Ctx* c, const PBuf& src, double* dst) {
   else k_cast<float>(src.b.as<float>(), dst, src.n, c->stream);
 }
 
+// src at precision p: the buffer itself when the storage type already matches,
+// otherwise a converted copy held in tmp (FP64 <-> float storage)
+const PBuf& at_prec(Ctx* c, const PBuf& src, Prec p, PBuf& tmp) {
+  const bool src64 = src.p == Prec::F64, want64 = p == Prec::F64;
+  if (src64 == want64) return src;
+  tmp = PBuf(c, p, src.n);
+  if (src64) {
+    to_prec(c, src.b.as<double>(), tmp);
+  } else {
+    k_cast<float>(src.b.as<float>(), tmp.b.as<double>(), src.n, c->stream);
+  }
+  return tmp;
+}
+
 void tc(Ctx* c, Prec p, const TView& A, const void* a, const TView& B, const void* b, const TView& C, void* o,
         double alpha, double beta) {
-  tc_launch(p, contract_desc(A, B, C, alpha, beta), a, b, o, c->stream);
+  tc_launch(c, p, contract_desc(A, B, C, alpha, beta), a, b, o, c->stream);
+}
+
+// Opt-in diagnostics: with TNAD_DEBUG_DUMP=<dir> set, named FP64 device arrays are
+// written as raw little-endian files <dir>/<name>.bin (synchronises the stream).
+static void debug_dump(Ctx* c, const char* name, const double* dev, size_t n) {
+  const char* dir = std::getenv("TNAD_DEBUG_DUMP");
+  if (!dir || !*dir) return;
+  std::vector<double> h(n);
+  TNAD_HIP_TRY(hipMemcpyAsync(h.data(), dev, n * 8, hipMemcpyDeviceToHost, c->stream));
+  c->sync();
+  std::string path = std::string(dir) + "/" + name + ".bin";
+  if (FILE* f = std::fopen(path.c_str(), "wb")) {
+    std::fwrite(h.data(), 8, n, f);
+    std::fclose(f);
+  }
 }
 
 // ==========================================================================
@@ -219,9 +250,9 @@ static void orthonormalize(Ctx* c, double* Y, int n, int k, double* X, EigWork& 
   double* src = Y;
   double* dst = X;
   for (int pass = 0; pass < 3; ++pass) {
-    tc_launch(Prec::F64, gemm_desc('T', 'N', k, k, n, n, n, k), src, src, w.G.as<double>(), s);
+    tc_launch(c, Prec::F64, gemm_desc('T', 'N', k, k, n, n, n, k), src, src, w.G.as<double>(), s);
     chol_inv(w.G.as<double>(), k, pass == 0 ? shift : 0.0, w.Rinv.as<double>(), w.info.as<int>(), s);
-    tc_launch(Prec::F64, gemm_desc('N', 'N', n, k, k, n, k, n), src, w.Rinv.as<double>(), dst, s);
+    tc_launch(c, Prec::F64, gemm_desc('N', 'N', n, k, k, n, k, n), src, w.Rinv.as<double>(), dst, s);
     std::swap(src, dst);
   }
   // three passes alternate Y -> X -> Y -> X: the orthonormal block ends in X (Y is scratch)
@@ -239,7 +270,7 @@ int topk_eig(Ctx* c, const double* A, int n, int want, int k, const double* X0, 
     // full spectrum: exact reference semantics (all n singular values / vectors)
     DevBuf Vw(c, (size_t)n * n * 8);
     jacobi_eig_ws(A, n, n, theta, X, Vw.as<double>(), 60, nullptr, s);
-    tc_launch(Prec::F64, gemm_desc('N', 'N', n, n, n, n, n, n), A, X, AX, s);
+    tc_launch(c, Prec::F64, gemm_desc('N', 'N', n, n, n, n, n, n), A, X, AX, s);
     if (st) { st->iters = 1; st->k = n; st->resid = 0.0; }
     return n;
   }
@@ -257,11 +288,11 @@ int topk_eig(Ctx* c, const double* A, int n, int want, int k, const double* X0, 
   double rel = 0.0;
   for (it = 1; it <= maxit; ++it) {
     // Y = A X ; H = X^T Y ; H = W diag(theta) W^T ; Xr = X W ; Yr = Y W
-    tc_launch(Prec::F64, gemm_desc('N', 'N', n, k, n, n, n, n), A, X, w.Y.as<double>(), s);
-    tc_launch(Prec::F64, gemm_desc('T', 'N', k, k, n, n, n, k), X, w.Y.as<double>(), w.H.as<double>(), s);
+    tc_launch(c, Prec::F64, gemm_desc('N', 'N', n, k, n, n, n, n), A, X, w.Y.as<double>(), s);
+    tc_launch(c, Prec::F64, gemm_desc('T', 'N', k, k, n, n, n, k), X, w.Y.as<double>(), w.H.as<double>(), s);
     jacobi_eig_ws(w.H.as<double>(), k, k, theta, w.W.as<double>(), w.Vw.as<double>(), 60, nullptr, s);
-    tc_launch(Prec::F64, gemm_desc('N', 'N', n, k, k, n, k, n), X, w.W.as<double>(), w.Xr.as<double>(), s);
-    tc_launch(Prec::F64, gemm_desc('N', 'N', n, k, k, n, k, n), w.Y.as<double>(), w.W.as<double>(),
+    tc_launch(c, Prec::F64, gemm_desc('N', 'N', n, k, k, n, k, n), X, w.W.as<double>(), w.Xr.as<double>(), s);
+    tc_launch(c, Prec::F64, gemm_desc('N', 'N', n, k, k, n, k, n), w.Y.as<double>(), w.W.as<double>(),
               w.Yr.as<double>(), s);
     hipLaunchKernelGGL(resid_norms_kernel, dim3(want), dim3(NT), 0, s, w.Xr.as<double>(), w.Yr.as<double>(),
                        theta, n, w.res.as<double>());
@@ -350,7 +381,7 @@ void ctmrg_step(Ctx* c, const StepDims& d, const StepOpts& o, const double* a64,
   double* C2 = tape ? tape->C2.as<double>() : nullptr;
   DevBuf C2tmp;
   if (!C2) { C2tmp = DevBuf(c, (size_t)chi * chi * 8); C2 = C2tmp.as<double>(); }
-  tc_launch(Prec::F64, gemm_desc('T', 'N', chi, chi, n, n, n, chi), z, AX, C2, s);
+  tc_launch(c, Prec::F64, gemm_desc('T', 'N', chi, chi, n, n, n, chi), z, AX, C2, s);
   // T' = ein"abjcd,abi,dck -> ijk"(tp, z, z), tp = ein"iam,jkla -> ijklm"(T, a)  (src/ctmrg.jl:131,140)
   // evaluated as ((T z) a) z without materialising tp.
   PBuf zp(c, p, (size_t)n * chi);
@@ -423,12 +454,19 @@ void eig_adjoint(Ctx* c, const StepDims& d, const StepTape& tp, const double* G,
   const double* lam = tp.theta.as<double>();
   DevBuf g(c, (size_t)k * chi * 8), coef(c, (size_t)k * chi * 8), W(c, (size_t)n * chi * 8);
   // g = U^T G  (k x chi)
-  tc_launch(Prec::F64, gemm_desc('T', 'N', k, chi, n, n, n, k), X, G, g.as<double>(), s);
+  tc_launch(c, Prec::F64, gemm_desc('T', 'N', k, chi, n, n, n, k), X, G, g.as<double>(), s);
   hipLaunchKernelGGL(eigback_coef_kernel, dim3((k * chi + NT - 1) / NT), dim3(NT), 0, s, g.as<double>(), lam,
                      C1bar, k, chi, 1e-40, coef.as<double>());
   TNAD_HIP_TRY(hipGetLastError());
   // W = U coef  (n x k . k x chi)
-  tc_launch(Prec::F64, gemm_desc('N', 'N', n, chi, k, n, k, n), X, coef.as<double>(), W.as<double>(), s);
+  tc_launch(c, Prec::F64, gemm_desc('N', 'N', n, chi, k, n, k, n), X, coef.as<double>(), W.as<double>(), s);
+  debug_dump(c, "eb_X", X, (size_t)n * k);
+  debug_dump(c, "eb_lam", lam, (size_t)k);
+  debug_dump(c, "eb_G", G, (size_t)n * chi);
+  debug_dump(c, "eb_C1bar", C1bar, (size_t)chi * chi);
+  debug_dump(c, "eb_g", g.as<double>(), (size_t)k * chi);
+  debug_dump(c, "eb_coef", coef.as<double>(), (size_t)k * chi);
+  debug_dump(c, "eb_W0", W.as<double>(), (size_t)n * chi);
   int terms = 0;
   if (k < n) {
     // remainder of the spectrum via the Neumann series
@@ -455,23 +493,23 @@ void eig_adjoint(Ctx* c, const StepDims& d, const StepTape& tp, const double* G,
       TNAD_HIP_TRY(hipMemcpyAsync(is2.ptr, inv_s2.data(), (size_t)chi * 8, hipMemcpyHostToDevice, s));
       // V0 = P G = G - U (U^T G)
       k_copy(G, V.as<double>(), (size_t)n * chi, s);
-      tc_launch(Prec::F64, gemm_desc('N', 'N', n, chi, k, n, k, n, -1.0, 1.0), X, g.as<double>(), V.as<double>(), s);
+      tc_launch(c, Prec::F64, gemm_desc('N', 'N', n, chi, k, n, k, n, -1.0, 1.0), X, g.as<double>(), V.as<double>(), s);
       k_fill(Xr.as<double>(), (size_t)n * chi, 0.0, s);
       for (int m = 0; m < terms; ++m) {
         hipLaunchKernelGGL(neumann_acc_kernel, dim3(nb((size_t)n * chi)), dim3(NT), 0, s, V.as<double>(),
                            Xr.as<double>(), is2.as<double>(), n, chi, m);
         if (m + 1 == terms) break;
         // V <- P A (A V)
-        tc_launch(Prec::F64, gemm_desc('N', 'N', n, chi, n, n, n, n), tp.M.as<double>(), V.as<double>(),
+        tc_launch(c, Prec::F64, gemm_desc('N', 'N', n, chi, n, n, n, n), tp.M.as<double>(), V.as<double>(),
                   V2.as<double>(), s);
-        tc_launch(Prec::F64, gemm_desc('N', 'N', n, chi, n, n, n, n), tp.M.as<double>(), V2.as<double>(),
+        tc_launch(c, Prec::F64, gemm_desc('N', 'N', n, chi, n, n, n, n), tp.M.as<double>(), V2.as<double>(),
                   V.as<double>(), s);
-        tc_launch(Prec::F64, gemm_desc('T', 'N', k, chi, n, n, n, k), X, V.as<double>(), pj.as<double>(), s);
-        tc_launch(Prec::F64, gemm_desc('N', 'N', n, chi, k, n, k, n, -1.0, 1.0), X, pj.as<double>(),
+        tc_launch(c, Prec::F64, gemm_desc('T', 'N', k, chi, n, n, n, k), X, V.as<double>(), pj.as<double>(), s);
+        tc_launch(c, Prec::F64, gemm_desc('N', 'N', n, chi, k, n, k, n, -1.0, 1.0), X, pj.as<double>(),
                   V.as<double>(), s);
       }
       // W += Xr Lambda1 + A Xr
-      tc_launch(Prec::F64, gemm_desc('N', 'N', n, chi, n, n, n, n), tp.M.as<double>(), Xr.as<double>(),
+      tc_launch(c, Prec::F64, gemm_desc('N', 'N', n, chi, n, n, n, n), tp.M.as<double>(), Xr.as<double>(),
                 V2.as<double>(), s);
       hipLaunchKernelGGL(add_lam_cols_kernel, dim3(nb((size_t)n * chi)), dim3(NT), 0, s, W.as<double>(),
                          Xr.as<double>(), V2.as<double>(), lam, n, chi);
@@ -479,8 +517,10 @@ void eig_adjoint(Ctx* c, const StepDims& d, const StepTape& tp, const double* G,
     }
   }
   // cpbar = W U1^T + U1 W^T
-  tc_launch(Prec::F64, gemm_desc('N', 'T', n, n, chi, n, n, n), W.as<double>(), X, cpbar, s);
-  tc_launch(Prec::F64, gemm_desc('N', 'T', n, n, chi, n, n, n, 1.0, 1.0), X, W.as<double>(), cpbar, s);
+  tc_launch(c, Prec::F64, gemm_desc('N', 'T', n, n, chi, n, n, n), W.as<double>(), X, cpbar, s);
+  tc_launch(c, Prec::F64, gemm_desc('N', 'T', n, n, chi, n, n, n, 1.0, 1.0), X, W.as<double>(), cpbar, s);
+  debug_dump(c, "eb_W", W.as<double>(), (size_t)n * chi);
+  debug_dump(c, "eb_cpbar", cpbar, (size_t)n * n);
   if (st) st->neumann_terms = terms;
 }
 
@@ -518,6 +558,10 @@ void ctmrg_step_back(Ctx* c, const StepDims& d, Prec p, const StepTape& tp, cons
   k_sym_add(C2bar.as<double>(), chi, C1bar.as<double>(), s);
   k_edge_sym(T2bar.as<double>(), chi, D, T1bar.as<double>(), s);
 
+  // tape intermediates stored at the forward step's precision, converted to the sweep's
+  PBuf Y2tmp, X2tmp;
+  const PBuf& Y2 = at_prec(c, tp.Y2, p, Y2tmp);
+  const PBuf& X2 = at_prec(c, tp.X2, p, X2tmp);
   // --- T' chain adjoint (tp never materialised) ----------------------------
   PBuf T1b(c, p, nT), zp(c, p, (size_t)n * chi), Tp(c, p, nT), Cp(c, p, nC);
   to_prec(c, T1bar.as<double>(), T1b);
@@ -545,7 +589,7 @@ void ctmrg_step_back(Ctx* c, const StepDims& d, Prec p, const StepTape& tp, cons
   tc(c, p, cm("abjy", {lc, lD, lD, lc}), P1.ptr(), cm("xjy", {lc, lD, lc}), T1b.ptr(), cm("abx", {lc, lD, lc}),
      zbp.ptr());
   // zbar[d,c,y] += sum_{x,j} Y2[d,c,x,j] T1bar[x,j,y]
-  tc(c, p, cm("dcxj", {lc, lD, lc, lD}), tp.Y2.ptr(), cm("xjy", {lc, lD, lc}), T1b.ptr(), cm("dcy", {lc, lD, lc}),
+  tc(c, p, cm("dcxj", {lc, lD, lc, lD}), Y2.ptr(), cm("xjy", {lc, lD, lc}), T1b.ptr(), cm("dcy", {lc, lD, lc}),
      zbp.ptr(), 1.0, 1.0);
   DevBuf zbar(c, (size_t)n * chi * 8);
   from_prec(c, zbp, zbar.as<double>());
@@ -556,7 +600,7 @@ void ctmrg_step_back(Ctx* c, const StepDims& d, Prec p, const StepTape& tp, cons
   from_prec(c, Tbp, Tacc.as<double>());
 
   // --- C' chain: zbar += (cpmat + cpmat') z C1bar  (FP64; A z kept from the forward)
-  tc_launch(Prec::F64, gemm_desc('N', 'N', n, chi, chi, n, chi, n, 1.0, 1.0), tp.AX.as<double>(),
+  tc_launch(c, Prec::F64, gemm_desc('N', 'N', n, chi, chi, n, chi, n, 1.0, 1.0), tp.AX.as<double>(),
             C1bar.as<double>(), zbar.as<double>(), s);
 
   // --- eigen adjoint -> cpbar (n x n, FP64) ---------------------------------
@@ -573,7 +617,7 @@ void ctmrg_step_back(Ctx* c, const StepDims& d, Prec p, const StepTape& tp, cons
   tc(c, p, cm("ijlk", {lc, lD, lc, lD}), cpb.ptr(), cm("jkcb", {lD, lD, lD, lD}), ap.ptr(),
      cm("ibcl", {lc, lD, lD, lc}), X2b.ptr());
   // abar[j,k,c,b] += sum_{i,l} cpbar[i,j,l,k] X2[i,b,c,l]
-  tc(c, p, cm("ijlk", {lc, lD, lc, lD}), cpb.ptr(), cm("ibcl", {lc, lD, lD, lc}), tp.X2.ptr(),
+  tc(c, p, cm("ijlk", {lc, lD, lc, lD}), cpb.ptr(), cm("ibcl", {lc, lD, lD, lc}), X2.ptr(),
      cm("jkcb", {lD, lD, lD, lD}), abp.ptr());
   from_prec(c, abp, a_acc.as<double>());
   k_axpby(1.0, a_acc.as<double>(), 1.0, abar, nA, s);

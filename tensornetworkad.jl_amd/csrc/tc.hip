@@ -45,17 +45,23 @@ __device__ __forceinline__ long long off2(int i, int I0, long long s0, long long
 }
 
 template <typename T> struct MfmaOp;
+// Accumulator lane layout of the 16x16 output block (lane l, accumulator register r):
+//   f64 16x16x4 : row = (l / 16) + 4 r, column = l % 16
+//   f32 16x16x4 : row = 4 (l / 16) + r, column = l % 16
+// (operands for both: lane l supplies A[l % 16][l / 16] and B[l / 16][l % 16]).
 template <> struct MfmaOp<double> {
   typedef d4 acc_t;
   static __device__ __forceinline__ acc_t run(double a, double b, acc_t c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
   }
+  static __device__ __forceinline__ int row(int lane, int r) { return (lane >> 4) + 4 * r; }
 };
 template <> struct MfmaOp<float> {
   typedef f4 acc_t;
   static __device__ __forceinline__ acc_t run(float a, float b, acc_t c) {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
   }
+  static __device__ __forceinline__ int row(int lane, int r) { return 4 * (lane >> 4) + r; }
 };
 
 // ---------------------------------------------------------------------------
@@ -153,7 +159,7 @@ __global__ __launch_bounds__(NT) void tc_kernel(TcDesc d, const T* __restrict__ 
     }
   }
 
-  // epilogue: lane holds rows 4*(lane/16)+r, column lane%16 of each 16x16 block
+  // epilogue: column lane%16 of each 16x16 block, rows per MfmaOp<T>::row
   const T alpha = (T)d.alpha, beta = (T)d.beta;
 #pragma unroll
   for (int j = 0; j < 2; ++j) {
@@ -164,7 +170,7 @@ __global__ __launch_bounds__(NT) void tc_kernel(TcDesc d, const T* __restrict__ 
     for (int i = 0; i < 2; ++i) {
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        int m = tile_m + wm * 32 + i * 16 + 4 * (lane >> 4) + r;
+        int m = tile_m + wm * 32 + i * 16 + MfmaOp<T>::row(lane, r);
         if (m >= M) continue;
         long long o = noff + off2(m, d.M0, d.cm0, d.cm1);
         T v = alpha * (T)acc[i][j][r];
@@ -301,8 +307,21 @@ __global__ __launch_bounds__(NT) void tc_kernel_bf16(TcDesc d, const float* __re
 
 }  // namespace
 
-void tc_launch(Prec p, const TcDesc& d, const void* A, const void* B, void* C, hipStream_t s) {
+static size_t extent_bytes(long long x0, long long s0, long long x1, long long s1, long long y0, long long t0,
+                           long long y1, long long t1, long long nb, long long sb, size_t es) {
+  if (s0 < 0 || s1 < 0 || t0 < 0 || t1 < 0 || sb < 0) throw Error(TNAD_ERR_ARG, "tc: negative stride");
+  long long last = (x0 - 1) * s0 + (x1 - 1) * s1 + (y0 - 1) * t0 + (y1 - 1) * t1 + (nb - 1) * sb;
+  return (size_t)(last + 1) * es;
+}
+
+void tc_launch(Ctx* c, Prec p, const TcDesc& d, const void* A, const void* B, void* C, hipStream_t s) {
   const long long M = (long long)d.M0 * d.M1, N = (long long)d.N0 * d.N1;
+  {
+    const size_t es = p == Prec::F64 ? 8 : 4;
+    c->check_range(A, extent_bytes(d.M0, d.am0, d.M1, d.am1, d.K0, d.ak0, d.K1, d.ak1, d.batch, d.abat, es), "tc A");
+    c->check_range(B, extent_bytes(d.K0, d.bk0, d.K1, d.bk1, d.N0, d.bn0, d.N1, d.bn1, d.batch, d.bbat, es), "tc B");
+    c->check_range(C, extent_bytes(d.M0, d.cm0, d.M1, d.cm1, d.N0, d.cn0, d.N1, d.cn1, d.batch, d.cbat, es), "tc C");
+  }
   if (M <= 0 || N <= 0 || d.batch <= 0) return;
   if ((long long)d.K0 * d.K1 <= 0) {
     throw Error(TNAD_ERR_ARG, "tc_launch: empty contraction index");

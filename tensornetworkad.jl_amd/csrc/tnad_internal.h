@@ -67,8 +67,11 @@ TcDesc gemm_desc(char ta, char tb, int M, int N, int K, long long lda, long long
 
 enum class Prec { F64 = TNAD_PREC_FP64, F32 = TNAD_PREC_FP32, BF16 = TNAD_PREC_BF16 };
 
+struct Ctx;
 // A, B, C in double (F64) or float (F32 / BF16: BF16 rounds operands on the way into LDS).
-void tc_launch(Prec p, const TcDesc& d, const void* A, const void* B, void* C, hipStream_t s);
+// Every operand's full strided extent is checked against the context's live device
+// blocks before the launch (a size or precision mismatch throws instead of faulting).
+void tc_launch(Ctx* c, Prec p, const TcDesc& d, const void* A, const void* B, void* C, hipStream_t s);
 
 // ---------------------------------------------------------------------------
 // Small dense kernels (eig.hip)
@@ -120,7 +123,10 @@ struct Ctx {
   hipStream_t stream = nullptr;
   std::string last_error;
   std::multimap<size_t, void*> free_blocks;   // size -> ptr cache
+  std::map<uintptr_t, size_t> live;           // start -> bytes of every block handed out
   size_t cached_bytes = 0;
+  // throw unless [p, p + bytes) lies inside one live block
+  void check_range(const void* p, size_t bytes, const char* what) const;
   void* alloc(size_t bytes);
   void release(void* p, size_t bytes);
   void trim();

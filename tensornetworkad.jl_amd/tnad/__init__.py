@@ -17,6 +17,7 @@ from .api import (  # noqa: F401
     SquareIPEPS,
     TFIsing,
     ctmrg,
+    contract,
     ctmrgstep,
     default_context,
     diaglocalhamiltonian,
@@ -38,6 +39,7 @@ from .api import (  # noqa: F401
     optimiseipeps,
     rrule_energy,
     rrule_trg,
+    time_contract,
     trg,
     trg_and_grad,
     trg_svd,

@@ -76,6 +76,8 @@ SIGNATURES = {
     "tnad_energy": (I, [P, P, P, I, I, I, D, I, C.POINTER(Opts), PD, P, PI]),
     "tnad_trg": (I, [P, P, I, I, I, D, PD, P]),
     "tnad_trg_svd": (I, [P, P, I, I, I, I, I, D, P, P, PI]),
+    "tnad_contract": (I, [P, I, C.c_char_p, P, P, C.c_char_p, P, P, C.c_char_p, P, D, D, P]),
+    "tnad_time_contract": (I, [P, I, C.c_char_p, P, C.c_char_p, P, C.c_char_p, P, I, PD]),
 }
 
 _lib = None

@@ -429,6 +429,44 @@ def rrule_energy(h, ipeps: SquareIPEPS, chi: int, tol: float, maxit: int, ctx: C
     return e, (lambda ebar: ebar * g)
 
 
+def contract(spec: str, A, B, precision="fp64", ctx: Context | None = None, alpha: float = 1.0,
+             beta: float = 0.0, Cin=None) -> np.ndarray:
+    """One library contraction ``C = alpha ein"spec"(A, B) + beta Cin`` (column-major tensors,
+    single-character index names, every output index must appear in A or B)."""
+    ctx = ctx or default_context()
+    lhs, ic = spec.replace(" ", "").split("->")
+    ia, ib = lhs.split(",")
+    A, B = _host(A), _host(B)
+    ext = {}
+    for idx, T in ((ia, A), (ib, B)):
+        for ch, n in zip(idx, T.shape):
+            ext[ch] = n
+    dc = tuple(ext[ch] for ch in ic)
+    arr = lambda t: (C.c_longlong * len(t))(*t)
+    pa, _a = _in(A)
+    pb, _b = _in(B)
+    pc, out = _out(dc)
+    if beta != 0.0:
+        out[...] = _host(Cin)
+    ctx.check(ctx.lib.tnad_contract(ctx.h, _lib.PREC[precision], ia.encode(), arr(A.shape), pa, ib.encode(),
+                                    arr(B.shape), pb, ic.encode(), arr(dc), float(alpha), float(beta), pc))
+    return out
+
+
+def time_contract(spec: str, extents: dict, precision="fp64", reps: int = 20,
+                  ctx: Context | None = None) -> float:
+    """Average device milliseconds of one launch of ``ein"spec"`` at the given index extents
+    (HIP events on the context's stream, operands resident on the device)."""
+    ctx = ctx or default_context()
+    lhs, ic = spec.replace(" ", "").split("->")
+    ia, ib = lhs.split(",")
+    arr = lambda idx: (C.c_longlong * len(idx))(*[extents[ch] for ch in idx])
+    ms = C.c_double(0.0)
+    ctx.check(ctx.lib.tnad_time_contract(ctx.h, _lib.PREC[precision], ia.encode(), arr(ia), ib.encode(), arr(ib),
+                                         ic.encode(), arr(ic), int(reps), C.byref(ms)))
+    return ms.value
+
+
 # ---------------------------------------------------------------------------
 # TRG (src/trg.jl)
 # ---------------------------------------------------------------------------

@@ -1,0 +1,84 @@
+"""GPU parity of the matrix-core contraction kernel (tnad_contract) against NumPy
+einsum: plain GEMM shapes (tile edges, K not a multiple of the K-tile) and every
+strided contraction pattern of the CTMRG step, its adjoint and the energy chain."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+TOL = {"fp64": 1e-13, "fp32": 2e-6, "bf16": 2e-2}
+
+
+def _rel(a, b):
+    return np.max(np.abs(a - b)) / max(1e-300, np.max(np.abs(b)))
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32", "bf16"])
+@pytest.mark.parametrize("m,n,k", [(16, 16, 4), (64, 64, 16), (70, 33, 45), (1, 5, 3), (130, 200, 300)])
+def test_plain_gemm(ctx, prec, m, n, k):
+    import tnad
+    print(f"case: gemm {prec} {m}x{n}x{k}", flush=True)
+    rng = np.random.default_rng(m * 1000 + n + k)
+    A = rng.standard_normal((m, k))
+    B = rng.standard_normal((k, n))
+    Cm = tnad.contract("ik,kj->ij", A, B, precision=prec, ctx=ctx)
+    assert _rel(Cm, A @ B) < TOL[prec]
+
+
+SPECS = [
+    "iba,ad->ibd", "ibd,dcl->ibcl", "ibcl,jkcb->ijlk", "aed,abx->edbx", "edbx,bjce->dcxj",
+    "dcxj,dcy->xjy", "abx,xjy->abjy", "abjy,bjce->aecy", "aecy,dcy->aed", "abjy,aecy->bjce",
+    "ijlk,jkcb->ibcl", "ijlk,ibcl->jkcb", "ibcl,dcl->ibd", "ibd,ibcl->dcl", "iba,ibd->ad",
+    "abcij,ijkl->abckl",
+]
+
+
+def _dims(spec, chi, D, s):
+    ext = {}
+    for ch in "abcdefgijklmxy":
+        ext[ch] = D
+    lhs, out = spec.split("->")
+    ia, ib = lhs.split(",")
+    return ia, ib, out
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+@pytest.mark.parametrize("spec", SPECS)
+def test_step_contractions(ctx, prec, spec):
+    import tnad
+    print(f"case: contract {prec} {spec}", flush=True)
+    rng = np.random.default_rng(abs(hash(spec)) % 1000)
+    chi, D = 7, 3
+    # extents: indices that carry the environment bond get chi, the others D
+    chi_idx = {"iba,ad->ibd": "iad", "ibd,dcl->ibcl": "idl", "ibcl,jkcb->ijlk": "il",
+               "aed,abx->edbx": "adx", "edbx,bjce->dcxj": "dx", "dcxj,dcy->xjy": "dxy",
+               "abx,xjy->abjy": "axy", "abjy,bjce->aecy": "ay", "aecy,dcy->aed": "ady",
+               "abjy,aecy->bjce": "ay", "ijlk,jkcb->ibcl": "il", "ijlk,ibcl->jkcb": "il",
+               "ibcl,dcl->ibd": "idl", "ibd,ibcl->dcl": "idl", "iba,ibd->ad": "iad",
+               "abcij,ijkl->abckl": "ac"}[spec]
+    lhs, out = spec.split("->")
+    ia, ib = lhs.split(",")
+    ext = {ch: (chi if ch in chi_idx else D) for ch in set(ia + ib + out)}
+    if spec == "abcij,ijkl->abckl":
+        ext.update({"i": 2, "j": 2, "k": 2, "l": 2})
+    A = rng.standard_normal([ext[c] for c in ia])
+    B = rng.standard_normal([ext[c] for c in ib])
+    got = tnad.contract(spec, A, B, precision=prec, ctx=ctx)
+    ref = np.einsum(spec, A, B)
+    assert _rel(got, ref) < 10 * TOL[prec]
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+@pytest.mark.parametrize("tr", ["NN", "NT", "TN"])
+def test_gemm_alpha_beta_accumulate(ctx, prec, tr):
+    import tnad
+    print(f"case: gemm alpha/beta {prec} {tr}", flush=True)
+    rng = np.random.default_rng(7)
+    m, n, k = 40, 24, 10
+    spec = {"NN": "ik,kj->ij", "NT": "ik,jk->ij", "TN": "ki,kj->ij"}[tr]
+    A = rng.standard_normal((m, k) if tr[0] == "N" else (k, m))
+    B = rng.standard_normal((k, n) if tr[1] == "N" else (n, k))
+    C0 = rng.standard_normal((m, n))
+    got = tnad.contract(spec, A, B, precision=prec, ctx=ctx, alpha=-0.5, beta=1.0, Cin=C0)
+    ref = -0.5 * np.einsum(spec, A, B) + C0
+    assert _rel(got, ref) < 10 * TOL[prec]
