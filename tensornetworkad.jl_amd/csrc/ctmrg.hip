@@ -48,6 +48,20 @@ __global__ void resid_norms_kernel(const double* __restrict__ X, const double* _
   if (threadIdx.x == 0) out[j] = sqrt(red[0] + red[1] + red[2] + red[3]);
 }
 
+// next subspace block from the Rayleigh-Ritz pairs: Z[:, j] = Yr[:, j] / theta_j = Xr[:, j] + R[:, j] / theta_j
+// (R = Yr - Xr Theta is orthogonal to Xr, so Z^T Z = I + (R Theta^-1)^T (R Theta^-1) >= I stays well
+// conditioned however collinear the residuals are); numerically null Ritz values keep Z[:, j] = Xr[:, j].
+__global__ void ritz_scale_kernel(const double* __restrict__ Xr, const double* __restrict__ Yr,
+                                  const double* __restrict__ theta, int n, int k, double* __restrict__ Z) {
+  const double t0 = fabs(theta[0]);
+  size_t tot = (size_t)n * k;
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < tot; e += (size_t)gridDim.x * blockDim.x) {
+    int j = e / n;
+    double th = theta[j];
+    Z[e] = (fabs(th) > 1e-14 * t0) ? Yr[e] / th : Xr[e];
+  }
+}
+
 // make the largest-|.| entry of every column positive (first occurrence wins);
 // the same sign is applied to the companion block Y (A*X) when given.
 __global__ void canon_sign_kernel(double* __restrict__ X, double* __restrict__ Y, int n) {
@@ -89,9 +103,14 @@ __global__ void canon_sign_kernel(double* __restrict__ X, double* __restrict__ Y
 //                     R11_ij = (F_ij g_ij + F_ji g_ji) s_j
 //   rows chi..k-1   : F_ij g_ij (lambda_i + lambda_j)
 // F_ij = (s_j^2 - s_i^2) / ((s_j^2 - s_i^2)^2 + eta),  s = |lambda|.
+// Evaluated without FP contraction: the reference forms S2 = S.^2 first and then
+// S2' .- S2 (src/trg.jl:76-78), so the diagonal difference is exactly zero and the
+// Lorentzian F vanishes there; a fused multiply-add would leave the rounding residual of
+// s^2 (~1e-17) in place of that zero and F would blow up to ~1e23.
 __global__ void eigback_coef_kernel(const double* __restrict__ g, const double* __restrict__ lam,
                                     const double* __restrict__ C1bar, int k, int chi, double eta,
                                     double* __restrict__ coef) {
+#pragma clang fp contract(off)
   int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= k * chi) return;
   int i = e % k, j = e / k;
@@ -244,31 +263,55 @@ static void orthonormalize(Ctx* c, double* Y, int n, int k, double* X, EigWork& 
   k_random_fill_cols(Y, n, k, w.nrm.as<double>(), 1e-13, seed, s);
   k_col_norms(Y, n, k, w.nrm.as<double>(), s);
   k_scale_cols(Y, n, k, w.nrm.as<double>(), 1, s);
-  // shifted CholeskyQR followed by two plain CholeskyQR passes (sCholQR3)
+  // shifted CholeskyQR followed by two plain CholeskyQR passes (sCholQR3).  Columns that the
+  // shifted pass finds linearly dependent on earlier ones (pivot at the shift's level: the
+  // block has lower rank than k, e.g. a rank-deficient corner matrix) are replaced by random
+  // directions first, so the block always spans k dimensions.
   const double u = 1.1102230246251565e-16;
   const double shift = 11.0 * ((double)n * k + (double)k * (k + 1)) * u * (double)k;
-  double* src = Y;
-  double* dst = X;
-  for (int pass = 0; pass < 3; ++pass) {
+  std::vector<double> piv(k);
+  std::vector<int> dead(k);
+  for (int attempt = 0; attempt < 3; ++attempt) {
+    tc_launch(c, Prec::F64, gemm_desc('T', 'N', k, k, n, n, n, k), Y, Y, w.G.as<double>(), s);
+    chol_inv(w.G.as<double>(), k, shift, w.Rinv.as<double>(), w.info.as<int>(), w.res.as<double>(), s);
+    TNAD_HIP_TRY(hipMemcpyAsync(piv.data(), w.res.ptr, (size_t)k * 8, hipMemcpyDeviceToHost, s));
+    c->sync();
+    // a column with no component outside the span of the earlier ones keeps only the shift
+    const double thresh = 2.0 * std::sqrt(shift);
+    int ndead = 0;
+    for (int j = 0; j < k; ++j) {
+      dead[j] = !(piv[j] > thresh);
+      ndead += dead[j];
+    }
+    if (ndead == 0) break;
+    TNAD_HIP_TRY(hipMemcpyAsync(w.info.ptr, dead.data(), (size_t)k * sizeof(int), hipMemcpyHostToDevice, s));
+    k_random_fill_flagged(Y, n, k, w.info.as<int>(), seed + 7919u * (attempt + 1), s);
+    k_col_norms(Y, n, k, w.nrm.as<double>(), s);
+    k_scale_cols(Y, n, k, w.nrm.as<double>(), 1, s);
+  }
+  tc_launch(c, Prec::F64, gemm_desc('N', 'N', n, k, k, n, k, n), Y, w.Rinv.as<double>(), X, s);
+  double* src = X;
+  double* dst = Y;
+  for (int pass = 1; pass < 3; ++pass) {
     tc_launch(c, Prec::F64, gemm_desc('T', 'N', k, k, n, n, n, k), src, src, w.G.as<double>(), s);
-    chol_inv(w.G.as<double>(), k, pass == 0 ? shift : 0.0, w.Rinv.as<double>(), w.info.as<int>(), s);
+    chol_inv(w.G.as<double>(), k, 0.0, w.Rinv.as<double>(), w.info.as<int>(), nullptr, s);
     tc_launch(c, Prec::F64, gemm_desc('N', 'N', n, k, k, n, k, n), src, w.Rinv.as<double>(), dst, s);
     std::swap(src, dst);
   }
-  // three passes alternate Y -> X -> Y -> X: the orthonormal block ends in X (Y is scratch)
+  // passes alternate X -> Y -> X: the orthonormal block ends in X (Y is scratch)
 }
 
 EigWork::EigWork(Ctx* c, int n, int k)
     : Y(c, (size_t)n * k * 8), Xr(c, (size_t)n * k * 8), Yr(c, (size_t)n * k * 8), H(c, (size_t)k * k * 8),
-      W(c, (size_t)k * k * 8), Vw(c, (size_t)k * k * 8), G(c, (size_t)k * k * 8), Rinv(c, (size_t)k * k * 8),
-      nrm(c, (size_t)k * 8), res(c, (size_t)k * 8), info(c, 16), sweeps(c, 16) {}
+      W(c, (size_t)k * k * 8), Vw(c, jacobi_work_doubles(k, 60) * 8), G(c, (size_t)k * k * 8), Rinv(c, (size_t)k * k * 8),
+      nrm(c, (size_t)k * 8), res(c, (size_t)k * 8), info(c, (size_t)k * sizeof(int) + 16), sweeps(c, 16) {}
 
 int topk_eig(Ctx* c, const double* A, int n, int want, int k, const double* X0, int maxit, double tol,
              double* X, double* AX, double* theta, EigStats* st) {
   hipStream_t s = c->stream;
   if (n <= JACOBI_MAX) {
     // full spectrum: exact reference semantics (all n singular values / vectors)
-    DevBuf Vw(c, (size_t)n * n * 8);
+    DevBuf Vw(c, jacobi_work_doubles(n, 60) * 8);
     jacobi_eig_ws(A, n, n, theta, X, Vw.as<double>(), 60, nullptr, s);
     tc_launch(c, Prec::F64, gemm_desc('N', 'N', n, n, n, n, n, n), A, X, AX, s);
     if (st) { st->iters = 1; st->k = n; st->resid = 0.0; }
@@ -305,7 +348,10 @@ int topk_eig(Ctx* c, const double* A, int n, int want, int k, const double* X0, 
     rel = mx / std::max(std::fabs(hth[0]), 1e-300);
     if (!(rel == rel)) throw Error(TNAD_ERR_NAN, "eigensolver: NaN in Ritz residual");
     if (rel <= tol || it == maxit) break;
-    orthonormalize(c, w.Yr.as<double>(), n, k, X, w, 0x51ed2701u + it);
+    hipLaunchKernelGGL(ritz_scale_kernel, dim3(nb((size_t)n * k)), dim3(NT), 0, s, w.Xr.as<double>(),
+                       w.Yr.as<double>(), theta, n, k, w.Y.as<double>());
+    TNAD_HIP_TRY(hipGetLastError());
+    orthonormalize(c, w.Y.as<double>(), n, k, X, w, 0x51ed2701u + it);
   }
   k_copy(w.Xr.as<double>(), X, (size_t)n * k, s);
   k_copy(w.Yr.as<double>(), AX, (size_t)n * k, s);

@@ -31,10 +31,20 @@ __device__ __forceinline__ void decode_tri(int b, int& i, int& j) {
   i = b - jj * (jj + 1) / 2;
 }
 
-__global__ __launch_bounds__(EIG_NT) void jacobi_eig_kernel(const double* __restrict__ H, int ldh,
-                                                            int m, double* __restrict__ evals,
-                                                            double* __restrict__ V,
-                                                            double* __restrict__ Vw, int max_sweeps,
+__device__ __forceinline__ void rr_pair(int r_in, int t, int mm, int& p, int& q) {
+  // round-robin (circle) pairing of round r_in, slot t
+  if (t == 0) { p = r_in; q = mm - 1; }
+  else { p = (r_in + t) % (mm - 1); q = (r_in - t + (mm - 1)) % (mm - 1); }
+  if (p > q) { int tmp = p; p = q; q = tmp; }
+}
+
+// Jacobi sweeps on the packed matrix in LDS.  The rotations of every round are
+// recorded (rot[(round * P + slot) * 2 + {0,1}] = {c, s}); the eigenvector
+// matrix is formed afterwards by jacobi_apply_kernel.  Writes the (unsorted)
+// diagonal to diag and the number of rounds to *rounds_out.
+__global__ __launch_bounds__(EIG_NT) void jacobi_rot_kernel(const double* __restrict__ H, int ldh, int m,
+                                                            double* __restrict__ rot, int max_sweeps,
+                                                            double* __restrict__ diag, int* rounds_out,
                                                             int* sweeps_out) {
   extern __shared__ double smem[];
   const int mm = m + (m & 1);
@@ -55,9 +65,7 @@ __global__ __launch_bounds__(EIG_NT) void jacobi_eig_kernel(const double* __rest
     decode_tri(b, i, j);
     a[b] = 0.5 * (H[i + (long long)j * ldh] + H[j + (long long)i * ldh]);
   }
-  for (int e = t; e < m * m; e += EIG_NT) Vw[e] = ((e % m) == (e / m)) ? 1.0 : 0.0;
   __syncthreads();
-  // Frobenius norm for the absolute threshold
   double part = 0.0;
   for (int b = t; b < m * (m + 1) / 2; b += EIG_NT) {
     int i, j;
@@ -69,25 +77,24 @@ __global__ __launch_bounds__(EIG_NT) void jacobi_eig_kernel(const double* __rest
   if ((t & 63) == 0) red[t >> 6] = part;
   __syncthreads();
   if (t == 0) {
-    double s = 0.0;
-    for (int w = 0; w < EIG_NT / 64; ++w) s += red[w];
-    fro_s = sqrt(s);
+    double sacc = 0.0;
+    for (int w = 0; w < EIG_NT / 64; ++w) sacc += red[w];
+    fro_s = sqrt(sacc);
   }
   __syncthreads();
   const double abs_floor = 1e-18 * fro_s + 1e-300;
   const double rel_tol = 1e-15;
 
+  int round = 0;
   int sweep = 0;
   for (; sweep < max_sweeps; ++sweep) {
     if (t == 0) nrot_s = 0;
     __syncthreads();
-    for (int r = 0; r < mm - 1; ++r) {
+    for (int r = 0; r < mm - 1; ++r, ++round) {
       if (t < P) {
         int p, q;
-        if (t == 0) { p = r; q = mm - 1; }
-        else { p = (r + t) % (mm - 1); q = (r - t + (mm - 1)) % (mm - 1); }
-        if (p > q) { int tmp = p; p = q; q = tmp; }
-        double c = 1.0, s = 0.0;
+        rr_pair(r, t, mm, p, q);
+        double c = 1.0, sn = 0.0;
         if (q < m) {
           double apq = a[pk(p, q)];
           double app = a[pk(p, p)], aqq = a[pk(q, q)];
@@ -95,11 +102,13 @@ __global__ __launch_bounds__(EIG_NT) void jacobi_eig_kernel(const double* __rest
             double theta = (aqq - app) / (2.0 * apq);
             double tt = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
             c = 1.0 / sqrt(tt * tt + 1.0);
-            s = tt * c;
+            sn = tt * c;
             atomicAdd(&nrot_s, 1);
           }
         }
-        rc[t] = c; rs[t] = s; rp[t] = p; rq[t] = q;
+        rc[t] = c; rs[t] = sn; rp[t] = p; rq[t] = q;
+        rot[((size_t)round * P + t) * 2 + 0] = c;
+        rot[((size_t)round * P + t) * 2 + 1] = sn;
       }
       __syncthreads();
       // 2x2 block updates A' = Ji^T A Jj over unordered pairs of rotation slots
@@ -124,7 +133,6 @@ __global__ __launch_bounds__(EIG_NT) void jacobi_eig_kernel(const double* __rest
           double xps = sv ? a[pku(p, s0)] : 0.0;
           double xqr = qv ? a[pku(q, r0)] : 0.0;
           double xqs = (qv && sv) ? a[pku(q, s0)] : 0.0;
-          // left: rows (p,q) by Ji^T ; right: cols (r,s) by Jj
           double ypr = ci * xpr - si * xqr, yps = ci * xps - si * xqs;
           double yqr = si * xpr + ci * xqr, yqs = si * xps + ci * xqs;
           double zpr = cj * ypr - sj * yps, zps = sj * ypr + cj * yps;
@@ -135,43 +143,87 @@ __global__ __launch_bounds__(EIG_NT) void jacobi_eig_kernel(const double* __rest
           if (qv && sv) a[pku(q, s0)] = zqs;
         }
       }
-      // eigenvector columns: V' = V J for every active slot
-      for (int e = t; e < P * m; e += EIG_NT) {
-        int slot = e / m, row = e % m;
-        double s = rs[slot];
-        if (s == 0.0) continue;
-        double c = rc[slot];
-        int p = rp[slot], q = rq[slot];
-        double vp = Vw[row + (long long)p * m], vq = Vw[row + (long long)q * m];
-        Vw[row + (long long)p * m] = c * vp - s * vq;
-        Vw[row + (long long)q * m] = s * vp + c * vq;
-      }
-      __threadfence_block();
       __syncthreads();
     }
-    __syncthreads();
-    if (nrot_s == 0) break;
+    if (nrot_s == 0) { ++sweep; break; }
     __syncthreads();
   }
-  if (t == 0 && sweeps_out) *sweeps_out = sweep;
+  for (int i = t; i < m; i += EIG_NT) diag[i] = a[pk(i, i)];
+  if (t == 0) {
+    *rounds_out = round;
+    if (sweeps_out) *sweeps_out = sweep;
+  }
+}
+
+// Replays the recorded rotations on rows of V (V = product of all rotations,
+// starting from the identity).  Each workgroup owns JROWS rows in LDS; rows
+// are independent, rotations of one round touch disjoint column pairs.
+constexpr int JROWS = 8;
+constexpr int JCHUNK = 16;    // rounds of rotation parameters staged in LDS at once
+__global__ __launch_bounds__(256) void jacobi_apply_kernel(const double* __restrict__ rot,
+                                                           const int* __restrict__ rounds_ptr, int m,
+                                                           double* __restrict__ Vrows) {
+  extern __shared__ double smem[];
+  const int mm = m + (m & 1);
+  const int P = mm / 2;
+  double* rows = smem;                        // JROWS x m
+  double* prm = rows + JROWS * m;             // JCHUNK x P x 2
+  const int t = threadIdx.x;
+  const int row0 = blockIdx.x * JROWS;
+  const int nrows = min(JROWS, m - row0);
+  for (int e = t; e < JROWS * m; e += blockDim.x) {
+    int rr = e / m, cc = e % m;
+    rows[e] = (rr < nrows && row0 + rr == cc) ? 1.0 : 0.0;
+  }
+  const int R = *rounds_ptr;
   __syncthreads();
-  __threadfence();
-  // sort by |lambda| descending (stable), fix sign: largest |component| positive
-  for (int i = t; i < m; i += EIG_NT) {
-    double li = a[pk(i, i)];
+  for (int base = 0; base < R; base += JCHUNK) {
+    const int nr = min(JCHUNK, R - base);
+    for (int e = t; e < nr * P * 2; e += blockDim.x) prm[e] = rot[(size_t)base * P * 2 + e];
+    __syncthreads();
+    for (int rr = 0; rr < nr; ++rr) {
+      const int r_in = (base + rr) % (mm - 1);
+      const double* pr = prm + (size_t)rr * P * 2;
+      for (int e = t; e < nrows * P; e += blockDim.x) {
+        const int row = e / P, slot = e % P;
+        const double sn = pr[2 * slot + 1];
+        if (sn == 0.0) continue;
+        const double c = pr[2 * slot];
+        int p, q;
+        rr_pair(r_in, slot, mm, p, q);
+        double* R_ = rows + row * m;
+        double vp = R_[p], vq = R_[q];
+        R_[p] = c * vp - sn * vq;
+        R_[q] = sn * vp + c * vq;
+      }
+      __syncthreads();
+    }
+  }
+  for (int e = t; e < nrows * m; e += blockDim.x) {
+    int rr = e / m, cc = e % m;
+    Vrows[(row0 + rr) + (size_t)m * cc] = rows[rr * m + cc];   // column-major V
+  }
+}
+
+// sort by |lambda| descending (stable), sign convention: largest |component| of each vector positive
+__global__ __launch_bounds__(EIG_NT) void jacobi_sort_kernel(const double* __restrict__ diag,
+                                                             const double* __restrict__ Vu, int m,
+                                                             double* __restrict__ evals, double* __restrict__ V) {
+  for (int i = threadIdx.x; i < m; i += EIG_NT) {
+    double li = diag[i];
     double ai = fabs(li);
     int rank = 0;
     for (int j = 0; j < m; ++j) {
-      double aj = fabs(a[pk(j, j)]);
+      double aj = fabs(diag[j]);
       if (aj > ai || (aj == ai && j < i)) ++rank;
     }
     evals[rank] = li;
     double best = -1.0, sgn = 1.0;
     for (int row = 0; row < m; ++row) {
-      double v = Vw[row + (long long)i * m];
+      double v = Vu[row + (long long)i * m];
       if (fabs(v) > best) { best = fabs(v); sgn = v < 0 ? -1.0 : 1.0; }
     }
-    for (int row = 0; row < m; ++row) V[row + (long long)rank * m] = sgn * Vw[row + (long long)i * m];
+    for (int row = 0; row < m; ++row) V[row + (long long)rank * m] = sgn * Vu[row + (long long)i * m];
   }
 }
 
@@ -179,7 +231,7 @@ __global__ __launch_bounds__(EIG_NT) void jacobi_eig_kernel(const double* __rest
 // (upper triangular, full m x m column-major, zeros below the diagonal).
 __global__ __launch_bounds__(EIG_NT) void chol_inv_kernel(const double* __restrict__ G, int m,
                                                           double shift, double* __restrict__ Rinv,
-                                                          int* info) {
+                                                          int* info, double* __restrict__ piv) {
   extern __shared__ double smem[];
   // packed lower, column-major: element (i,j), i >= j at  j*m - j(j-1)/2 + (i-j)
   double* L = smem;
@@ -200,6 +252,7 @@ __global__ __launch_bounds__(EIG_NT) void chol_inv_kernel(const double* __restri
         d = 1e-300;
       }
       L[pl(j, j)] = sqrt(d);
+      if (piv) piv[j] = L[pl(j, j)];
     }
     __syncthreads();
     double djj = L[pl(j, j)];
@@ -246,23 +299,41 @@ static size_t jacobi_lds(int m) {
   return (size_t)m * (m + 1) / 2 * sizeof(double) + 2 * P * sizeof(double) + 2 * P * sizeof(int);
 }
 
-void jacobi_eig_ws(const double* H, int ldh, int m, double* evals, double* V, double* Vwork,
-                   int max_sweeps, int* sweeps_out, hipStream_t s) {
+size_t jacobi_work_doubles(int m, int max_sweeps) {
+  const int mm = m + (m & 1);
+  const size_t P = mm / 2;
+  // rotation record + unsorted V + diagonal + round counter
+  return (size_t)max_sweeps * (mm - 1) * P * 2 + (size_t)m * m + m + 8;
+}
+
+void jacobi_eig_ws(const double* H, int ldh, int m, double* evals, double* V, double* Vwork, int max_sweeps,
+                   int* sweeps_out, hipStream_t s) {
   if (m < 1 || m > JACOBI_MAX) throw Error(TNAD_ERR_ARG, "jacobi_eig: size out of range");
+  const int mm = m + (m & 1);
+  const size_t P = mm / 2;
+  double* rot = Vwork;
+  double* Vu = rot + (size_t)max_sweeps * (mm - 1) * P * 2;
+  double* diag = Vu + (size_t)m * m;
+  int* rounds = reinterpret_cast<int*>(diag + m);
   size_t lds = jacobi_lds(m);
-  TNAD_HIP_TRY(hipFuncSetAttribute((const void*)jacobi_eig_kernel,
-                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-  hipLaunchKernelGGL(jacobi_eig_kernel, dim3(1), dim3(EIG_NT), lds, s, H, ldh, m, evals, V, Vwork,
-                     max_sweeps, sweeps_out);
+  TNAD_HIP_TRY(hipFuncSetAttribute((const void*)jacobi_rot_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)lds));
+  hipLaunchKernelGGL(jacobi_rot_kernel, dim3(1), dim3(EIG_NT), lds, s, H, ldh, m, rot, max_sweeps, diag, rounds,
+                     sweeps_out);
+  TNAD_HIP_TRY(hipGetLastError());
+  const size_t lds2 = ((size_t)JROWS * m + (size_t)JCHUNK * P * 2) * sizeof(double);
+  hipLaunchKernelGGL(jacobi_apply_kernel, dim3((m + JROWS - 1) / JROWS), dim3(256), lds2, s, rot, rounds, m, Vu);
+  TNAD_HIP_TRY(hipGetLastError());
+  hipLaunchKernelGGL(jacobi_sort_kernel, dim3(1), dim3(EIG_NT), 0, s, diag, Vu, m, evals, V);
   TNAD_HIP_TRY(hipGetLastError());
 }
 
-void chol_inv(const double* G, int m, double shift, double* Rinv, int* info, hipStream_t s) {
+void chol_inv(const double* G, int m, double shift, double* Rinv, int* info, double* piv, hipStream_t s) {
   if (m < 1 || m > JACOBI_MAX) throw Error(TNAD_ERR_ARG, "chol_inv: size out of range");
   size_t lds = ((size_t)m * (m + 1) / 2 + m) * sizeof(double);
   TNAD_HIP_TRY(hipFuncSetAttribute((const void*)chol_inv_kernel,
                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-  hipLaunchKernelGGL(chol_inv_kernel, dim3(1), dim3(EIG_NT), lds, s, G, m, shift, Rinv, info);
+  hipLaunchKernelGGL(chol_inv_kernel, dim3(1), dim3(EIG_NT), lds, s, G, m, shift, Rinv, info, piv);
   TNAD_HIP_TRY(hipGetLastError());
 }
 

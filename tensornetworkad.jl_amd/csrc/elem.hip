@@ -138,6 +138,14 @@ __global__ void random_fill_cols_kernel(double* X, int n, int k, const double* n
   }
 }
 
+__global__ void random_fill_flagged_kernel(double* X, int n, int k, const int* flag, unsigned seed) {
+  size_t tot = (size_t)n * k;
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < tot; e += (size_t)gridDim.x * blockDim.x) {
+    int j = e / n;
+    if (flag[j]) X[e] = 2.0 * hash_uniform(seed, e) - 1.0;
+  }
+}
+
 __global__ void random_matrix_kernel(double* X, size_t n, unsigned seed) {
   for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x)
     X[e] = 2.0 * hash_uniform(seed, e) - 1.0;
@@ -355,6 +363,10 @@ void k_random_fill_cols(double* X, int n, int k, const double* nrm, double rel, 
                         hipStream_t s) {
   hipLaunchKernelGGL(random_fill_cols_kernel, dim3(nblk((size_t)n * k)), dim3(NT), 0, s, X, n, k, nrm,
                      rel, seed);
+  TNAD_HIP_TRY(hipGetLastError());
+}
+void k_random_fill_flagged(double* X, int n, int k, const int* flag, unsigned seed, hipStream_t s) {
+  hipLaunchKernelGGL(random_fill_flagged_kernel, dim3(nblk((size_t)n * k)), dim3(NT), 0, s, X, n, k, flag, seed);
   TNAD_HIP_TRY(hipGetLastError());
 }
 void k_random_matrix(double* X, size_t n, unsigned seed, hipStream_t s) {

@@ -80,13 +80,16 @@ constexpr int JACOBI_MAX = 192;   // packed FP64 upper triangle of 192x192 fits 
 
 // Full symmetric eigendecomposition of an m x m matrix (m <= JACOBI_MAX).
 // Output eigenpairs sorted by |lambda| descending; V column-major (ld = m).
-// Vwork: m*m doubles of scratch.  sweeps_out: device int (nullable).
+// Vwork: jacobi_work_doubles(m, max_sweeps) doubles of scratch (rotation record
+// and unsorted eigenvectors).  sweeps_out: device int (nullable).
+size_t jacobi_work_doubles(int m, int max_sweeps);
 void jacobi_eig_ws(const double* H, int ldh, int m, double* evals, double* V, double* Vwork,
                    int max_sweeps, int* sweeps_out, hipStream_t s);
 
 // Cholesky of the m x m SPD matrix G (+ shift on the diagonal), returns Linv^T
 // (upper triangular, column-major) so that X = Y * Rinv with Rinv = L^-T.
-void chol_inv(const double* G, int m, double shift, double* Rinv, int* info, hipStream_t s);
+// piv (nullable, m doubles) receives the Cholesky pivots diag(L).
+void chol_inv(const double* G, int m, double shift, double* Rinv, int* info, double* piv, hipStream_t s);
 
 // ---------------------------------------------------------------------------
 // Elementwise / reduction kernels (elem.hip)
@@ -106,6 +109,8 @@ void k_scale_cols(double* X, int n, int k, const double* scale, int invert, hipS
 // refill (with hash-random entries) every column whose norm is <= rel * max norm
 void k_random_fill_cols(double* X, int n, int k, const double* nrm, double rel, unsigned seed,
                         hipStream_t s);
+// overwrite the columns j with flag[j] != 0 by hash-random entries
+void k_random_fill_flagged(double* X, int n, int k, const int* flag, unsigned seed, hipStream_t s);
 void k_random_matrix(double* X, size_t n, unsigned seed, hipStream_t s);
 void k_init_raw(const double* a, int D, int chi, double* C, double* T, hipStream_t s);
 void k_symmetrize_bulk(const double* x, int d, int ph, double* y, hipStream_t s);

@@ -1,0 +1,70 @@
+"""GPU parity of TRG (src/trg.jl) through the C ABI: the reference's golden
+values, the oracle's gradient, and the trg_svd split."""
+import numpy as np
+import pytest
+
+from oracle import tnad_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _dlnz(tnad, ctx, beta, chi, niter):
+    lnz, abar = tnad.trg_and_grad(tnad.model_tensor(tnad.Ising(), beta), chi, niter, ctx=ctx)
+    return lnz, float(np.sum(abar * tnad.dmodel_tensor(tnad.Ising(), beta)))
+
+
+def test_trg_tensorgrad_golden_gpu(ctx):
+    import tnad
+    print("case: trg beta=0.4 chi=5 niter=5", flush=True)
+    # test/trg.jl:18
+    assert tnad.trg(tnad.model_tensor(tnad.Ising(), 0.4), 5, 5, ctx=ctx) == pytest.approx(0.8919788686747141,
+                                                                                          rel=1e-12, abs=0)
+
+
+def test_trg_userguide_gradient_gpu(ctx):
+    import tnad
+    print("case: trg grad beta=0.5 chi=5 niter=5", flush=True)
+    lnz, g = _dlnz(tnad, ctx, 0.5, 5, 5)
+    assert g == pytest.approx(1.7502426939979507, rel=1e-10)     # docs/src/userguide.md:28
+    lnz_o, g_o = O.trg_and_grad(0.5, 5, 5)
+    assert lnz == pytest.approx(lnz_o, rel=1e-13, abs=0)
+
+
+def test_trg_readme_config_a_gpu(ctx):
+    import tnad
+    print("case: trg grad beta=0.5 chi=20 niter=20", flush=True)
+    lnz, g = _dlnz(tnad, ctx, 0.5, 20, 20)
+    assert lnz == pytest.approx(1.0257933734351765, rel=1e-12, abs=0)   # README.md:60
+    assert g == pytest.approx(1.7455677143228514, rel=1e-9)            # README.md:70
+    assert abs(lnz - O.onsager_lnz(0.5)) < 1e-6
+
+
+def test_trg_full_gradient_matches_oracle_direction(ctx):
+    import tnad
+    print("case: trg full gradient random direction chi=8 niter=4", flush=True)
+    a = tnad.model_tensor(tnad.Ising(), 0.45)
+    lnz, abar = tnad.trg_and_grad(a, 8, 4, ctx=ctx)
+    rng = np.random.default_rng(0)
+    direc = rng.standard_normal(a.shape)
+    _, g_o = O.trg(O.model_tensor_ising(0.45), 8, 4, with_grad_seed=direc)
+    assert float(np.sum(abar * direc)) == pytest.approx(g_o, rel=1e-9)
+
+
+@pytest.mark.parametrize("shape", [(6, 6, 6, 6), (2, 3, 4, 5)])
+def test_trg_svd_reconstruction_gpu(ctx, shape):
+    import tnad
+    print(f"case: trg_svd {shape}", flush=True)
+    rng = np.random.default_rng(1)
+    t = rng.standard_normal(shape)
+    u, v = tnad.trg_svd(t, 100, 0.0, ctx=ctx)
+    assert np.allclose(np.einsum("ija,akl->ijkl", u, v), t, atol=1e-11)
+
+
+def test_trg_svd_keep_count_gpu(ctx):
+    import tnad
+    print("case: trg_svd keep count", flush=True)
+    rng = np.random.default_rng(1)
+    t = np.reshape(rng.standard_normal((4, 2)) @ rng.standard_normal((2, 4)), (2, 2, 2, 2), order="F")
+    u, v = tnad.trg_svd(t, 10, 1e-8, ctx=ctx)
+    assert u.shape[2] == 3 and v.shape[0] == 3       # src/trg.jl:37 keeps #{s > tol} + 1
+    assert np.allclose(np.einsum("ija,akl->ijkl", u, v), t, atol=1e-12)
