@@ -54,10 +54,11 @@ typedef struct tnad_opts {
   double switch_tol;   /* precision != FP64: switch to FP64 once |vals - oldvals| <= switch_tol */
   int bwd_precision;   /* contraction precision of the reverse sweep */
   int eig_block;       /* subspace block size k for chi*D > 192 (0 = 1.5 chi, capped at 192) */
-  int eig_max_iters;   /* subspace iterations per step, cold start */
+  int eig_max_iters;   /* cap on subspace iterations per step, cold start (stops at eig_tol) */
   int eig_warm;        /* 1: start each step's eigensolver from the previous step's Ritz vectors */
-  int eig_warm_iters;  /* subspace iterations per step, warm start */
-  double eig_tol;      /* relative Ritz residual required of the kept chi vectors */
+  int eig_warm_iters;  /* cap on subspace iterations per step, warm start (stops at eig_tol) */
+  double eig_tol;      /* relative Ritz residual required of the kept chi vectors in FP64 steps;
+                          low-precision steps use max(eig_tol, 1e-3 switch_tol) */
   int bwd_last;        /* reverse sweep over the last bwd_last steps only (0 = all steps) */
 } tnad_opts;
 
@@ -100,7 +101,8 @@ int tnad_symeig(tnad_ctx* ctx, const double* A, int n, int want, int k, double* 
  * cpmat += cpmat', src/ctmrg.jl:135). */
 int tnad_svd_back_sym(tnad_ctx* ctx, const double* A, int n, int chi, int k, const double* G, double* Abar);
 
-/* ctmrgstep (src/ctmrg.jl:126-153): one step; vals has chi*D entries (uncomputed tail = 0). */
+/* ctmrgstep (src/ctmrg.jl:126-153): one step; vals has chi*D entries: all of them when chi*D <= 192
+ * (full spectrum), otherwise the kept chi values followed by zeros. */
 int tnad_ctmrgstep(tnad_ctx* ctx, const double* a, int D, int chi, const double* C, const double* T,
                    const tnad_opts* opts, double* Cout, double* Tout, double* vals);
 
@@ -149,6 +151,45 @@ int tnad_contract(tnad_ctx* ctx, int precision, const char* ia, const long long*
  * operands (no transfers inside the timed region).  For roofline measurement. */
 int tnad_time_contract(tnad_ctx* ctx, int precision, const char* ia, const long long* da, const char* ib,
                        const long long* db, const char* ic, const long long* dc, int reps, double* avg_ms);
+
+/* ---------------------------------------------------------------------------------------------
+ * Learned warm start.  The reference starts CTMRG from _initializect_square (src/ctmrg.jl:62-86,
+ * :raw or :random, @nograd per src/autodiff.jl:5) and has no learned counterpart; these entries
+ * let a caller start tnad_ctmrg / a CTMRG run from a predicted environment instead.  CTMRG still
+ * runs to the caller's tolerance, so the fixed point and its gradient are unchanged.
+ * ------------------------------------------------------------------------------------------- */
+typedef struct tnad_mlp tnad_mlp;
+
+/* Fully connected network n_in -> hidden (tanh) x n_layers -> n_out (linear), FP32 parameters on
+ * the context's GPU, weights uniform(+-1/sqrt(fan_in)) from `seed`, biases 0.  Parameter order:
+ * W1 (hidden x n_in, column-major), b1, ..., W_out (n_out x hidden), b_out.  Free with
+ * tnad_mlp_free before destroying the context. */
+int tnad_mlp_create(tnad_ctx* ctx, int n_in, int hidden, int n_layers, int n_out, unsigned long long seed,
+                    tnad_mlp** mlp);
+int tnad_mlp_free(tnad_mlp* mlp);
+int tnad_mlp_num_params(const tnad_mlp* mlp, long long* n);
+int tnad_mlp_get_params(tnad_ctx* ctx, const tnad_mlp* mlp, float* theta);
+int tnad_mlp_set_params(tnad_ctx* ctx, tnad_mlp* mlp, const float* theta);   /* also resets Adam */
+/* Y (n_out x B) = network(X (n_in x B)) */
+int tnad_mlp_forward(tnad_ctx* ctx, const tnad_mlp* mlp, const float* X, int B, float* Y);
+/* loss = mean_b 0.5 |network(x_b) - t_b|^2 and (grad != NULL) its gradient (num_params floats) */
+int tnad_mlp_loss_grad(tnad_ctx* ctx, tnad_mlp* mlp, const float* X, const float* T, int B, double* loss,
+                       float* grad);
+/* one Adam step (beta 0.9 / 0.999, eps 1e-8) on that loss; *loss (nullable) is the loss before the step */
+int tnad_mlp_train_step(tnad_ctx* ctx, tnad_mlp* mlp, const float* X, const float* T, int B, float lr,
+                        double* loss);
+
+/* Network input of a bulk (d x d x d x d x s): the double layer a of the symmetrised bulk
+ * (src/ipeps.jl:32-39, src/variationalipeps.jl:30-34), normalised; D^4 floats with D = d^2. */
+int tnad_bulk_features(tnad_ctx* ctx, const double* bulk, int d, int s, float* feat);
+/* Canonical-gauge features of an environment (C chi x chi, T chi x D x chi, chi <= 192):
+ * C = O diag(lam) O^T (|lam| descending), T~ = O^T T O with each column of O signed so that
+ * sum_{r<i,a} (a+1)/(r+1) T~[r,a,i] >= 0 (column 0: largest-|.| component positive), then
+ * lam[0] >= 0 and sum_a (a+1) T~[0,a,0] >= 0 (global signs of C and T);
+ * feat = [lam ; T~] (chi + chi*D*chi floats).  Invariant under C -> S^T C S, T -> S^T T S. */
+int tnad_env_features(tnad_ctx* ctx, const double* C, const double* T, int chi, int D, float* feat);
+/* Starting environment from features: C0 = diag(lam), T0 = (T~ + T~^{kji}) / 2, both normalised. */
+int tnad_env_from_features(tnad_ctx* ctx, const float* feat, int chi, int D, double* C0, double* T0);
 
 #ifdef __cplusplus
 }

@@ -7,6 +7,7 @@
 #include <limits>
 #include <memory>
 
+#include "api_internal.h"
 #include "ctmrg.h"
 #include "trg.h"
 
@@ -100,10 +101,6 @@ void copy_out(Ctx* c, void* dst_any, const void* src_dev, size_t bytes) {
 
 using namespace tnad;
 
-struct tnad_ctx {
-  Ctx c;
-  tnad_stats stats{};
-};
 
 struct tnad_tape {
   tnad_ctx* ctx = nullptr;
@@ -116,25 +113,6 @@ struct tnad_tape {
 };
 
 namespace {
-
-template <typename F>
-int guarded(tnad_ctx* ctx, F&& f) {
-  if (!ctx) return TNAD_ERR_ARG;
-  try {
-    TNAD_HIP_TRY(hipSetDevice(ctx->c.device));
-    f();
-    ctx->c.sync();
-    ctx->c.last_error.clear();
-    return TNAD_OK;
-  } catch (const Error& e) {
-    ctx->c.last_error = e.msg;
-    (void)hipGetLastError();
-    return e.code;
-  } catch (const std::exception& e) {
-    ctx->c.last_error = e.what();
-    return TNAD_ERR_ARG;
-  }
-}
 
 Prec to_prec_enum(int p) {
   if (p == TNAD_PREC_FP64) return Prec::F64;
@@ -176,9 +154,12 @@ void run_ctmrg(tnad_ctx* ctx, tnad_tape& t, const double* C0, const double* T0, 
   bool switched = !lowp;
   StepOpts so;
   so.eig_block = o.eig_block;
-  so.eig_max_iters = o.eig_max_iters > 0 ? o.eig_max_iters : 60;
-  so.eig_warm_iters = o.eig_warm_iters > 0 ? o.eig_warm_iters : 3;
+  so.eig_max_iters = o.eig_max_iters > 0 ? o.eig_max_iters : 200;
+  so.eig_warm_iters = o.eig_warm_iters > 0 ? o.eig_warm_iters : 200;
   so.eig_tol = o.eig_tol > 0 ? o.eig_tol : 1e-13;
+  // a low-precision step's contractions carry ~1e-7 relative error: its eigenvectors need not
+  // be resolved beyond what the switch test can see
+  so.eig_tol_low = std::max(so.eig_tol, 1e-3 * o.switch_tol);
   const int k = step_k(d, so);
   DevBuf Xprev;            // warm start block when no tape is kept
   const double* X0 = nullptr;
@@ -244,9 +225,9 @@ void tnad_opts_default(tnad_opts* o) {
   o->switch_tol = 0.0;
   o->bwd_precision = TNAD_PREC_FP64;
   o->eig_block = 0;
-  o->eig_max_iters = 60;
+  o->eig_max_iters = 200;
   o->eig_warm = 0;
-  o->eig_warm_iters = 3;
+  o->eig_warm_iters = 200;
   o->eig_tol = 1e-13;
   o->bwd_last = 0;
 }
@@ -372,8 +353,9 @@ int tnad_ctmrgstep(tnad_ctx* ctx, const double* a, int D, int chi, const double*
     StepOpts so;
     so.prec = to_prec_enum(o.precision);
     so.eig_block = o.eig_block;
-    so.eig_max_iters = o.eig_max_iters > 0 ? o.eig_max_iters : 60;
+    so.eig_max_iters = o.eig_max_iters > 0 ? o.eig_max_iters : 200;
     so.eig_tol = o.eig_tol > 0 ? o.eig_tol : 1e-13;
+    so.eig_tol_low = so.eig_tol;
     PBuf ap(c, so.prec, nA);
     to_prec(c, ba.as<double>(), ap);
     std::vector<double> v;

@@ -33,9 +33,10 @@ struct StepDims {
 struct StepOpts {
   Prec prec = Prec::F64;      // contraction precision of this step
   int eig_block = 0;          // subspace block size k (n > 192); 0 = automatic
-  int eig_max_iters = 60;     // cold start
-  int eig_warm_iters = 3;     // warm start (previous step's Ritz vectors)
-  double eig_tol = 1e-13;     // relative Ritz residual of the kept vectors
+  int eig_max_iters = 200;    // cap, cold start
+  int eig_warm_iters = 200;   // cap, warm start (previous step's Ritz vectors)
+  double eig_tol = 1e-13;     // relative Ritz residual of the kept vectors (FP64 steps)
+  double eig_tol_low = 1e-13; // the same for steps at a lower contraction precision
 };
 
 struct StepState {
@@ -47,9 +48,10 @@ struct StepState {
 struct EigStats {
   int iters = 0, k = 0;
   double resid = 0.0;
+  int jacobi = 0;   // Rayleigh-Ritz steps that needed Jacobi sweeps
 };
 struct EigBackStats {
-  int neumann_terms = 0;
+  int pcg_iters = 0;
 };
 
 struct EigWork {

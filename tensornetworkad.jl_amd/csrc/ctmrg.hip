@@ -14,6 +14,7 @@
 // obtained from an eigendecomposition: s = |lambda| sorted descending,
 // U = eigenvectors, V = U sign(lambda).
 #include "ctmrg.h"
+#include "resolvent.h"
 
 #include <algorithm>
 #include <cmath>
@@ -98,53 +99,32 @@ __global__ void canon_sign_kernel(double* __restrict__ X, double* __restrict__ Y
   }
 }
 
-// coefficient block of the eigen adjoint (see eig_adjoint below).
-//   rows 0..chi-1   : Kt/2 with Kt = R11 Sigma + Sigma R11^T + C1bar,
-//                     R11_ij = (F_ij g_ij + F_ji g_ji) s_j
-//   rows chi..k-1   : F_ij g_ij (lambda_i + lambda_j)
+// coefficient block of the eigen adjoint among the kept pairs (see eig_adjoint below):
+//   coef = Kt/2 with Kt = R11 Sigma + Sigma R11^T + C1bar,  R11_ij = (F_ij g_ij + F_ji g_ji) s_j,
 // F_ij = (s_j^2 - s_i^2) / ((s_j^2 - s_i^2)^2 + eta),  s = |lambda|.
 // Evaluated without FP contraction: the reference forms S2 = S.^2 first and then
 // S2' .- S2 (src/trg.jl:76-78), so the diagonal difference is exactly zero and the
 // Lorentzian F vanishes there; a fused multiply-add would leave the rounding residual of
 // s^2 (~1e-17) in place of that zero and F would blow up to ~1e23.
 __global__ void eigback_coef_kernel(const double* __restrict__ g, const double* __restrict__ lam,
-                                    const double* __restrict__ C1bar, int k, int chi, double eta,
+                                    const double* __restrict__ C1bar, int chi, double eta,
                                     double* __restrict__ coef) {
 #pragma clang fp contract(off)
   int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= k * chi) return;
-  int i = e % k, j = e / k;
+  if (e >= chi * chi) return;
+  int i = e % chi, j = e / chi;
   double li = lam[i], lj = lam[j];
   double si2 = li * li, sj2 = lj * lj;
   double Fij = (sj2 - si2) / ((sj2 - si2) * (sj2 - si2) + eta);
-  double gij = g[i + (long long)j * k];
-  double v;
-  if (i < chi) {
-    double Fji = -Fij;
-    double gji = g[j + (long long)i * k];
-    double sj = fabs(lj), si = fabs(li);
-    double sigj = lj < 0 ? -1.0 : 1.0, sigi = li < 0 ? -1.0 : 1.0;
-    double R_ij = (Fij * gij + Fji * gji) * sj;   // R11[i,j]
-    double R_ji = (Fji * gji + Fij * gij) * si;   // R11[j,i]
-    double Kt = R_ij * sigj + sigi * R_ji + C1bar[i + (long long)j * chi];
-    v = 0.5 * Kt;
-  } else {
-    v = Fij * gij * (li + lj);
-  }
-  coef[e] = v;
-}
-
-// X_r[:, j] += scale_j^(m+1) * V[:, j]   with scale_j = 1 / s_j^2 (0 for masked columns)
-__global__ void neumann_acc_kernel(const double* __restrict__ V, double* __restrict__ Xr,
-                                   const double* __restrict__ inv_s2, int n, int chi, int power) {
-  size_t tot = (size_t)n * chi;
-  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < tot; e += (size_t)gridDim.x * blockDim.x) {
-    int j = e / n;
-    double f = inv_s2[j];
-    double p = f;
-    for (int q = 0; q < power; ++q) p *= f;
-    Xr[e] += p * V[e];
-  }
+  double Fji = -Fij;
+  double gij = g[i + (long long)j * chi];
+  double gji = g[j + (long long)i * chi];
+  double sj = fabs(lj), si = fabs(li);
+  double sigj = lj < 0 ? -1.0 : 1.0, sigi = li < 0 ? -1.0 : 1.0;
+  double R_ij = (Fij * gij + Fji * gji) * sj;   // R11[i,j]
+  double R_ji = (Fji * gji + Fij * gij) * si;   // R11[j,i]
+  double Kt = R_ij * sigj + sigi * R_ji + C1bar[i + (long long)j * chi];
+  coef[e] = 0.5 * Kt;
 }
 
 // W[:, j] += lambda_j * Xr[:, j] + AXr[:, j]
@@ -327,13 +307,33 @@ int topk_eig(Ctx* c, const double* A, int n, int want, int k, const double* X0, 
   }
   orthonormalize(c, w.Y.as<double>(), n, k, X, w, 0x9e3779b9u);
   std::vector<double> hres(k), hth(k);
+  // opt-in trace of the iteration (TNAD_DEBUG_EIG=1), to stderr
+  static const bool dbg = [] {
+    const char* e = std::getenv("TNAD_DEBUG_EIG");
+    return e && *e && *e != '0';
+  }();
   int it = 0;
-  double rel = 0.0;
+  double rel = 0.0, best = HUGE_VAL, best_mark = HUGE_VAL, prev_rel = HUGE_VAL;
+  int best_it = 0, n_jacobi = 0;
+  bool force_jacobi = false;
   for (it = 1; it <= maxit; ++it) {
     // Y = A X ; H = X^T Y ; H = W diag(theta) W^T ; Xr = X W ; Yr = Y W
     tc_launch(c, Prec::F64, gemm_desc('N', 'N', n, k, n, n, n, n), A, X, w.Y.as<double>(), s);
     tc_launch(c, Prec::F64, gemm_desc('T', 'N', k, k, n, n, n, k), X, w.Y.as<double>(), w.H.as<double>(), s);
-    jacobi_eig_ws(w.H.as<double>(), k, k, theta, w.W.as<double>(), w.Vw.as<double>(), 60, nullptr, s);
+    // Rayleigh-Ritz: once the block is nearly an invariant subspace (H nearly diagonal in the
+    // previous Ritz basis) second-order perturbation theory replaces the Jacobi sweeps; Jacobi on
+    // the first cold iteration, whenever H is not close enough to diagonal, and whenever the
+    // residual stopped falling (by 10% per iteration)
+    bool jac = force_jacobi || (it == 1 && !X0);
+    if (!jac) {
+      ritz_perturb(w.H.as<double>(), k, k, want, 1e-3, theta, w.W.as<double>(), w.info.as<int>(), s);
+      int flag = 1;
+      TNAD_HIP_TRY(hipMemcpyAsync(&flag, w.info.ptr, sizeof(int), hipMemcpyDeviceToHost, s));
+      c->sync();
+      jac = flag != 0;
+    }
+    if (jac) jacobi_eig_ws(w.H.as<double>(), k, k, theta, w.W.as<double>(), w.Vw.as<double>(), 60, nullptr, s);
+    n_jacobi += jac;
     tc_launch(c, Prec::F64, gemm_desc('N', 'N', n, k, k, n, k, n), X, w.W.as<double>(), w.Xr.as<double>(), s);
     tc_launch(c, Prec::F64, gemm_desc('N', 'N', n, k, k, n, k, n), w.Y.as<double>(), w.W.as<double>(),
               w.Yr.as<double>(), s);
@@ -346,8 +346,15 @@ int topk_eig(Ctx* c, const double* A, int n, int want, int k, const double* X0, 
     double mx = 0.0;
     for (int j = 0; j < want; ++j) mx = std::max(mx, hres[j]);
     rel = mx / std::max(std::fabs(hth[0]), 1e-300);
+    if (dbg) std::fprintf(stderr, "[tnad eig] n=%d k=%d want=%d it=%d rel_resid=%.3e theta0=%.6e rr=%s\n", n, k,
+                          want, it, rel, hth[0], jac ? "jacobi" : "perturb");
+    force_jacobi = !(rel < 0.9 * prev_rel);
+    prev_rel = rel;
     if (!(rel == rel)) throw Error(TNAD_ERR_NAN, "eigensolver: NaN in Ritz residual");
     if (rel <= tol || it == maxit) break;
+    // stagnation at the rounding level: no halving of the best residual in 8 iterations
+    if (rel < best) { if (rel < 0.5 * best_mark) { best_mark = rel; best_it = it; } best = rel; }
+    if (best < 1e3 * tol && it - best_it >= 8) break;
     hipLaunchKernelGGL(ritz_scale_kernel, dim3(nb((size_t)n * k)), dim3(NT), 0, s, w.Xr.as<double>(),
                        w.Yr.as<double>(), theta, n, k, w.Y.as<double>());
     TNAD_HIP_TRY(hipGetLastError());
@@ -357,7 +364,7 @@ int topk_eig(Ctx* c, const double* A, int n, int want, int k, const double* X0, 
   k_copy(w.Yr.as<double>(), AX, (size_t)n * k, s);
   hipLaunchKernelGGL(canon_sign_kernel, dim3(k), dim3(NT), 0, s, X, AX, n);
   TNAD_HIP_TRY(hipGetLastError());
-  if (st) { st->iters = std::min(it, maxit); st->k = k; st->resid = rel; }
+  if (st) { st->iters = std::min(it, maxit); st->k = k; st->resid = rel; st->jacobi = n_jacobi; }
   return k;
 }
 
@@ -405,7 +412,8 @@ void ctmrg_step(Ctx* c, const StepDims& d, const StepOpts& o, const double* a64,
     k_sym_add(cp64.as<double>(), n, M, s);
   }
   // u, s, v = svd(cpmat) -> top-k eigenpairs by |lambda|        (src/ctmrg.jl:136)
-  const int kreq = (n <= JACOBI_MAX) ? n : std::min(std::max(o.eig_block, chi), JACOBI_MAX);
+  // block size: n (full spectrum) or 1.5 chi guard-extended (eig_block if set), capped at 192
+  const int kreq = step_k(d, o);
   DevBuf Xtmp, AXtmp, thtmp;
   double *X = nullptr, *AX = nullptr, *theta = nullptr;
   if (tape) { X = tape->X.as<double>(); AX = tape->AX.as<double>(); theta = tape->theta.as<double>(); }
@@ -416,7 +424,7 @@ void ctmrg_step(Ctx* c, const StepDims& d, const StepOpts& o, const double* a64,
   EigStats est;
   const bool warm = X0 != nullptr && n > JACOBI_MAX;
   int k = topk_eig(c, M, n, chi, kreq, warm ? X0 : nullptr, warm ? o.eig_warm_iters : o.eig_max_iters,
-                   o.eig_tol, X, AX, theta, &est);
+                   p == Prec::F64 ? o.eig_tol : o.eig_tol_low, X, AX, theta, &est);
   state.k = k;
   if (X_keep) k_copy(X, X_keep, (size_t)n * k, s);
   state.eig_iters = est.iters;
@@ -467,14 +475,18 @@ void ctmrg_step(Ctx* c, const StepDims& d, const StepOpts& o, const double* a64,
     k_copy(C, tape->C.as<double>(), (size_t)chi * chi, s);
     k_copy(T, tape->T.as<double>(), (size_t)chi * D * chi, s);
   }
-  // vals = s ./ s[1]  (computed part of the spectrum; the uncomputed tail is 0)   (src/ctmrg.jl:142)
-  std::vector<double> th(k);
-  TNAD_HIP_TRY(hipMemcpyAsync(th.data(), theta, (size_t)k * 8, hipMemcpyDeviceToHost, s));
+  // vals = s ./ s[1]   (src/ctmrg.jl:142).  With the whole spectrum computed (n <= 192) all n
+  // values; otherwise the kept chi values, the ones the eigensolver converges (its further
+  // Ritz values are not converged), with the rest 0.  Over the oracle's CTMRG runs the
+  // kept part carries 98-99% of the norm of the change of the full vector near convergence.
+  const int nv = (n <= JACOBI_MAX) ? k : std::min(k, chi);
+  std::vector<double> th(nv);
+  TNAD_HIP_TRY(hipMemcpyAsync(th.data(), theta, (size_t)nv * 8, hipMemcpyDeviceToHost, s));
   c->sync();
   vals.assign(n, 0.0);
   double s0 = std::fabs(th[0]);
-  for (int i = 0; i < k; ++i) vals[i] = std::fabs(th[i]) / s0;
-  for (int i = 0; i < k; ++i)
+  for (int i = 0; i < nv; ++i) vals[i] = std::fabs(th[i]) / s0;
+  for (int i = 0; i < nv; ++i)
     if (!(vals[i] == vals[i])) throw Error(TNAD_ERR_NAN, "ctmrgstep: NaN in singular values");
   (void)es;
 }
@@ -484,13 +496,15 @@ void ctmrg_step(Ctx* c, const StepDims& d, const StepOpts& o, const double* a64,
 // -> cotangent of cpmat, i.e. svd_back (src/trg.jl:72-106) with dU nonzero in
 // the first chi columns, dS = dV = nothing, followed by cpmat += cpmat'.
 //
-//   cpbar = W U1^T + U1 W^T,  W = U [Kt/2 ; F.g.(l_i + l_j)] + (Xr Lambda1 + A Xr)
+//   cpbar = W U1^T + U1 W^T,  W = U1 Kt/2 + (X_c Lambda1 + A X_c)
 //
-// where the last term is the contribution of the part of the spectrum the
-// truncated eigensolver did not compute (indices >= k): x_j = P (s_j^2 - A^2)^-1 P G_j
-// is summed as a Neumann series in A^2 / s_j^2 on the complement of the k
-// computed eigenvectors.  For n <= 192 the whole spectrum is known and the
-// formula is exactly the reference's.
+// Kt/2 holds the pairs among the kept chi eigenvectors (reference formula
+// verbatim).  Every other eigenpair i contributes g_ij / (lambda_j - lambda_i)
+// along u_i, i.e. (lambda_j - A)^-1 = (lambda_j + A)(lambda_j^2 - A^2)^-1 on the
+// complement of the kept vectors: X_c solves (lambda_j^2 - A^2) x_j = P G_j there
+// (resolvent_pcg, preconditioned by the extra Ritz pairs the eigensolver carries).
+// With the whole spectrum known (n <= 192) the preconditioner is the exact
+// inverse and the solve ends after its first step.
 // ==========================================================================
 void eig_adjoint(Ctx* c, const StepDims& d, const StepTape& tp, const double* G, const double* C1bar,
                  double* cpbar, EigBackStats* st) {
@@ -498,76 +512,36 @@ void eig_adjoint(Ctx* c, const StepDims& d, const StepTape& tp, const double* G,
   const int chi = d.chi, n = d.n, k = tp.k;
   const double* X = tp.X.as<double>();
   const double* lam = tp.theta.as<double>();
-  DevBuf g(c, (size_t)k * chi * 8), coef(c, (size_t)k * chi * 8), W(c, (size_t)n * chi * 8);
-  // g = U^T G  (k x chi)
-  tc_launch(c, Prec::F64, gemm_desc('T', 'N', k, chi, n, n, n, k), X, G, g.as<double>(), s);
-  hipLaunchKernelGGL(eigback_coef_kernel, dim3((k * chi + NT - 1) / NT), dim3(NT), 0, s, g.as<double>(), lam,
-                     C1bar, k, chi, 1e-40, coef.as<double>());
+  DevBuf g(c, (size_t)chi * chi * 8), coef(c, (size_t)chi * chi * 8), W(c, (size_t)n * chi * 8);
+  // g = U1^T G  (chi x chi)
+  tc_launch(c, Prec::F64, gemm_desc('T', 'N', chi, chi, n, n, n, chi), X, G, g.as<double>(), s);
+  hipLaunchKernelGGL(eigback_coef_kernel, dim3((chi * chi + NT - 1) / NT), dim3(NT), 0, s, g.as<double>(), lam,
+                     C1bar, chi, 1e-40, coef.as<double>());
   TNAD_HIP_TRY(hipGetLastError());
-  // W = U coef  (n x k . k x chi)
-  tc_launch(c, Prec::F64, gemm_desc('N', 'N', n, chi, k, n, k, n), X, coef.as<double>(), W.as<double>(), s);
-  debug_dump(c, "eb_X", X, (size_t)n * k);
-  debug_dump(c, "eb_lam", lam, (size_t)k);
-  debug_dump(c, "eb_G", G, (size_t)n * chi);
-  debug_dump(c, "eb_C1bar", C1bar, (size_t)chi * chi);
-  debug_dump(c, "eb_g", g.as<double>(), (size_t)k * chi);
-  debug_dump(c, "eb_coef", coef.as<double>(), (size_t)k * chi);
-  debug_dump(c, "eb_W0", W.as<double>(), (size_t)n * chi);
-  int terms = 0;
-  if (k < n) {
-    // remainder of the spectrum via the Neumann series
-    std::vector<double> hl(k);
-    TNAD_HIP_TRY(hipMemcpyAsync(hl.data(), lam, (size_t)k * 8, hipMemcpyDeviceToHost, s));
-    c->sync();
-    const double s_tail = std::fabs(hl[k - 1]);   // upper bound of the uncomputed |lambda|
-    const double s_top = std::fabs(hl[0]);
-    std::vector<double> inv_s2(chi, 0.0);
-    double rho_max = 0.0;
-    for (int j = 0; j < chi; ++j) {
-      double sj = std::fabs(hl[j]);
-      double rho = (s_tail * s_tail) / (sj * sj);
-      if (sj > 1e-7 * s_top && rho < 0.5) {   // well separated from the uncomputed tail
-        inv_s2[j] = 1.0 / (sj * sj);
-        rho_max = std::max(rho_max, rho);
-      }
-    }
-    if (rho_max > 0.0 || inv_s2[0] != 0.0) {
-      terms = (rho_max > 0.0) ? (int)std::ceil(std::log(1e-17) / std::log(rho_max)) : 1;
-      terms = std::max(1, std::min(terms, 80));
-      DevBuf is2(c, (size_t)chi * 8), V(c, (size_t)n * chi * 8), V2(c, (size_t)n * chi * 8),
-          Xr(c, (size_t)n * chi * 8), pj(c, (size_t)k * chi * 8);
-      TNAD_HIP_TRY(hipMemcpyAsync(is2.ptr, inv_s2.data(), (size_t)chi * 8, hipMemcpyHostToDevice, s));
-      // V0 = P G = G - U (U^T G)
-      k_copy(G, V.as<double>(), (size_t)n * chi, s);
-      tc_launch(c, Prec::F64, gemm_desc('N', 'N', n, chi, k, n, k, n, -1.0, 1.0), X, g.as<double>(), V.as<double>(), s);
-      k_fill(Xr.as<double>(), (size_t)n * chi, 0.0, s);
-      for (int m = 0; m < terms; ++m) {
-        hipLaunchKernelGGL(neumann_acc_kernel, dim3(nb((size_t)n * chi)), dim3(NT), 0, s, V.as<double>(),
-                           Xr.as<double>(), is2.as<double>(), n, chi, m);
-        if (m + 1 == terms) break;
-        // V <- P A (A V)
-        tc_launch(c, Prec::F64, gemm_desc('N', 'N', n, chi, n, n, n, n), tp.M.as<double>(), V.as<double>(),
-                  V2.as<double>(), s);
-        tc_launch(c, Prec::F64, gemm_desc('N', 'N', n, chi, n, n, n, n), tp.M.as<double>(), V2.as<double>(),
-                  V.as<double>(), s);
-        tc_launch(c, Prec::F64, gemm_desc('T', 'N', k, chi, n, n, n, k), X, V.as<double>(), pj.as<double>(), s);
-        tc_launch(c, Prec::F64, gemm_desc('N', 'N', n, chi, k, n, k, n, -1.0, 1.0), X, pj.as<double>(),
-                  V.as<double>(), s);
-      }
-      // W += Xr Lambda1 + A Xr
-      tc_launch(c, Prec::F64, gemm_desc('N', 'N', n, chi, n, n, n, n), tp.M.as<double>(), Xr.as<double>(),
-                V2.as<double>(), s);
-      hipLaunchKernelGGL(add_lam_cols_kernel, dim3(nb((size_t)n * chi)), dim3(NT), 0, s, W.as<double>(),
-                         Xr.as<double>(), V2.as<double>(), lam, n, chi);
-      TNAD_HIP_TRY(hipGetLastError());
-    }
+  // W = U1 coef  (n x chi . chi x chi)
+  tc_launch(c, Prec::F64, gemm_desc('N', 'N', n, chi, chi, n, chi, n), X, coef.as<double>(), W.as<double>(), s);
+  int iters = 0;
+  if (n > chi) {
+    DevBuf sq(c, (size_t)k * 8), Xc(c, (size_t)n * chi * 8), AXc(c, (size_t)n * chi * 8),
+        tmp(c, (size_t)n * chi * 8);
+    k_square(lam, sq.as<double>(), k, s);
+    const double* M = tp.M.as<double>();
+    auto applyA2 = [&](const double* V, double* out) {
+      tc_launch(c, Prec::F64, gemm_desc('N', 'N', n, chi, n, n, n, n), M, V, tmp.as<double>(), s);
+      tc_launch(c, Prec::F64, gemm_desc('N', 'N', n, chi, n, n, n, n), M, tmp.as<double>(), out, s);
+    };
+    iters = resolvent_pcg(c, n, chi, k, X, sq.as<double>(), sq.as<double>() + chi, applyA2, G, Xc.as<double>(),
+                          1e-13, 200);
+    // W += X_c Lambda1 + A X_c
+    tc_launch(c, Prec::F64, gemm_desc('N', 'N', n, chi, n, n, n, n), M, Xc.as<double>(), AXc.as<double>(), s);
+    hipLaunchKernelGGL(add_lam_cols_kernel, dim3(nb((size_t)n * chi)), dim3(NT), 0, s, W.as<double>(),
+                       Xc.as<double>(), AXc.as<double>(), lam, n, chi);
+    TNAD_HIP_TRY(hipGetLastError());
   }
   // cpbar = W U1^T + U1 W^T
   tc_launch(c, Prec::F64, gemm_desc('N', 'T', n, n, chi, n, n, n), W.as<double>(), X, cpbar, s);
   tc_launch(c, Prec::F64, gemm_desc('N', 'T', n, n, chi, n, n, n, 1.0, 1.0), X, W.as<double>(), cpbar, s);
-  debug_dump(c, "eb_W", W.as<double>(), (size_t)n * chi);
-  debug_dump(c, "eb_cpbar", cpbar, (size_t)n * n);
-  if (st) st->neumann_terms = terms;
+  if (st) st->pcg_iters = iters;
 }
 
 // ==========================================================================

@@ -1,21 +1,22 @@
-// Small dense FP64 kernels for the eigensolver: one workgroup each, the
-// working matrix held as a packed triangle in LDS (192x192 FP64 = 148 KB of
-// the 160 KB a CU offers).
+// Small dense FP64 kernels for the eigensolver, one workgroup each (m <= 192).
 //
-//  * jacobi_eig_kernel : cyclic two-sided Jacobi with the round-robin
-//    (circle) ordering, so each round applies m/2 disjoint rotations in
-//    parallel; every 2x2 block of the matrix is updated by one thread.  The
-//    eigenvector matrix accumulates in global memory.  Used for the full
-//    spectrum when n <= 192 and for the Rayleigh-Ritz problem of the block
-//    subspace iteration.
-//  * chol_inv_kernel   : Cholesky of a (shifted) Gram matrix followed by an
-//    in-place triangular inverse, for CholeskyQR orthonormalisation.
+//  * jacobi_rot_kernel / jacobi_apply_kernel : cyclic two-sided Jacobi with the
+//    round-robin (circle) ordering on a packed triangle in LDS (192x192 FP64 =
+//    148 KB of the 160 KB a CU offers), so each round applies m/2 disjoint
+//    rotations in parallel and every 2x2 block is updated by one thread; the
+//    rotations are recorded and replayed on the identity by many workgroups.
+//    Used for the full spectrum when n <= 192 and for the Rayleigh-Ritz problem
+//    of the block subspace iteration.
+//  * chol_inv_kernel : Cholesky of a (shifted) Gram matrix followed by the
+//    triangular inverse, register-resident, for CholeskyQR orthonormalisation.
 #include "tnad_internal.h"
 
 namespace tnad {
 namespace {
 
 constexpr int EIG_NT = 1024;
+// 2x2 blocks per thread per Jacobi round: ceil(P (P + 1) / 2 / EIG_NT) for P = JACOBI_MAX / 2
+constexpr int JB_MAX = ((JACOBI_MAX / 2) * (JACOBI_MAX / 2 + 1) / 2 + EIG_NT - 1) / EIG_NT;
 
 __device__ __forceinline__ int pk(int i, int j) {  // packed upper, i <= j, column-major
   return j * (j + 1) / 2 + i;
@@ -84,6 +85,18 @@ __global__ __launch_bounds__(EIG_NT) void jacobi_rot_kernel(const double* __rest
   __syncthreads();
   const double abs_floor = 1e-18 * fro_s + 1e-300;
   const double rel_tol = 1e-15;
+  // unordered slot pairs (i <= j) handled by this thread, packed i | j << 16 (-1: none)
+  int bij[JB_MAX];
+  {
+    const int nblk = P * (P + 1) / 2;
+#pragma unroll
+    for (int u = 0; u < JB_MAX; ++u) {
+      const int b = t + u * EIG_NT;
+      int i = 0, j = 0;
+      if (b < nblk) decode_tri(b, i, j);
+      bij[u] = (b < nblk) ? (i | (j << 16)) : -1;
+    }
+  }
 
   int round = 0;
   int sweep = 0;
@@ -111,11 +124,12 @@ __global__ __launch_bounds__(EIG_NT) void jacobi_rot_kernel(const double* __rest
         rot[((size_t)round * P + t) * 2 + 1] = sn;
       }
       __syncthreads();
-      // 2x2 block updates A' = Ji^T A Jj over unordered pairs of rotation slots
-      const int nblk = P * (P + 1) / 2;
-      for (int b = t; b < nblk; b += EIG_NT) {
-        int i, j;
-        decode_tri(b, i, j);
+      // 2x2 block updates A' = Ji^T A Jj over unordered pairs of rotation slots (this thread's
+      // slot pairs are fixed for the whole run and were decoded once, see bij)
+#pragma unroll
+      for (int u = 0; u < JB_MAX; ++u) {
+        if (bij[u] < 0) continue;
+        const int i = bij[u] & 0xffff, j = bij[u] >> 16;
         const double ci = rc[i], si = rs[i], cj = rc[j], sj = rs[j];
         if (si == 0.0 && sj == 0.0) continue;
         const int p = rp[i], q = rq[i];
@@ -227,70 +241,244 @@ __global__ __launch_bounds__(EIG_NT) void jacobi_sort_kernel(const double* __res
   }
 }
 
-// Cholesky (with diagonal shift) + in-place triangular inverse.  Output Rinv = L^{-T}
-// (upper triangular, full m x m column-major, zeros below the diagonal).
-__global__ __launch_bounds__(EIG_NT) void chol_inv_kernel(const double* __restrict__ G, int m,
-                                                          double shift, double* __restrict__ Rinv,
-                                                          int* info, double* __restrict__ piv) {
+// Cholesky (with diagonal shift) + triangular inverse of an m x m SPD matrix, m <= 192,
+// in one workgroup of 32 x 32 threads.  Thread (ty, tx) holds the 6 x 6 elements
+// (ty + 32a, tx + 32b) in registers (cyclic distribution, so every step keeps all
+// threads busy); each elimination step broadcasts one column through LDS and costs
+// one barrier (double-buffered).  L is then parked in LDS (packed lower) and
+// B = L^-1 is formed by right-looking forward substitution with the same register
+// layout.  Output Rinv = L^-T (upper triangular, column-major, zeros below the
+// diagonal) so that X = Y Rinv; piv (nullable) receives diag(L).
+constexpr int CH_G = 32;                  // thread grid edge
+constexpr int CH_R = JACOBI_MAX / CH_G;   // elements per thread per dimension
+static_assert(CH_R * CH_G == JACOBI_MAX, "Cholesky tiling");
+__global__ __launch_bounds__(CH_G * CH_G) void chol_inv_kernel(const double* __restrict__ G, int m,
+                                                               double shift, double* __restrict__ Rinv,
+                                                               int* info, double* __restrict__ piv) {
   extern __shared__ double smem[];
-  // packed lower, column-major: element (i,j), i >= j at  j*m - j(j-1)/2 + (i-j)
-  double* L = smem;
-  double* col = L + m * (m + 1) / 2;
+  double* Lp = smem;                                 // packed lower, column-major
+  double* cb = Lp + (size_t)m * (m + 1) / 2;        // 2 x JACOBI_MAX broadcast buffers
   __shared__ int bad;
-  const int t = threadIdx.x;
+  const int t = threadIdx.x, tx = t % CH_G, ty = t / CH_G;
   auto pl = [m](int i, int j) { return j * m - j * (j - 1) / 2 + (i - j); };
-  for (int j = 0; j < m; ++j)
-    for (int i = j + t; i < m; i += EIG_NT) L[pl(i, j)] = G[i + (long long)j * m] + (i == j ? shift : 0.0);
+  double A[CH_R][CH_R];
+#pragma unroll
+  for (int a = 0; a < CH_R; ++a) {
+#pragma unroll
+    for (int b = 0; b < CH_R; ++b) {
+      const int i = ty + CH_G * a, j = tx + CH_G * b;
+      double v = (i == j) ? 1.0 : 0.0;   // padding beyond m: identity, never coupled to the rest
+      if (i < m && j < m) v = G[j + (size_t)m * i] + (i == j ? shift : 0.0);
+      A[a][b] = v;
+    }
+  }
   if (t == 0) bad = 0;
   __syncthreads();
-  // right-looking Cholesky
-  for (int j = 0; j < m; ++j) {
-    if (t == 0) {
-      double d = L[pl(j, j)];
-      if (!(d > 0.0)) {
-        if (bad == 0) bad = j + 1;
-        d = 1e-300;
+  // ---- right-looking Cholesky ----
+  for (int jj = 0; jj < m; ++jj) {
+    double* col = cb + (jj & 1) * JACOBI_MAX;
+    const int bj = jj / CH_G, tj = jj % CH_G;
+    if (tx == tj) {
+#pragma unroll
+      for (int a = 0; a < CH_R; ++a) {
+        const int i = ty + CH_G * a;
+        if (i >= jj && i < m) {
+          double v = 0.0;
+#pragma unroll
+          for (int b = 0; b < CH_R; ++b) v = (b == bj) ? A[a][b] : v;
+          col[i] = v;
+        }
       }
-      L[pl(j, j)] = sqrt(d);
-      if (piv) piv[j] = L[pl(j, j)];
     }
     __syncthreads();
-    double djj = L[pl(j, j)];
-    for (int i = j + 1 + t; i < m; i += EIG_NT) L[pl(i, j)] /= djj;
-    __syncthreads();
-    int rem = m - j - 1;
-    int tot = rem * (rem + 1) / 2;
-    for (int b = t; b < tot; b += EIG_NT) {
-      // (i, k) with j < k <= i < m
-      int ii, kk;
-      decode_tri(b, kk, ii);    // kk <= ii
-      int i = j + 1 + ii, k = j + 1 + kk;
-      L[pl(i, k)] -= L[pl(i, j)] * L[pl(k, j)];
+    double d = col[jj];
+    if (!(d > 0.0)) {
+      if (t == 0 && bad == 0) bad = jj + 1;
+      d = 1e-300;
+    }
+    const double sd = sqrt(d);
+    const double isd = 1.0 / sd;
+    if (t == 0 && piv) piv[jj] = sd;
+    double li[CH_R], lj[CH_R];
+#pragma unroll
+    for (int a = 0; a < CH_R; ++a) {
+      const int i = ty + CH_G * a;
+      li[a] = (i > jj && i < m) ? col[i] * isd : 0.0;
+    }
+#pragma unroll
+    for (int b = 0; b < CH_R; ++b) {
+      const int j = tx + CH_G * b;
+      lj[b] = (j > jj && j < m) ? col[j] * isd : 0.0;
+    }
+#pragma unroll
+    for (int a = 0; a < CH_R; ++a)
+#pragma unroll
+      for (int b = 0; b < CH_R; ++b) A[a][b] -= li[a] * lj[b];
+    if (tx == tj) {
+#pragma unroll
+      for (int a = 0; a < CH_R; ++a) {
+        const int i = ty + CH_G * a;
+        const double lv = (i == jj) ? sd : li[a];
+#pragma unroll
+        for (int b = 0; b < CH_R; ++b)
+          if (b == bj && i >= jj) A[a][b] = lv;
+      }
+    }
+  }
+  // park L (lower part of A) in LDS
+#pragma unroll
+  for (int a = 0; a < CH_R; ++a)
+#pragma unroll
+    for (int b = 0; b < CH_R; ++b) {
+      const int i = ty + CH_G * a, j = tx + CH_G * b;
+      if (i < m && j <= i) Lp[pl(i, j)] = A[a][b];
+    }
+  __syncthreads();
+  // ---- B = L^-1: row jj = B[jj,:] / L[jj,jj]; rows i > jj: B[i,:] -= L[i,jj] B[jj,:] ----
+#pragma unroll
+  for (int a = 0; a < CH_R; ++a)
+#pragma unroll
+    for (int b = 0; b < CH_R; ++b) A[a][b] = (ty + CH_G * a == tx + CH_G * b) ? 1.0 : 0.0;
+  for (int jj = 0; jj < m; ++jj) {
+    double* row = cb + (jj & 1) * JACOBI_MAX;
+    const int aj = jj / CH_G, tj = jj % CH_G;
+    const double ild = 1.0 / Lp[pl(jj, jj)];
+    if (ty == tj) {
+#pragma unroll
+      for (int a = 0; a < CH_R; ++a) {
+        if (a != aj) continue;
+#pragma unroll
+        for (int b = 0; b < CH_R; ++b) {
+          const int j = tx + CH_G * b;
+          const double v = A[a][b] * ild;
+          A[a][b] = v;
+          if (j < m) row[j] = v;
+        }
+      }
     }
     __syncthreads();
-  }
-  // in-place inverse, columns from last to first:
-  // X[j,j] = 1/L[j,j];  X[i,j] = -X[j,j] * sum_{k=j+1..i} X[i,k] L[k,j]
-  for (int j = m - 1; j >= 0; --j) {
-    for (int i = j + t; i < m; i += EIG_NT) col[i] = L[pl(i, j)];
-    __syncthreads();
-    double xjj = 1.0 / col[j];
-    for (int i = j + 1 + t; i < m; i += EIG_NT) {
-      double acc = 0.0;
-      for (int k = j + 1; k <= i; ++k) acc += L[pl(i, k)] * col[k];
-      L[pl(i, j)] = -xjj * acc;
+    double xr[CH_R];
+#pragma unroll
+    for (int b = 0; b < CH_R; ++b) {
+      const int j = tx + CH_G * b;
+      xr[b] = (j <= jj) ? row[j] : 0.0;   // row jj of B is zero right of the diagonal
     }
-    if (t == 0) L[pl(j, j)] = xjj;
-    __syncthreads();
+#pragma unroll
+    for (int a = 0; a < CH_R; ++a) {
+      const int i = ty + CH_G * a;
+      const double lij = (i > jj && i < m) ? Lp[pl(i, jj)] : 0.0;
+#pragma unroll
+      for (int b = 0; b < CH_R; ++b) A[a][b] -= lij * xr[b];
+    }
   }
-  // Rinv = X^T : Rinv[r, c] = X[c, r] for r <= c
-  for (int e = t; e < m * m; e += EIG_NT) {
-    int r = e % m, c = e / m;
-    Rinv[e] = (r <= c) ? L[pl(c, r)] : 0.0;
-  }
+  // Rinv(r, c) = B(c, r)
+#pragma unroll
+  for (int a = 0; a < CH_R; ++a)
+#pragma unroll
+    for (int b = 0; b < CH_R; ++b) {
+      const int i = ty + CH_G * a, j = tx + CH_G * b;
+      if (i < m && j < m) Rinv[j + (size_t)m * i] = (j <= i) ? A[a][b] : 0.0;
+    }
   if (t == 0 && info) *info = bad;
 }
 
+
+// Rayleigh-Ritz of a nearly diagonal symmetric H (m <= 192, one workgroup) for its first nh
+// ("kept") columns: with d = diag(H) and E the off-diagonal part restricted to pairs that involve
+// a kept column, when max |E_ij| / |d_i - d_j| over those pairs is <= thresh the eigenpairs are,
+// to second order in E,
+//   theta_j = d_j + sum_{i != j} E_ij^2 / (d_j - d_i),   v_j = (e_j + sum_{i != j} e_i E_ij / (d_j - d_i)) / norm,
+// written sorted by |theta| descending (stable).  Pairs among the further columns (i, j >= nh) are
+// left unrotated: those vectors are not converged and only widen the search space, which any
+// mixing among them leaves unchanged -- provided the further block cannot hold a direction that
+// belongs among the kept ones, which its Gershgorin bound (below the smallest kept |d_i| by 0.1%)
+// rules out.  Otherwise *flag = 1 and nothing else is written (the caller falls back to Jacobi).  The vectors are orthonormal to O((E/gap)^2); the subspace iteration's
+// residual test sees any remainder.
+__global__ __launch_bounds__(EIG_NT) void ritz_perturb_kernel(const double* __restrict__ H, int ldh, int m,
+                                                              int nh, double thresh, double* __restrict__ evals,
+                                                              double* __restrict__ V, int* __restrict__ flag) {
+  __shared__ double dg[JACOBI_MAX], th[JACOBI_MAX], nr[JACOBI_MAX];
+  __shared__ int rk[JACOBI_MAX];
+  __shared__ double red[EIG_NT / 64];
+  __shared__ int bad;
+  const int t = threadIdx.x;
+  for (int i = t; i < m; i += EIG_NT) dg[i] = H[i + (long long)ldh * i];
+  __syncthreads();
+  double worst = 0.0;
+  for (int e = t; e < m * m; e += EIG_NT) {
+    const int i = e % m, j = e / m;
+    if (i >= j || i >= nh) continue;   // i < j: pairs with a kept column have i < nh
+    const double eij = 0.5 * (H[i + (long long)ldh * j] + H[j + (long long)ldh * i]);
+    const double gap = fabs(dg[i] - dg[j]);
+    const double r = (eij == 0.0) ? 0.0 : (gap > 0.0 ? fabs(eij) / gap : HUGE_VAL);
+    worst = fmax(worst, r);
+  }
+  // the unrotated further columns must not hide a direction that belongs among the kept ones:
+  // Gershgorin bound of the further block below the smallest kept |diagonal| by a margin
+  double gmax = 0.0;
+  for (int i = nh + t; i < m; i += EIG_NT) {
+    double rs = 0.0;
+    for (int j = nh; j < m; ++j) rs += fabs(i == j ? dg[i] : 0.5 * (H[i + (long long)ldh * j] + H[j + (long long)ldh * i]));
+    gmax = fmax(gmax, rs);
+  }
+  double hmin = HUGE_VAL;
+  for (int i = t; i < nh; i += EIG_NT) hmin = fmin(hmin, fabs(dg[i]));
+  for (int o = 32; o > 0; o >>= 1) {
+    worst = fmax(worst, __shfl_down(worst, o));
+    gmax = fmax(gmax, __shfl_down(gmax, o));
+    hmin = fmin(hmin, __shfl_down(hmin, o));
+  }
+  __shared__ double red2[EIG_NT / 64], red3[EIG_NT / 64];
+  if ((t & 63) == 0) { red[t >> 6] = worst; red2[t >> 6] = gmax; red3[t >> 6] = hmin; }
+  __syncthreads();
+  if (t == 0) {
+    double w = 0.0, g = 0.0, hm = HUGE_VAL;
+    for (int q = 0; q < EIG_NT / 64; ++q) { w = fmax(w, red[q]); g = fmax(g, red2[q]); hm = fmin(hm, red3[q]); }
+    bad = !(w <= thresh) || !(g < (1.0 - 1e-3) * hm);
+    *flag = bad;
+  }
+  __syncthreads();
+  if (bad) return;
+  auto E = [&](int i, int j) {
+    return (i >= nh && j >= nh) ? 0.0 : 0.5 * (H[i + (long long)ldh * j] + H[j + (long long)ldh * i]);
+  };
+  for (int j = t; j < m; j += EIG_NT) {
+    double acc = 0.0, n2 = 1.0;
+    for (int i = 0; i < m; ++i) {
+      if (i == j) continue;
+      const double eij = E(i, j);
+      if (eij == 0.0) continue;
+      const double kij = eij / (dg[j] - dg[i]);
+      acc += eij * kij;
+      n2 += kij * kij;
+    }
+    th[j] = dg[j] + acc;
+    nr[j] = 1.0 / sqrt(n2);
+  }
+  __syncthreads();
+  for (int j = t; j < m; j += EIG_NT) {
+    const double aj = fabs(th[j]);
+    int r = 0;
+    for (int l = 0; l < m; ++l) {
+      const double al = fabs(th[l]);
+      r += (al > aj) || (al == aj && l < j);
+    }
+    rk[j] = r;
+    evals[r] = th[j];
+  }
+  __syncthreads();
+  for (int e = t; e < m * m; e += EIG_NT) {
+    const int i = e % m, j = e / m;
+    double v;
+    if (i == j) {
+      v = 1.0;
+    } else {
+      const double eij = E(i, j);
+      v = (eij == 0.0) ? 0.0 : eij / (dg[j] - dg[i]);
+    }
+    V[i + (long long)m * rk[j]] = v * nr[j];
+  }
+}
 }  // namespace
 
 static size_t jacobi_lds(int m) {
@@ -328,12 +516,19 @@ void jacobi_eig_ws(const double* H, int ldh, int m, double* evals, double* V, do
   TNAD_HIP_TRY(hipGetLastError());
 }
 
+void ritz_perturb(const double* H, int ldh, int m, int nh, double thresh, double* evals, double* V, int* flag,
+                  hipStream_t s) {
+  if (m < 1 || m > JACOBI_MAX || nh < 1 || nh > m) throw Error(TNAD_ERR_ARG, "ritz_perturb: size out of range");
+  hipLaunchKernelGGL(ritz_perturb_kernel, dim3(1), dim3(EIG_NT), 0, s, H, ldh, m, nh, thresh, evals, V, flag);
+  TNAD_HIP_TRY(hipGetLastError());
+}
+
 void chol_inv(const double* G, int m, double shift, double* Rinv, int* info, double* piv, hipStream_t s) {
   if (m < 1 || m > JACOBI_MAX) throw Error(TNAD_ERR_ARG, "chol_inv: size out of range");
-  size_t lds = ((size_t)m * (m + 1) / 2 + m) * sizeof(double);
+  size_t lds = ((size_t)m * (m + 1) / 2 + 2 * JACOBI_MAX) * sizeof(double);
   TNAD_HIP_TRY(hipFuncSetAttribute((const void*)chol_inv_kernel,
                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-  hipLaunchKernelGGL(chol_inv_kernel, dim3(1), dim3(EIG_NT), lds, s, G, m, shift, Rinv, info, piv);
+  hipLaunchKernelGGL(chol_inv_kernel, dim3(1), dim3(CH_G * CH_G), lds, s, G, m, shift, Rinv, info, piv);
   TNAD_HIP_TRY(hipGetLastError());
 }
 

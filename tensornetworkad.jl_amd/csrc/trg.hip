@@ -11,10 +11,9 @@
 // eigensolver as the CTMRG corner matrix (full spectrum up to 192, top-k
 // subspace iteration beyond).
 //
-// Reverse: svd_back (src/trg.jl:72-106) is realised as the adjoint of the
-// symmetric eigendecomposition of B (equal to the SVD adjoint for distinct
-// singular values), with the uncomputed part of the spectrum summed as a
-// Neumann series of the resolvent; M-bar = B-bar[0:m, m:] + B-bar[m:, 0:m]^T.
+// Reverse: svd_back (src/trg.jl:72-106) in its SVD form: the pairs among the
+// kept triplets by the reference's formula, the rest of the spectrum (and the
+// null spaces) by preconditioned resolvent solves with M M^T and M^T M.
 #include "trg.h"
 
 #include <algorithm>
@@ -23,6 +22,7 @@
 #include <vector>
 
 #include "ctmrg.h"
+#include "resolvent.h"
 
 namespace tnad {
 namespace {
@@ -186,18 +186,6 @@ __global__ void scale_cols_by_kernel(double* __restrict__ X, int rows, int cols,
     X[e] *= sv[e / rows];
 }
 
-// X[:, j] += inv_s2[j]^(q+1) V[:, j]
-__global__ void neumann_acc_trg_kernel(const double* __restrict__ V, double* __restrict__ X,
-                                       const double* __restrict__ inv_s2, int rows, int cols, int power) {
-  size_t tot = (size_t)rows * cols;
-  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < tot; e += (size_t)gridDim.x * blockDim.x) {
-    double f = inv_s2[e / rows];
-    double p = f;
-    for (int q = 0; q < power; ++q) p *= f;
-    X[e] += p * V[e];
-  }
-}
-
 // R (K x K) of svd_back (src/trg.jl:72-106) restricted to the K computed singular triplets,
 // cotangents dU, dV on the first kk columns (gu = U^T dU, gv = V^T dV, K x kk), dS on kk:
 //   i,j < kk : (F_ij gu_ij + F_ji gu_ji) s_j + s_i (F_ij gv_ij + F_ji gv_ji) + delta_ij dS_j
@@ -350,11 +338,14 @@ static void svd_split(Ctx* c, const double* t, int d1, int d2, int d3, int d4, i
 }
 
 // svd_back (src/trg.jl:72-106) for u = U sqrt(s), v = sqrt(s) V^T: cotangents of u (m x k) and
-// v (k x mp) -> cotangent of M (m x mp).  Pairs among the K computed triplets use the
-// reference's formula verbatim; the uncomputed part of the spectrum adds, for each kept j,
+// v (k x mp) -> cotangent of M (m x mp).  Pairs among the kk kept, resolved triplets use the
+// reference's formula verbatim (R, kk x kk).  Every other singular pair, and the null spaces
+// of a rectangular or rank-deficient M, enter through
 //   x_j = P_U (s_j^2 - M M^T)^-1 P_U (s_j dU_j + M dV_j)   (times v_j^T)
 //   y_j = P_V (s_j^2 - M^T M)^-1 P_V (M^T dU_j + s_j dV_j)  (u_j times y_j^T)
-// summed as Neumann series; P_U, P_V project out the computed singular vectors.
+// with P_U, P_V the projectors off the kept vectors, solved by resolvent_pcg with the
+// further computed triplets as preconditioner.  Kept columns beyond the resolved spectrum
+// (zero singular values, zero columns of u and v) carry no cotangent.
 static void svd_split_back(Ctx* c, const SvdTape& tp, const double* ubar, const double* vbar, double* tbar) {
   hipStream_t s = c->stream;
   const int m = tp.m, mp = tp.mp, k = tp.k, K = tp.K;
@@ -373,74 +364,54 @@ static void svd_split_back(Ctx* c, const SvdTape& tp, const double* ubar, const 
   hipLaunchKernelGGL(unsplit_kernel, dim3(nb((size_t)N * kk)), dim3(NT), 0, s, Wtmp.as<double>(), m, mp, kk,
                      dU.as<double>(), dV.as<double>());
   TNAD_HIP_TRY(hipGetLastError());
-  DevBuf gu(c, (size_t)K * kk * 8), gv(c, (size_t)K * kk * 8), R(c, (size_t)K * K * 8), UR(c, (size_t)m * K * 8);
-  tc_launch(c, Prec::F64, gemm_desc('T', 'N', K, kk, m, m, m, K), U, dU.as<double>(), gu.as<double>(), s);
-  tc_launch(c, Prec::F64, gemm_desc('T', 'N', K, kk, mp, mp, mp, K), V, dV.as<double>(), gv.as<double>(), s);
-  hipLaunchKernelGGL(svdback_R_kernel, dim3((K * K + NT - 1) / NT), dim3(NT), 0, s, gu.as<double>(), gv.as<double>(),
-                     tp.sv.as<double>(), dS.as<double>(), K, kk, 1e-40, R.as<double>());
+  DevBuf gu(c, (size_t)kk * kk * 8), gv(c, (size_t)kk * kk * 8), R(c, (size_t)kk * kk * 8), UR(c, (size_t)m * kk * 8);
+  tc_launch(c, Prec::F64, gemm_desc('T', 'N', kk, kk, m, m, m, kk), U, dU.as<double>(), gu.as<double>(), s);
+  tc_launch(c, Prec::F64, gemm_desc('T', 'N', kk, kk, mp, mp, mp, kk), V, dV.as<double>(), gv.as<double>(), s);
+  hipLaunchKernelGGL(svdback_R_kernel, dim3((kk * kk + NT - 1) / NT), dim3(NT), 0, s, gu.as<double>(),
+                     gv.as<double>(), tp.sv.as<double>(), dS.as<double>(), kk, kk, 1e-40, R.as<double>());
   TNAD_HIP_TRY(hipGetLastError());
-  // Mbar = U R V^T
-  tc_launch(c, Prec::F64, gemm_desc('N', 'N', m, K, K, m, K, m), U, R.as<double>(), UR.as<double>(), s);
-  tc_launch(c, Prec::F64, gemm_desc('N', 'T', m, mp, K, m, mp, m), UR.as<double>(), V, tbar, s);
-  if (K >= std::min(m, mp) && m == mp) return;   // full spectrum: nothing left
-  // remainder of the spectrum
-  std::vector<double> inv_s2(kk, 0.0);
-  const double s0 = tp.hs[0], sK = tp.hs[K - 1];
-  double rho = 0.0;
-  bool any = false;
-  for (int j = 0; j < kk; ++j) {
-    const double sj = tp.hs[j];
-    const double r = (K < std::min(m, mp)) ? (sK * sK) / (sj * sj) : 0.0;
-    if (sj > 1e-7 * s0 && r < 0.5) { inv_s2[j] = 1.0 / (sj * sj); rho = std::max(rho, r); any = true; }
-  }
-  if (!any) return;
-  const int terms = rho > 0.0 ? std::max(1, std::min(200, (int)std::ceil(std::log(1e-17) / std::log(rho)))) : 1;
-  DevBuf is2(c, (size_t)kk * 8), sdev(c, (size_t)kk * 8);
-  TNAD_HIP_TRY(hipMemcpyAsync(is2.ptr, inv_s2.data(), (size_t)kk * 8, hipMemcpyHostToDevice, s));
+  // Mbar = U_kk R V_kk^T
+  tc_launch(c, Prec::F64, gemm_desc('N', 'N', m, kk, kk, m, kk, m), U, R.as<double>(), UR.as<double>(), s);
+  tc_launch(c, Prec::F64, gemm_desc('N', 'T', m, mp, kk, m, mp, m), UR.as<double>(), V, tbar, s);
+  if (m == kk && mp == kk) return;   // square, fully kept: nothing outside
   const double* M = tp.M.as<double>();
-  // left: b = s dU + M dV ; right: b' = M^T dU + s dV
-  DevBuf bl(c, (size_t)m * kk * 8), br(c, (size_t)mp * kk * 8);
-  k_copy(dU.as<double>(), bl.as<double>(), (size_t)m * kk, s);
-  hipLaunchKernelGGL(scale_cols_by_kernel, dim3(nb((size_t)m * kk)), dim3(NT), 0, s, bl.as<double>(), m, kk,
-                     tp.sv.as<double>());
-  tc_launch(c, Prec::F64, gemm_desc('N', 'N', m, kk, mp, m, mp, m, 1.0, 1.0), M, dV.as<double>(), bl.as<double>(), s);
-  k_copy(dV.as<double>(), br.as<double>(), (size_t)mp * kk, s);
-  hipLaunchKernelGGL(scale_cols_by_kernel, dim3(nb((size_t)mp * kk)), dim3(NT), 0, s, br.as<double>(), mp, kk,
-                     tp.sv.as<double>());
-  tc_launch(c, Prec::F64, gemm_desc('T', 'N', mp, kk, m, m, m, mp, 1.0, 1.0), M, dU.as<double>(), br.as<double>(), s);
-  TNAD_HIP_TRY(hipGetLastError());
-  auto neumann = [&](const double* Q, int rows, DevBuf& b, bool left) {
-    // x = sum_q s^-2(q+1) (P G P)^q P b with G = M M^T (left) or M^T M (right), P = I - Q Q^T
-    DevBuf pj(c, (size_t)K * kk * 8), V1(c, (size_t)rows * kk * 8), V2(c, (size_t)rows * kk * 8),
-        T1(c, (size_t)(left ? mp : m) * kk * 8), X(c, (size_t)rows * kk * 8);
-    k_copy(b.as<double>(), V1.as<double>(), (size_t)rows * kk, s);
-    tc_launch(c, Prec::F64, gemm_desc('T', 'N', K, kk, rows, rows, rows, K), Q, V1.as<double>(), pj.as<double>(), s);
-    tc_launch(c, Prec::F64, gemm_desc('N', 'N', rows, kk, K, rows, K, rows, -1.0, 1.0), Q, pj.as<double>(),
-              V1.as<double>(), s);
-    k_fill(X.as<double>(), (size_t)rows * kk, 0.0, s);
-    for (int q = 0; q < terms; ++q) {
-      hipLaunchKernelGGL(neumann_acc_trg_kernel, dim3(nb((size_t)rows * kk)), dim3(NT), 0, s, V1.as<double>(),
-                         X.as<double>(), is2.as<double>(), rows, kk, q);
-      if (q + 1 == terms) break;
-      if (left) {   // V2 = M (M^T V1)
-        tc_launch(c, Prec::F64, gemm_desc('T', 'N', mp, kk, m, m, m, mp), M, V1.as<double>(), T1.as<double>(), s);
-        tc_launch(c, Prec::F64, gemm_desc('N', 'N', m, kk, mp, m, mp, m), M, T1.as<double>(), V2.as<double>(), s);
-      } else {      // V2 = M^T (M V1)
-        tc_launch(c, Prec::F64, gemm_desc('N', 'N', m, kk, mp, m, mp, m), M, V1.as<double>(), T1.as<double>(), s);
-        tc_launch(c, Prec::F64, gemm_desc('T', 'N', mp, kk, m, m, m, mp), M, T1.as<double>(), V2.as<double>(), s);
-      }
-      tc_launch(c, Prec::F64, gemm_desc('T', 'N', K, kk, rows, rows, rows, K), Q, V2.as<double>(), pj.as<double>(), s);
-      tc_launch(c, Prec::F64, gemm_desc('N', 'N', rows, kk, K, rows, K, rows, -1.0, 1.0), Q, pj.as<double>(),
-                V2.as<double>(), s);
-      std::swap(V1, V2);
-    }
-    return X;
-  };
-  DevBuf Xl = neumann(U, m, bl, true);
-  DevBuf Yr = neumann(V, mp, br, false);
-  // Mbar += X_l V_k^T + U_k Y_r^T
-  tc_launch(c, Prec::F64, gemm_desc('N', 'T', m, mp, kk, m, mp, m, 1.0, 1.0), Xl.as<double>(), V, tbar, s);
-  tc_launch(c, Prec::F64, gemm_desc('N', 'T', m, mp, kk, m, mp, m, 1.0, 1.0), U, Yr.as<double>(), tbar, s);
+  DevBuf sq2(c, (size_t)K * 8);
+  k_square(tp.sv.as<double>(), sq2.as<double>(), K, s);
+  const double* sig = sq2.as<double>();
+  const double* tz = sq2.as<double>() + kk;
+  DevBuf T1(c, (size_t)std::max(m, mp) * kk * 8);
+  if (m > kk) {
+    // left: b = s dU + M dV ; G = M M^T
+    DevBuf bl(c, (size_t)m * kk * 8), X(c, (size_t)m * kk * 8);
+    k_copy(dU.as<double>(), bl.as<double>(), (size_t)m * kk, s);
+    hipLaunchKernelGGL(scale_cols_by_kernel, dim3(nb((size_t)m * kk)), dim3(NT), 0, s, bl.as<double>(), m, kk,
+                       tp.sv.as<double>());
+    TNAD_HIP_TRY(hipGetLastError());
+    tc_launch(c, Prec::F64, gemm_desc('N', 'N', m, kk, mp, m, mp, m, 1.0, 1.0), M, dV.as<double>(), bl.as<double>(), s);
+    auto applyMMt = [&](const double* Vb, double* out) {
+      tc_launch(c, Prec::F64, gemm_desc('T', 'N', mp, kk, m, m, m, mp), M, Vb, T1.as<double>(), s);
+      tc_launch(c, Prec::F64, gemm_desc('N', 'N', m, kk, mp, m, mp, m), M, T1.as<double>(), out, s);
+    };
+    resolvent_pcg(c, m, kk, K, U, sig, tz, applyMMt, bl.as<double>(), X.as<double>(), 1e-13, 200);
+    // Mbar += X V_kk^T
+    tc_launch(c, Prec::F64, gemm_desc('N', 'T', m, mp, kk, m, mp, m, 1.0, 1.0), X.as<double>(), V, tbar, s);
+  }
+  if (mp > kk) {
+    // right: b' = M^T dU + s dV ; G = M^T M
+    DevBuf br(c, (size_t)mp * kk * 8), Y(c, (size_t)mp * kk * 8);
+    k_copy(dV.as<double>(), br.as<double>(), (size_t)mp * kk, s);
+    hipLaunchKernelGGL(scale_cols_by_kernel, dim3(nb((size_t)mp * kk)), dim3(NT), 0, s, br.as<double>(), mp, kk,
+                       tp.sv.as<double>());
+    TNAD_HIP_TRY(hipGetLastError());
+    tc_launch(c, Prec::F64, gemm_desc('T', 'N', mp, kk, m, m, m, mp, 1.0, 1.0), M, dU.as<double>(), br.as<double>(), s);
+    auto applyMtM = [&](const double* Vb, double* out) {
+      tc_launch(c, Prec::F64, gemm_desc('N', 'N', m, kk, mp, m, mp, m), M, Vb, T1.as<double>(), s);
+      tc_launch(c, Prec::F64, gemm_desc('T', 'N', mp, kk, m, m, m, mp), M, T1.as<double>(), out, s);
+    };
+    resolvent_pcg(c, mp, kk, K, V, sig, tz, applyMtM, br.as<double>(), Y.as<double>(), 1e-13, 200);
+    // Mbar += U_kk Y^T
+    tc_launch(c, Prec::F64, gemm_desc('N', 'T', m, mp, kk, m, mp, m, 1.0, 1.0), U, Y.as<double>(), tbar, s);
+  }
 }
 // ---------------------------------------------------------------------------
 // trg (src/trg.jl:13-30)

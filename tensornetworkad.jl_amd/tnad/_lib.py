@@ -78,6 +78,17 @@ SIGNATURES = {
     "tnad_trg_svd": (I, [P, P, I, I, I, I, I, D, P, P, PI]),
     "tnad_contract": (I, [P, I, C.c_char_p, P, P, C.c_char_p, P, P, C.c_char_p, P, D, D, P]),
     "tnad_time_contract": (I, [P, I, C.c_char_p, P, C.c_char_p, P, C.c_char_p, P, I, PD]),
+    "tnad_mlp_create": (I, [P, I, I, I, I, C.c_ulonglong, C.POINTER(P)]),
+    "tnad_mlp_free": (I, [P]),
+    "tnad_mlp_num_params": (I, [P, C.POINTER(C.c_longlong)]),
+    "tnad_mlp_get_params": (I, [P, P, P]),
+    "tnad_mlp_set_params": (I, [P, P, P]),
+    "tnad_mlp_forward": (I, [P, P, P, I, P]),
+    "tnad_mlp_loss_grad": (I, [P, P, P, P, I, PD, P]),
+    "tnad_mlp_train_step": (I, [P, P, P, P, I, C.c_float, PD]),
+    "tnad_bulk_features": (I, [P, P, I, I, P]),
+    "tnad_env_features": (I, [P, P, P, I, I, P]),
+    "tnad_env_from_features": (I, [P, P, I, I, P, P]),
 }
 
 _lib = None

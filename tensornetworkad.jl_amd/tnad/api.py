@@ -102,8 +102,8 @@ def _out(shape):
     return C.c_void_p(a.ctypes.data), a
 
 
-def make_opts(precision="fp64", switch_tol=0.0, bwd_precision="fp64", eig_block=0, eig_max_iters=60,
-              eig_warm=False, eig_warm_iters=3, eig_tol=1e-13, bwd_last=0) -> Opts:
+def make_opts(precision="fp64", switch_tol=0.0, bwd_precision="fp64", eig_block=0, eig_max_iters=200,
+              eig_warm=False, eig_warm_iters=200, eig_tol=1e-13, bwd_last=0) -> Opts:
     """Run options (tnad_opts).  Defaults give the reference semantics."""
     o = Opts()
     _lib.load().tnad_opts_default(C.byref(o))
@@ -119,8 +119,8 @@ def make_opts(precision="fp64", switch_tol=0.0, bwd_precision="fp64", eig_block=
     return o
 
 
-MIXED_FP32 = dict(precision="fp32", switch_tol=1e-5, eig_warm=True, eig_warm_iters=2)
-MIXED_BF16 = dict(precision="bf16", switch_tol=1e-2, eig_warm=True, eig_warm_iters=2)
+MIXED_FP32 = dict(precision="fp32", switch_tol=1e-5, eig_warm=True)
+MIXED_BF16 = dict(precision="bf16", switch_tol=1e-2, eig_warm=True)
 
 
 # ---------------------------------------------------------------------------

@@ -40,7 +40,7 @@ def test_ctypes_layer_loads_and_reports_version():
     assert lib.tnad_version() == 1
     o = L.Opts()
     lib.tnad_opts_default(ctypes.byref(o))
-    assert o.precision == 0 and o.eig_max_iters == 60 and o.eig_tol == pytest.approx(1e-13)
+    assert o.precision == 0 and o.eig_max_iters == 200 and o.eig_tol == pytest.approx(1e-13)
 
 
 def test_no_cpu_fallback_in_product_path():

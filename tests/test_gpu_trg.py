@@ -39,14 +39,20 @@ def test_trg_readme_config_a_gpu(ctx):
     assert abs(lnz - O.onsager_lnz(0.5)) < 1e-6
 
 
-def test_trg_full_gradient_matches_oracle_direction(ctx):
+@pytest.mark.parametrize("chi,niter", [(6, 3), (8, 4)])
+def test_trg_full_gradient_matches_oracle_direction(ctx, chi, niter):
+    # a generic full-rank tensor: every singular value is resolved, so the reference's
+    # svd_back is well defined along every direction (for the rank-2 Ising tensor the
+    # directional derivative off the physical family depends on LAPACK's arbitrary
+    # null-space vectors through dS / (2 sqrt(s)) at s ~ 1e-17)
     import tnad
-    print("case: trg full gradient random direction chi=8 niter=4", flush=True)
-    a = tnad.model_tensor(tnad.Ising(), 0.45)
-    lnz, abar = tnad.trg_and_grad(a, 8, 4, ctx=ctx)
-    rng = np.random.default_rng(0)
-    direc = rng.standard_normal(a.shape)
-    _, g_o = O.trg(O.model_tensor_ising(0.45), 8, 4, with_grad_seed=direc)
+    print(f"case: trg full gradient random direction generic tensor chi={chi} niter={niter}", flush=True)
+    rng = np.random.default_rng(7)
+    a = rng.uniform(0.5, 1.5, (3, 3, 3, 3))
+    direc = np.random.default_rng(chi).standard_normal(a.shape)
+    lnz, abar = tnad.trg_and_grad(a, chi, niter, ctx=ctx)
+    lnz_o, g_o = O.trg(a, chi, niter, with_grad_seed=direc)
+    assert lnz == pytest.approx(lnz_o, rel=1e-13, abs=0)
     assert float(np.sum(abar * direc)) == pytest.approx(g_o, rel=1e-9)
 
 

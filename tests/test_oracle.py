@@ -36,6 +36,17 @@ def test_trg_gradient_matches_num_grad():
     assert O.num_grad(f, 0.4, 1e-6) == pytest.approx(g, rel=1e-7)
 
 
+@pytest.mark.parametrize("chi,niter", [(6, 3), (8, 4)])
+def test_trg_generic_tensor_gradient_matches_num_grad(chi, niter):
+    # full-rank tensor, truncated at chi: the reverse sweep along a random direction against
+    # central differences of lnZ (the same configuration the GPU parity test uses)
+    a = np.random.default_rng(7).uniform(0.5, 1.5, (3, 3, 3, 3))
+    direc = np.random.default_rng(chi).standard_normal(a.shape)
+    _, g = O.trg(a, chi, niter, with_grad_seed=direc)
+    f = lambda t: O.trg(a + t * direc, chi, niter)
+    assert O.num_grad(f, 0.0, 1e-6) == pytest.approx(g, rel=1e-6)
+
+
 def test_trg_onsager_closed_form():
     # independent pin: Onsager's exact free energy (chi=20 TRG is within 1e-6 of it)
     assert O.onsager_lnz(0.5) == pytest.approx(1.0257928126949176, abs=5e-14)
