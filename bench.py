@@ -37,6 +37,9 @@ METRIC = "CTMRG fwd+adjoint ms/iter at chi=128,D=4: mixed vs FP64 plain, |dE|<1e
 
 # MI355X published peaks
 PEAK = {"fp64": 78.6, "fp32": 157.3, "bf16": 2500.0}   # dense matrix TFLOP/s
+# HBM bytes per launch of the roofline kernel from rocprofv3 FETCH_SIZE + WRITE_SIZE (scaled by a plain copy of
+# known size); None until measured
+TRAFFIC = None
 
 
 def heisenberg_h():
@@ -100,6 +103,25 @@ def blas_threads():
     return os.cpu_count()
 
 
+def roofline_leg(tnad, ctx, chi, Dd):
+    """dominant contraction of the step, timed alone with HIP events on the library stream:
+    the enlarged corner cp = X2 (x) a, ein"ibcl,jkcb->ijlk", 2 chi^2 Dd^4 flops per launch"""
+    spec = "ibcl,jkcb->ijlk"
+    ext = {"i": chi, "l": chi, "b": Dd, "c": Dd, "j": Dd, "k": Dd}
+    kflops = 2.0 * chi * chi * Dd ** 4
+    roof = {}
+    for prec in ("fp64", "fp32"):
+        ms = tnad.time_contract(spec, ext, precision=prec, reps=20, ctx=ctx)
+        roof[prec] = dict(ms=ms, tflops=kflops / (ms * 1e-3) / 1e12)
+    return {"bound": "mfma", "kernel": f"tc_kernel<float> ein\"{spec}\" (fp32, {kflops:.3e} flop/launch)",
+            "achieved": round(roof["fp32"]["tflops"], 3), "peak": PEAK["fp32"], "unit": "TFLOP/s",
+            "frac": round(roof["fp32"]["tflops"] / PEAK["fp32"], 4), "traffic": TRAFFIC,
+            "avg_launch_ms": round(roof["fp32"]["ms"], 5),
+            "fp64_variant": {"achieved": round(roof["fp64"]["tflops"], 3), "peak": PEAK["fp64"],
+                             "frac": round(roof["fp64"]["tflops"] / PEAK["fp64"], 4),
+                             "avg_launch_ms": round(roof["fp64"]["ms"], 5)}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -112,6 +134,9 @@ def main():
     ap.add_argument("--maxit", type=int, default=100)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
+    ap.add_argument("--kernel-only", action="store_true",
+                    help="only the roofline leg (dominant contraction, 20 launches) and a 256 MiB device copy "
+                         "of known size, for hardware-counter runs")
     a = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if a.gpus != 1 or world != 1:
@@ -127,6 +152,15 @@ def main():
     Dd = D * D
     ctx = tnad.Context(0)
     dev = torch.device("cuda:0")
+    if a.kernel_only:
+        roofline = roofline_leg(tnad, ctx, chi, Dd)
+        # a plain copy of known size (256 MiB read + 256 MiB written) to scale FETCH_SIZE / WRITE_SIZE
+        x = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device=dev)
+        y = torch.empty_like(x)
+        y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"roofline": roofline, "copy_bytes_each_way": x.numel() * 4}))
+        return
     h = heisenberg_h()
     bulk = synthetic_bulk(D, s, a.seed)
     # inputs resident in HBM: column-major buffers behind torch-owned device memory
@@ -154,7 +188,7 @@ def main():
         ms_step = dt * 1e3 / a.steps
         results[name] = dict(ms_step=ms_step, ms_iter=ms_step / (sum(its) / len(its)), iters=its[-1], e=e,
                              stats=ctx.stats())
-        grads[name] = g_t.cpu().numpy()
+        grads[name] = np.reshape(g_t.cpu().numpy(), bulk.shape, order="F")
         print(f"[bench] {name}: {ms_step:.2f} ms per energy+gradient, {its[-1]} iterations", file=sys.stderr,
               flush=True)
 
@@ -166,20 +200,7 @@ def main():
                    grad_rel_err=float(np.linalg.norm(gmx - g64) / np.linalg.norm(g64)),
                    iters_fp64=results["plain_fp64"]["iters"], iters_mixed=results["mixed"]["iters"])
 
-    # roofline: dominant contraction of the step, timed alone with HIP events on the library stream
-    spec = "ibcl,jkcb->ijlk"          # enlarged corner, cp = X2 (x) a   (2 chi^2 Dd^4 flops)
-    ext = {"i": chi, "l": chi, "b": Dd, "c": Dd, "j": Dd, "k": Dd}
-    kflops = 2.0 * chi * chi * Dd ** 4
-    roof = {}
-    for prec in ("fp64", "fp32"):
-        ms = tnad.time_contract(spec, ext, precision=prec, reps=20, ctx=ctx)
-        tf = kflops / (ms * 1e-3) / 1e12
-        roof[prec] = dict(ms=ms, tflops=tf)
-    roofline = {"bound": "mfma", "kernel": f"tc_kernel<float> ein\"{spec}\" (fp32, {kflops:.3e} flop/launch)",
-                "achieved": round(roof["fp32"]["tflops"], 3), "peak": PEAK["fp32"], "unit": "TFLOP/s",
-                "frac": round(roof["fp32"]["tflops"] / PEAK["fp32"], 4), "traffic": None,
-                "fp64_variant": {"achieved": round(roof["fp64"]["tflops"], 3), "peak": PEAK["fp64"],
-                                 "frac": round(roof["fp64"]["tflops"] / PEAK["fp64"], 4)}}
+    roofline = roofline_leg(tnad, ctx, chi, Dd)
 
     cpu = None
     if not a.no_cpu_baseline:

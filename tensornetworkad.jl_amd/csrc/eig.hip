@@ -83,7 +83,11 @@ __global__ __launch_bounds__(EIG_NT) void jacobi_rot_kernel(const double* __rest
     fro_s = sqrt(sacc);
   }
   __syncthreads();
-  const double abs_floor = 1e-18 * fro_s + 1e-300;
+  // An off-diagonal entry at or below the rounding level of the matrix (2 eps |H|_F) carries no
+  // information: rotating it only reshuffles rounding noise, so sweeps would never end on matrices
+  // whose spectrum spans many decades.  Above that, the relative criterion resolves small
+  // eigenvalues to their own scale.
+  const double abs_floor = 4.5e-16 * fro_s + 1e-300;
   const double rel_tol = 1e-15;
   // unordered slot pairs (i <= j) handled by this thread, packed i | j << 16 (-1: none)
   int bij[JB_MAX];
@@ -303,17 +307,24 @@ __global__ __launch_bounds__(CH_G * CH_G) void chol_inv_kernel(const double* __r
 #pragma unroll
     for (int a = 0; a < CH_R; ++a) {
       const int i = ty + CH_G * a;
-      li[a] = (i > jj && i < m) ? col[i] * isd : 0.0;
+      li[a] = (a >= bj && i > jj && i < m) ? col[i] * isd : 0.0;
     }
 #pragma unroll
     for (int b = 0; b < CH_R; ++b) {
       const int j = tx + CH_G * b;
-      lj[b] = (j > jj && j < m) ? col[j] * isd : 0.0;
+      lj[b] = (b >= bj && j > jj && j < m) ? col[j] * isd : 0.0;
     }
+    // only the trailing lower blocks (bj <= b <= a) change; the branches are uniform (bj is
+    // the same for every thread), so finished and upper blocks cost nothing
 #pragma unroll
-    for (int a = 0; a < CH_R; ++a)
+    for (int a = 0; a < CH_R; ++a) {
+      if (a < bj) continue;
 #pragma unroll
-      for (int b = 0; b < CH_R; ++b) A[a][b] -= li[a] * lj[b];
+      for (int b = 0; b < CH_R; ++b) {
+        if (b < bj || b > a) continue;
+        A[a][b] -= li[a] * lj[b];
+      }
+    }
     if (tx == tj) {
 #pragma unroll
       for (int a = 0; a < CH_R; ++a) {
@@ -357,18 +368,24 @@ __global__ __launch_bounds__(CH_G * CH_G) void chol_inv_kernel(const double* __r
       }
     }
     __syncthreads();
+    // rows above jj are final and row jj of B is zero right of the diagonal: only blocks with
+    // a >= aj and b <= aj change (uniform branches)
     double xr[CH_R];
 #pragma unroll
     for (int b = 0; b < CH_R; ++b) {
       const int j = tx + CH_G * b;
-      xr[b] = (j <= jj) ? row[j] : 0.0;   // row jj of B is zero right of the diagonal
+      xr[b] = (b <= aj && j <= jj) ? row[j] : 0.0;
     }
 #pragma unroll
     for (int a = 0; a < CH_R; ++a) {
+      if (a < aj) continue;
       const int i = ty + CH_G * a;
       const double lij = (i > jj && i < m) ? Lp[pl(i, jj)] : 0.0;
 #pragma unroll
-      for (int b = 0; b < CH_R; ++b) A[a][b] -= lij * xr[b];
+      for (int b = 0; b < CH_R; ++b) {
+        if (b > aj) continue;
+        A[a][b] -= lij * xr[b];
+      }
     }
   }
   // Rinv(r, c) = B(c, r)

@@ -11,10 +11,18 @@
 // Tiling: 64x64 output tile per 256-thread workgroup (4 waves in 2x2), each
 // wave owns a 32x32 sub-tile made of 2x2 MFMA 16x16 blocks.  Operands are
 // staged through LDS with a register prefetch of the next K-tile.
+// Split-K: a contraction with few output tiles and a long K (the Gram and
+// projection products of the eigensolver, the bulk cotangent of the adjoint)
+// would leave most of the 256 CUs idle, so tc_launch cuts K into slices along
+// grid z; every slice writes its partial tile to a workspace and a second
+// kernel sums the slices in a fixed order (deterministic) and applies
+// alpha / beta to the strided output.
 //   F64 : v_mfma_f64_16x16x4f64       (BK = 16)
 //   F32 : v_mfma_f32_16x16x4f32       (BK = 16)
 //   BF16: v_mfma_f32_16x16x16bf16_1k  (BK = 32, fp32 in HBM, bf16 in LDS,
 //                                      fp32 accumulation)
+#include <algorithm>
+
 #include "tnad_internal.h"
 
 namespace tnad {
@@ -69,15 +77,19 @@ template <> struct MfmaOp<float> {
 // ---------------------------------------------------------------------------
 template <typename T>
 __global__ __launch_bounds__(NT) void tc_kernel(TcDesc d, const T* __restrict__ A,
-                                                const T* __restrict__ B, T* __restrict__ C) {
+                                                const T* __restrict__ B, T* __restrict__ C,
+                                                T* __restrict__ Wsp) {
   constexpr int BK = 16;
   constexpr int PAD = 16;
   __shared__ T As[BK][BM + PAD];
   __shared__ T Bs[BK][BN + PAD];
 
-  const int M = d.M0 * d.M1, N = d.N0 * d.N1, K = d.K0 * d.K1;
+  const int M = d.M0 * d.M1, N = d.N0 * d.N1;
   const int tile_m = blockIdx.x * BM, tile_n = blockIdx.y * BN;
-  const int bz = blockIdx.z;
+  // grid z: batch index, or split-K slice (batch 1) when a workspace is given
+  const int bz = Wsp ? 0 : blockIdx.z;
+  const int kbeg = Wsp ? blockIdx.z * d.kchunk : 0;
+  const int K = Wsp ? min(d.K0 * d.K1, kbeg + d.kchunk) : d.K0 * d.K1;
   A += (long long)bz * d.abat;
   B += (long long)bz * d.bbat;
   C += (long long)bz * d.cbat;
@@ -135,8 +147,8 @@ __global__ __launch_bounds__(NT) void tc_kernel(TcDesc d, const T* __restrict__ 
 #pragma unroll
     for (int j = 0; j < 2; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
 
-  gload(0);
-  for (int k0 = 0; k0 < K; k0 += BK) {
+  gload(kbeg);
+  for (int k0 = kbeg; k0 < K; k0 += BK) {
     __syncthreads();
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
@@ -161,6 +173,7 @@ __global__ __launch_bounds__(NT) void tc_kernel(TcDesc d, const T* __restrict__ 
 
   // epilogue: column lane%16 of each 16x16 block, rows per MfmaOp<T>::row
   const T alpha = (T)d.alpha, beta = (T)d.beta;
+  T* Wz = Wsp ? Wsp + (size_t)blockIdx.z * M * N : nullptr;   // this slice's M x N partial
 #pragma unroll
   for (int j = 0; j < 2; ++j) {
     int n = tile_n + wn * 32 + j * 16 + (lane & 15);
@@ -172,12 +185,34 @@ __global__ __launch_bounds__(NT) void tc_kernel(TcDesc d, const T* __restrict__ 
       for (int r = 0; r < 4; ++r) {
         int m = tile_m + wm * 32 + i * 16 + MfmaOp<T>::row(lane, r);
         if (m >= M) continue;
+        if (Wz) {
+          Wz[m + (size_t)M * n] = (T)acc[i][j][r];
+          continue;
+        }
         long long o = noff + off2(m, d.M0, d.cm0, d.cm1);
         T v = alpha * (T)acc[i][j][r];
         if (d.beta != 0.0) v += beta * C[o];
         C[o] = v;
       }
     }
+  }
+}
+
+// split-K reduction: C = alpha * sum_s W[s] + beta * C over the strided output (slices summed in
+// order, so the result does not depend on scheduling)
+template <typename T>
+__global__ __launch_bounds__(NT) void tc_splitk_reduce(TcDesc d, const T* __restrict__ Wsp, int nsplit,
+                                                       T* __restrict__ C) {
+  const long long M = (long long)d.M0 * d.M1, N = (long long)d.N0 * d.N1;
+  const long long MN = M * N;
+  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < MN; e += (long long)gridDim.x * blockDim.x) {
+    const int m = (int)(e % M), n = (int)(e / M);
+    double acc = 0.0;
+    for (int s = 0; s < nsplit; ++s) acc += (double)Wsp[e + (size_t)s * MN];
+    const long long o = off2(m, d.M0, d.cm0, d.cm1) + off2(n, d.N0, d.cn0, d.cn1);
+    double v = d.alpha * acc;
+    if (d.beta != 0.0) v += d.beta * (double)C[o];
+    C[o] = (T)v;
   }
 }
 
@@ -195,15 +230,17 @@ __device__ __forceinline__ unsigned short f2bf(float x) {
 
 __global__ __launch_bounds__(NT) void tc_kernel_bf16(TcDesc d, const float* __restrict__ A,
                                                      const float* __restrict__ B,
-                                                     float* __restrict__ C) {
+                                                     float* __restrict__ C, float* __restrict__ Wsp) {
   constexpr int BK = 32;
   constexpr int KP = BK + 4;   // row pitch in bf16 elements (8-byte aligned rows)
   __shared__ unsigned short As[BM][KP];
   __shared__ unsigned short Bs[BN][KP];
 
-  const int M = d.M0 * d.M1, N = d.N0 * d.N1, K = d.K0 * d.K1;
+  const int M = d.M0 * d.M1, N = d.N0 * d.N1;
   const int tile_m = blockIdx.x * BM, tile_n = blockIdx.y * BN;
-  const int bz = blockIdx.z;
+  const int bz = Wsp ? 0 : blockIdx.z;
+  const int kbeg = Wsp ? blockIdx.z * d.kchunk : 0;
+  const int K = Wsp ? min(d.K0 * d.K1, kbeg + d.kchunk) : d.K0 * d.K1;
   A += (long long)bz * d.abat;
   B += (long long)bz * d.bbat;
   C += (long long)bz * d.cbat;
@@ -258,8 +295,8 @@ __global__ __launch_bounds__(NT) void tc_kernel_bf16(TcDesc d, const float* __re
 #pragma unroll
     for (int j = 0; j < 2; ++j) acc[i][j] = f4{0, 0, 0, 0};
 
-  gload(0);
-  for (int k0 = 0; k0 < K; k0 += BK) {
+  gload(kbeg);
+  for (int k0 = kbeg; k0 < K; k0 += BK) {
     __syncthreads();
 #pragma unroll
     for (int r = 0; r < 8; ++r) {
@@ -285,6 +322,7 @@ __global__ __launch_bounds__(NT) void tc_kernel_bf16(TcDesc d, const float* __re
     }
   }
   const float alpha = (float)d.alpha, beta = (float)d.beta;
+  float* Wz = Wsp ? Wsp + (size_t)blockIdx.z * M * N : nullptr;
 #pragma unroll
   for (int j = 0; j < 2; ++j) {
     int n = tile_n + wn * 32 + j * 16 + (lane & 15);
@@ -296,6 +334,10 @@ __global__ __launch_bounds__(NT) void tc_kernel_bf16(TcDesc d, const float* __re
       for (int r = 0; r < 4; ++r) {
         int m = tile_m + wm * 32 + i * 16 + 4 * (lane >> 4) + r;
         if (m >= M) continue;
+        if (Wz) {
+          Wz[m + (size_t)M * n] = acc[i][j][r];
+          continue;
+        }
         long long o = noff + off2(m, d.M0, d.cm0, d.cm1);
         float v = alpha * acc[i][j][r];
         if (d.beta != 0.0) v += beta * C[o];
@@ -331,21 +373,52 @@ void tc_launch(Ctx* c, Prec p, const TcDesc& d, const void* A, const void* B, vo
     throw Error(TNAD_ERR_ARG, "tc_launch: extents out of range");
   dim3 grid((unsigned)((M + BM - 1) / BM), (unsigned)((N + BN - 1) / BN), (unsigned)d.batch);
   if (grid.y > 65535) throw Error(TNAD_ERR_ARG, "tc_launch: N too large");
+  // split-K when the output tiles cannot fill the chip and K is long: about 512 workgroups,
+  // slices of at least 128 K elements (a multiple of both kernels' K-tile)
+  const long long K = (long long)d.K0 * d.K1;
+  const long long tiles = (long long)grid.x * grid.y;
+  TcDesc dd = d;
+  int nsplit = 1;
+  if (d.batch == 1 && tiles < 128 && K >= 256) {
+    long long want = std::min<long long>((512 + tiles - 1) / tiles, (K + 127) / 128);
+    long long chunk = (K + want - 1) / want;
+    chunk = (chunk + 31) / 32 * 32;
+    nsplit = (int)((K + chunk - 1) / chunk);
+    dd.kchunk = (int)chunk;
+  }
+  DevBuf wsp;
+  if (nsplit > 1) {
+    const size_t es = p == Prec::F64 ? 8 : 4;
+    wsp = DevBuf(c, (size_t)nsplit * M * N * es);
+    grid.z = (unsigned)nsplit;
+  }
+  void* W = nsplit > 1 ? wsp.ptr : nullptr;
   switch (p) {
     case Prec::F64:
-      hipLaunchKernelGGL(tc_kernel<double>, grid, dim3(NT), 0, s, d, (const double*)A,
-                         (const double*)B, (double*)C);
+      hipLaunchKernelGGL(tc_kernel<double>, grid, dim3(NT), 0, s, dd, (const double*)A,
+                         (const double*)B, (double*)C, (double*)W);
       break;
     case Prec::F32:
-      hipLaunchKernelGGL(tc_kernel<float>, grid, dim3(NT), 0, s, d, (const float*)A,
-                         (const float*)B, (float*)C);
+      hipLaunchKernelGGL(tc_kernel<float>, grid, dim3(NT), 0, s, dd, (const float*)A,
+                         (const float*)B, (float*)C, (float*)W);
       break;
     case Prec::BF16:
-      hipLaunchKernelGGL(tc_kernel_bf16, grid, dim3(NT), 0, s, d, (const float*)A,
-                         (const float*)B, (float*)C);
+      hipLaunchKernelGGL(tc_kernel_bf16, grid, dim3(NT), 0, s, dd, (const float*)A,
+                         (const float*)B, (float*)C, (float*)W);
       break;
   }
   TNAD_HIP_TRY(hipGetLastError());
+  if (nsplit > 1) {
+    const long long MN = M * N;
+    unsigned nb = (unsigned)std::min<long long>((MN + NT - 1) / NT, 65535LL * 8);
+    if (p == Prec::F64)
+      hipLaunchKernelGGL(tc_splitk_reduce<double>, dim3(nb), dim3(NT), 0, s, dd, (const double*)W, nsplit,
+                         (double*)C);
+    else
+      hipLaunchKernelGGL(tc_splitk_reduce<float>, dim3(nb), dim3(NT), 0, s, dd, (const float*)W, nsplit,
+                         (float*)C);
+    TNAD_HIP_TRY(hipGetLastError());
+  }
 }
 
 }  // namespace tnad

@@ -59,6 +59,7 @@ struct TcDesc {
   int batch = 1;
   long long abat = 0, bbat = 0, cbat = 0;
   double alpha = 1.0, beta = 0.0;
+  int kchunk = 0;   // set by tc_launch: K elements per split-K slice (0: no split)
 };
 
 // Plain column-major GEMM helper: C(MxN) = alpha op(A) op(B) + beta C

@@ -32,6 +32,7 @@ from .api import (  # noqa: F401
     hamiltonian,
     indexperm_symmetrize,
     mag_tensor,
+    magnetisation,
     magofbeta,
     make_opts,
     model_tensor,

@@ -418,6 +418,27 @@ def energy_and_grad(h, ipeps: SquareIPEPS, chi: int, tol: float, maxit: int, ctx
     return e.value, g
 
 
+def magnetisation(model: HamiltonianModel, beta: float, chi: int, rng=None, tol: float = 1e-6, maxit: int = 100,
+                  ctx: Context | None = None) -> float:
+    """src/exampletensors.jl:57-69: |<m>| of the model from CTMRG (:random start, tol 1e-6,
+    maxit 100) and the four-site environment, every contraction on the GPU."""
+    ctx = ctx or default_context()
+    a = model_tensor(model, beta)
+    m = mag_tensor(model, beta)
+    rt = ctmrg(SquareCTMRGRuntime.new(a, "random", chi, rng=rng), tol, maxit, ctx=ctx)
+    Cm, Tm = rt.corner, rt.edge
+    # ctc = ein"ia,ajb,bk -> ijk"(corner, edge, corner)
+    ct = contract("ia,ajb->ijb", Cm, Tm, ctx=ctx)
+    ctc = contract("ijb,bk->ijk", ct, Cm, ctx=ctx)
+    # env = ein"alc,ckd,bjd,bia -> ijkl"(ctc, edge, ctc, edge), pairwise
+    x1 = contract("alc,ckd->alkd", ctc, Tm, ctx=ctx)
+    x2 = contract("bjd,bia->jdia", ctc, Tm, ctx=ctx)
+    env = contract("alkd,jdia->ijkl", x1, x2, ctx=ctx)
+    mag = contract("ijkl,ijkl->", env, m, ctx=ctx)
+    nrm = contract("ijkl,ijkl->", env, a, ctx=ctx)
+    return abs(float(np.ravel(mag)[0]) / float(np.ravel(nrm)[0]))
+
+
 def energy(h, ipeps: SquareIPEPS, chi: int, tol: float, maxit: int, ctx: Context | None = None, **opts) -> float:
     """src/variationalipeps.jl:28-40."""
     return energy_and_grad(h, ipeps, chi, tol, maxit, ctx=ctx, want_grad=False, **opts)[0]

@@ -82,3 +82,33 @@ def test_gemm_alpha_beta_accumulate(ctx, prec, tr):
     got = tnad.contract(spec, A, B, precision=prec, ctx=ctx, alpha=-0.5, beta=1.0, Cin=C0)
     ref = -0.5 * np.einsum(spec, A, B) + C0
     assert _rel(got, ref) < 10 * TOL[prec]
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32", "bf16"])
+@pytest.mark.parametrize("m,n,k", [(70, 50, 2000), (192, 192, 2048), (300, 20, 777)])
+def test_split_k_gemm_alpha_beta(ctx, prec, m, n, k):
+    # few output tiles and a long K: the launcher cuts K into slices and reduces them
+    import tnad
+    print(f"case: split-K gemm {prec} {m}x{n}x{k}", flush=True)
+    rng = np.random.default_rng(m + n + k)
+    A = rng.standard_normal((k, m))
+    B = rng.standard_normal((k, n))
+    C0 = rng.standard_normal((m, n))
+    got = tnad.contract("ki,kj->ij", A, B, precision=prec, ctx=ctx, alpha=0.75, beta=-1.25, Cin=C0)
+    ref = 0.75 * (A.T @ B) - 1.25 * C0
+    tol = {"fp64": 1e-13, "fp32": 2e-6, "bf16": 2e-2}[prec]
+    assert _rel(got, ref) < 10 * tol
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_split_k_strided_bulk_cotangent(ctx, prec):
+    # abar[j,k,c,b] = sum_{i,l} cpbar[i,j,l,k] X2[i,b,c,l]: a 256 x 256 output over K = chi^2
+    import tnad
+    print(f"case: split-K strided ijlk,ibcl->jkcb {prec}", flush=True)
+    rng = np.random.default_rng(3)
+    chi, D = 40, 4
+    A = rng.standard_normal((chi, D, chi, D))
+    B = rng.standard_normal((chi, D, D, chi))
+    got = tnad.contract("ijlk,ibcl->jkcb", A, B, precision=prec, ctx=ctx)
+    ref = np.einsum("ijlk,ibcl->jkcb", A, B)
+    assert _rel(got, ref) < 10 * TOL[prec]
