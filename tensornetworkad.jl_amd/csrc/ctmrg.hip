@@ -63,6 +63,31 @@ __global__ void ritz_scale_kernel(const double* __restrict__ Xr, const double* _
   }
 }
 
+// *out = sum_ij (G_ij - delta_ij)^2 (one workgroup; G is k x k)
+__global__ void gram_dev_kernel(const double* __restrict__ G, int k, double* __restrict__ out) {
+  __shared__ double red[NT / 64];
+  double acc = 0.0;
+  for (int e = threadIdx.x; e < k * k; e += NT) {
+    const double v = G[e] - ((e % k) == (e / k) ? 1.0 : 0.0);
+    acc += v * v;
+  }
+  for (int o = 32; o > 0; o >>= 1) acc += __shfl_down(acc, o);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int w = 0; w < NT / 64; ++w) t += red[w];
+    *out = t;
+  }
+}
+
+// T = (3 I - G) / 2  (one Newton-Schulz step towards G^-1/2)
+__global__ void ns_step_kernel(const double* __restrict__ G, int k, double* __restrict__ T) {
+  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= k * k) return;
+  T[e] = 0.5 * (((e % k) == (e / k) ? 3.0 : 0.0) - G[e]);
+}
+
 // make the largest-|.| entry of every column positive (first occurrence wins);
 // the same sign is applied to the companion block Y (A*X) when given.
 __global__ void canon_sign_kernel(double* __restrict__ X, double* __restrict__ Y, int n) {
@@ -233,6 +258,25 @@ static void debug_dump(Ctx* c, const char* name, const double* dev, size_t n) {
   }
 }
 
+// Opt-in diagnostics (TNAD_DEBUG_EIG=1): report whether a device array holds a non-finite value.
+static void dbg_finite(Ctx* c, const char* what, const double* dev, size_t n) {
+  static const bool on = [] {
+    const char* e = std::getenv("TNAD_DEBUG_EIG");
+    return e && *e && *e != '0';
+  }();
+  if (!on) return;
+  std::vector<double> h(n);
+  TNAD_HIP_TRY(hipMemcpyAsync(h.data(), dev, n * 8, hipMemcpyDeviceToHost, c->stream));
+  c->sync();
+  size_t bad = 0;
+  double mx = 0.0;
+  for (double v : h) {
+    if (!std::isfinite(v)) ++bad;
+    else mx = std::max(mx, std::fabs(v));
+  }
+  std::fprintf(stderr, "[tnad eig]   %s: %zu non-finite of %zu, max |finite| %.3e\n", what, bad, n, mx);
+}
+
 // ==========================================================================
 // Eigensolver: top-k eigenpairs of a symmetric n x n FP64 matrix by |lambda|
 // ==========================================================================
@@ -248,54 +292,78 @@ static void orthonormalize(Ctx* c, double* Y, int n, int k, double* X, EigWork& 
   // block has lower rank than k, e.g. a rank-deficient corner matrix) are replaced by random
   // directions first, so the block always spans k dimensions.
   std::vector<double> piv(k);
-  // fast path: a block whose unit columns are already nearly orthogonal (every Cholesky pivot
-  // >= 1/2, so its condition number is small) is orthonormal to rounding after one plain
-  // CholeskyQR pass -- the usual case once the Ritz-scaled blocks of a converging iteration
-  // come in
+  // fast path: a block whose unit columns are already nearly orthonormal, |Y^T Y - I|_F <= 1/2 (so
+  // every singular value lies in [0.7, 1.3]), is brought to orthonormality by Newton-Schulz steps
+  // Y <- Y (3 I - Y^T Y) / 2 (symmetric orthogonalisation; the error e falls as 3 e^2 / 4), which
+  // are matrix-core products only -- the usual case once the Ritz-scaled blocks of a converging
+  // iteration come in.  Everything else goes through shifted CholeskyQR3 below.
   {
     tc_launch(c, Prec::F64, gemm_desc('T', 'N', k, k, n, n, n, k), Y, Y, w.G.as<double>(), s);
-    chol_inv(w.G.as<double>(), k, 0.0, w.Rinv.as<double>(), w.info.as<int>(), w.res.as<double>(), s);
-    int info = 0;
-    TNAD_HIP_TRY(hipMemcpyAsync(piv.data(), w.res.ptr, (size_t)k * 8, hipMemcpyDeviceToHost, s));
-    TNAD_HIP_TRY(hipMemcpyAsync(&info, w.info.ptr, sizeof(int), hipMemcpyDeviceToHost, s));
+    hipLaunchKernelGGL(gram_dev_kernel, dim3(1), dim3(NT), 0, s, w.G.as<double>(), k, w.res.as<double>());
+    TNAD_HIP_TRY(hipGetLastError());
+    double dev2 = 0.0;
+    TNAD_HIP_TRY(hipMemcpyAsync(&dev2, w.res.ptr, 8, hipMemcpyDeviceToHost, s));
     c->sync();
-    // pmin stays NaN if any pivot is not finite (NaN compares false both ways)
-    double pmin = HUGE_VAL;
-    for (int j = 0; j < k; ++j) pmin = (piv[j] >= pmin) ? pmin : piv[j];
+    const double e0 = std::sqrt(dev2);
     if (std::getenv("TNAD_DEBUG_EIG"))
-      std::fprintf(stderr, "[tnad eig] orthonormalize n=%d k=%d info=%d min_pivot=%.3e\n", n, k, info, pmin);
-    if (info == 0 && pmin >= 0.5 && pmin <= 2.0) {
-      tc_launch(c, Prec::F64, gemm_desc('N', 'N', n, k, k, n, k, n), Y, w.Rinv.as<double>(), X, s);
+      std::fprintf(stderr, "[tnad eig] orthonormalize n=%d k=%d |Y^T Y - I|_F=%.3e\n", n, k, e0);
+    if (e0 <= 0.5) {
+      int passes = 0;
+      for (double e = e0; e > 1e-17 && passes < 8; e = 0.75 * e * e + 1e-18) ++passes;
+      double* src = Y;
+      double* dst = X;
+      for (int p = 0; p < passes; ++p) {
+        if (p > 0) tc_launch(c, Prec::F64, gemm_desc('T', 'N', k, k, n, n, n, k), src, src, w.G.as<double>(), s);
+        hipLaunchKernelGGL(ns_step_kernel, dim3((k * k + NT - 1) / NT), dim3(NT), 0, s, w.G.as<double>(), k,
+                           w.Rinv.as<double>());
+        TNAD_HIP_TRY(hipGetLastError());
+        tc_launch(c, Prec::F64, gemm_desc('N', 'N', n, k, k, n, k, n), src, w.Rinv.as<double>(), dst, s);
+        std::swap(src, dst);
+      }
+      if (src != X) k_copy(src, X, (size_t)n * k, s);
       return;
     }
   }
   const double u = 1.1102230246251565e-16;
   const double shift = 11.0 * ((double)n * k + (double)k * (k + 1)) * u * (double)k;
   std::vector<int> dead(k);
-  for (int attempt = 0; attempt < 3; ++attempt) {
+  // the Cholesky factor used below must belong to the block as it finally is: the last attempt
+  // factors without refilling afterwards
+  for (int attempt = 0;; ++attempt) {
     tc_launch(c, Prec::F64, gemm_desc('T', 'N', k, k, n, n, n, k), Y, Y, w.G.as<double>(), s);
     chol_inv(w.G.as<double>(), k, shift, w.Rinv.as<double>(), w.info.as<int>(), w.res.as<double>(), s);
     TNAD_HIP_TRY(hipMemcpyAsync(piv.data(), w.res.ptr, (size_t)k * 8, hipMemcpyDeviceToHost, s));
     c->sync();
-    // a column with no component outside the span of the earlier ones keeps only the shift
-    const double thresh = 2.0 * std::sqrt(shift);
+    // a column with less than 1% of its (unit) norm outside the span of the earlier ones is
+    // replaced: it adds next to no new direction, and keeping it would leave the block so
+    // ill-conditioned that the plain passes below could break down
+    const double thresh = std::max(2.0 * std::sqrt(shift), 1e-2);
     int ndead = 0;
     for (int j = 0; j < k; ++j) {
       dead[j] = !(piv[j] > thresh);
       ndead += dead[j];
     }
-    if (ndead == 0) break;
+    if (std::getenv("TNAD_DEBUG_EIG"))
+      std::fprintf(stderr, "[tnad eig] shifted CholeskyQR attempt %d: %d dependent columns\n", attempt, ndead);
+    if (ndead == 0 || attempt == 3) break;
     TNAD_HIP_TRY(hipMemcpyAsync(w.info.ptr, dead.data(), (size_t)k * sizeof(int), hipMemcpyHostToDevice, s));
     k_random_fill_flagged(Y, n, k, w.info.as<int>(), seed + 7919u * (attempt + 1), s);
     k_col_norms(Y, n, k, w.nrm.as<double>(), s);
     k_scale_cols(Y, n, k, w.nrm.as<double>(), 1, s);
   }
+  dbg_finite(c, "shifted Rinv", w.Rinv.as<double>(), (size_t)k * k);
   tc_launch(c, Prec::F64, gemm_desc('N', 'N', n, k, k, n, k, n), Y, w.Rinv.as<double>(), X, s);
   double* src = X;
   double* dst = Y;
   for (int pass = 1; pass < 3; ++pass) {
     tc_launch(c, Prec::F64, gemm_desc('T', 'N', k, k, n, n, n, k), src, src, w.G.as<double>(), s);
     chol_inv(w.G.as<double>(), k, 0.0, w.Rinv.as<double>(), w.info.as<int>(), nullptr, s);
+    int info = 0;
+    TNAD_HIP_TRY(hipMemcpyAsync(&info, w.info.ptr, sizeof(int), hipMemcpyDeviceToHost, s));
+    c->sync();
+    if (info != 0)   // breakdown (pivot <= 0): factor the shifted Gram instead
+      chol_inv(w.G.as<double>(), k, shift, w.Rinv.as<double>(), w.info.as<int>(), nullptr, s);
+    dbg_finite(c, "plain pass Rinv", w.Rinv.as<double>(), (size_t)k * k);
     tc_launch(c, Prec::F64, gemm_desc('N', 'N', n, k, k, n, k, n), src, w.Rinv.as<double>(), dst, s);
     std::swap(src, dst);
   }
@@ -341,6 +409,8 @@ int topk_eig(Ctx* c, const double* A, int n, int want, int k, const double* X0, 
     // Y = A X ; H = X^T Y ; H = W diag(theta) W^T ; Xr = X W ; Yr = Y W
     tc_launch(c, Prec::F64, gemm_desc('N', 'N', n, k, n, n, n, n), A, X, w.Y.as<double>(), s);
     tc_launch(c, Prec::F64, gemm_desc('T', 'N', k, k, n, n, n, k), X, w.Y.as<double>(), w.H.as<double>(), s);
+    dbg_finite(c, "X", X, (size_t)n * k);
+    dbg_finite(c, "H", w.H.as<double>(), (size_t)k * k);
     // Rayleigh-Ritz: once the block is nearly an invariant subspace (H nearly diagonal in the
     // previous Ritz basis) second-order perturbation theory replaces the Jacobi sweeps; Jacobi on
     // the first cold iteration, whenever H is not close enough to diagonal, and whenever the
@@ -360,6 +430,8 @@ int topk_eig(Ctx* c, const double* A, int n, int want, int k, const double* X0, 
     if (jac) jacobi_eig_ws(w.H.as<double>(), k, k, theta, w.W.as<double>(), w.Vw.as<double>(), rough ? 2 : 60,
                            nullptr, s);
     n_jacobi += jac;
+    dbg_finite(c, "W", w.W.as<double>(), (size_t)k * k);
+    dbg_finite(c, "theta", theta, (size_t)k);
     tc_launch(c, Prec::F64, gemm_desc('N', 'N', n, k, k, n, k, n), X, w.W.as<double>(), w.Xr.as<double>(), s);
     tc_launch(c, Prec::F64, gemm_desc('N', 'N', n, k, k, n, k, n), w.Y.as<double>(), w.W.as<double>(),
               w.Yr.as<double>(), s);

@@ -251,20 +251,111 @@ __global__ __launch_bounds__(EIG_NT) void jacobi_sort_kernel(const double* __res
 // threads busy); each elimination step broadcasts one column through LDS and costs
 // one barrier (double-buffered).  L is then parked in LDS (packed lower) and
 // B = L^-1 is formed by right-looking forward substitution with the same register
-// layout.  Output Rinv = L^-T (upper triangular, column-major, zeros below the
-// diagonal) so that X = Y Rinv; piv (nullable) receives diag(L).
+// layout.  The steps run in six 32-column blocks whose block index is a template
+// parameter, so every register access is static and blocks that a step cannot
+// touch (finished columns, the upper triangle) are not even emitted.
+// Output Rinv = L^-T (upper triangular, column-major, zeros below the diagonal) so
+// that X = Y Rinv; piv (nullable) receives diag(L).
 constexpr int CH_G = 32;                  // thread grid edge
 constexpr int CH_R = JACOBI_MAX / CH_G;   // elements per thread per dimension
 static_assert(CH_R * CH_G == JACOBI_MAX, "Cholesky tiling");
+
+struct CholShared {
+  double* Lp;   // packed lower L, column-major
+  double* cb;   // 2 x JACOBI_MAX broadcast buffers
+  int* bad;
+};
+
+__device__ __forceinline__ int chol_pl(int m, int i, int j) { return j * m - j * (j - 1) / 2 + (i - j); }
+
+// elimination steps jj = 32 BJ .. min(32 BJ + 31, m - 1)
+template <int BJ>
+__device__ __forceinline__ void chol_block(double (&A)[CH_R][CH_R], int m, const CholShared& sh, int tx, int ty,
+                                           int t, double* __restrict__ piv) {
+  const int j0 = BJ * CH_G, j1 = min(m, j0 + CH_G);
+  for (int jj = j0; jj < j1; ++jj) {
+    double* col = sh.cb + (jj & 1) * JACOBI_MAX;
+    const int tj = jj - j0;
+    if (tx == tj) {
+#pragma unroll
+      for (int a = BJ; a < CH_R; ++a) {
+        const int i = ty + CH_G * a;
+        if (i >= jj && i < m) col[i] = A[a][BJ];
+      }
+    }
+    __syncthreads();
+    double d = col[jj];
+    if (!(d > 0.0)) {
+      if (t == 0 && *sh.bad == 0) *sh.bad = jj + 1;
+      d = 1e-300;
+    }
+    const double sd = sqrt(d);
+    const double isd = 1.0 / sd;
+    if (t == 0 && piv) piv[jj] = sd;
+    double li[CH_R], lj[CH_R];
+#pragma unroll
+    for (int a = BJ; a < CH_R; ++a) {
+      const int i = ty + CH_G * a;
+      li[a] = (i > jj && i < m) ? col[i] * isd : 0.0;
+      const int j = tx + CH_G * a;
+      lj[a] = (j > jj && j < m) ? col[j] * isd : 0.0;
+    }
+    // trailing lower blocks BJ <= b <= a
+#pragma unroll
+    for (int a = BJ; a < CH_R; ++a)
+#pragma unroll
+      for (int b = BJ; b <= a; ++b) A[a][b] -= li[a] * lj[b];
+    if (tx == tj) {
+#pragma unroll
+      for (int a = BJ; a < CH_R; ++a) {
+        const int i = ty + CH_G * a;
+        if (i >= jj) A[a][BJ] = (i == jj) ? sd : li[a];
+      }
+    }
+  }
+}
+
+// forward-substitution steps jj = 32 AJ .. min(32 AJ + 31, m - 1) of B = L^-1:
+// row jj = B[jj,:] / L[jj,jj] ; rows i > jj: B[i,:] -= L[i,jj] B[jj,:]
+template <int AJ>
+__device__ __forceinline__ void inv_block(double (&A)[CH_R][CH_R], int m, const CholShared& sh, int tx, int ty) {
+  const int j0 = AJ * CH_G, j1 = min(m, j0 + CH_G);
+  for (int jj = j0; jj < j1; ++jj) {
+    double* row = sh.cb + (jj & 1) * JACOBI_MAX;
+    const double ild = 1.0 / sh.Lp[chol_pl(m, jj, jj)];
+    if (ty == jj - j0) {
+#pragma unroll
+      for (int b = 0; b <= AJ; ++b) {   // row jj of B is zero right of the diagonal
+        const int j = tx + CH_G * b;
+        const double v = A[AJ][b] * ild;
+        A[AJ][b] = v;
+        if (j < m) row[j] = v;
+      }
+    }
+    __syncthreads();
+    double xr[CH_R];
+#pragma unroll
+    for (int b = 0; b <= AJ; ++b) {
+      const int j = tx + CH_G * b;
+      xr[b] = (j <= jj) ? row[j] : 0.0;
+    }
+#pragma unroll
+    for (int a = AJ; a < CH_R; ++a) {
+      const int i = ty + CH_G * a;
+      const double lij = (i > jj && i < m) ? sh.Lp[chol_pl(m, i, jj)] : 0.0;
+#pragma unroll
+      for (int b = 0; b <= AJ; ++b) A[a][b] -= lij * xr[b];
+    }
+  }
+}
+
 __global__ __launch_bounds__(CH_G * CH_G) void chol_inv_kernel(const double* __restrict__ G, int m,
                                                                double shift, double* __restrict__ Rinv,
                                                                int* info, double* __restrict__ piv) {
   extern __shared__ double smem[];
-  double* Lp = smem;                                 // packed lower, column-major
-  double* cb = Lp + (size_t)m * (m + 1) / 2;        // 2 x JACOBI_MAX broadcast buffers
   __shared__ int bad;
+  CholShared sh{smem, smem + (size_t)m * (m + 1) / 2, &bad};
   const int t = threadIdx.x, tx = t % CH_G, ty = t / CH_G;
-  auto pl = [m](int i, int j) { return j * m - j * (j - 1) / 2 + (i - j); };
   double A[CH_R][CH_R];
 #pragma unroll
   for (int a = 0; a < CH_R; ++a) {
@@ -278,116 +369,33 @@ __global__ __launch_bounds__(CH_G * CH_G) void chol_inv_kernel(const double* __r
   }
   if (t == 0) bad = 0;
   __syncthreads();
-  // ---- right-looking Cholesky ----
-  for (int jj = 0; jj < m; ++jj) {
-    double* col = cb + (jj & 1) * JACOBI_MAX;
-    const int bj = jj / CH_G, tj = jj % CH_G;
-    if (tx == tj) {
-#pragma unroll
-      for (int a = 0; a < CH_R; ++a) {
-        const int i = ty + CH_G * a;
-        if (i >= jj && i < m) {
-          double v = 0.0;
-#pragma unroll
-          for (int b = 0; b < CH_R; ++b) v = (b == bj) ? A[a][b] : v;
-          col[i] = v;
-        }
-      }
-    }
-    __syncthreads();
-    double d = col[jj];
-    if (!(d > 0.0)) {
-      if (t == 0 && bad == 0) bad = jj + 1;
-      d = 1e-300;
-    }
-    const double sd = sqrt(d);
-    const double isd = 1.0 / sd;
-    if (t == 0 && piv) piv[jj] = sd;
-    double li[CH_R], lj[CH_R];
-#pragma unroll
-    for (int a = 0; a < CH_R; ++a) {
-      const int i = ty + CH_G * a;
-      li[a] = (a >= bj && i > jj && i < m) ? col[i] * isd : 0.0;
-    }
-#pragma unroll
-    for (int b = 0; b < CH_R; ++b) {
-      const int j = tx + CH_G * b;
-      lj[b] = (b >= bj && j > jj && j < m) ? col[j] * isd : 0.0;
-    }
-    // only the trailing lower blocks (bj <= b <= a) change; the branches are uniform (bj is
-    // the same for every thread), so finished and upper blocks cost nothing
-#pragma unroll
-    for (int a = 0; a < CH_R; ++a) {
-      if (a < bj) continue;
-#pragma unroll
-      for (int b = 0; b < CH_R; ++b) {
-        if (b < bj || b > a) continue;
-        A[a][b] -= li[a] * lj[b];
-      }
-    }
-    if (tx == tj) {
-#pragma unroll
-      for (int a = 0; a < CH_R; ++a) {
-        const int i = ty + CH_G * a;
-        const double lv = (i == jj) ? sd : li[a];
-#pragma unroll
-        for (int b = 0; b < CH_R; ++b)
-          if (b == bj && i >= jj) A[a][b] = lv;
-      }
-    }
-  }
+  // ---- right-looking Cholesky, six 32-column blocks ----
+  chol_block<0>(A, m, sh, tx, ty, t, piv);
+  chol_block<1>(A, m, sh, tx, ty, t, piv);
+  chol_block<2>(A, m, sh, tx, ty, t, piv);
+  chol_block<3>(A, m, sh, tx, ty, t, piv);
+  chol_block<4>(A, m, sh, tx, ty, t, piv);
+  chol_block<5>(A, m, sh, tx, ty, t, piv);
   // park L (lower part of A) in LDS
 #pragma unroll
   for (int a = 0; a < CH_R; ++a)
 #pragma unroll
-    for (int b = 0; b < CH_R; ++b) {
+    for (int b = 0; b <= a; ++b) {
       const int i = ty + CH_G * a, j = tx + CH_G * b;
-      if (i < m && j <= i) Lp[pl(i, j)] = A[a][b];
+      if (i < m && j <= i) sh.Lp[chol_pl(m, i, j)] = A[a][b];
     }
   __syncthreads();
-  // ---- B = L^-1: row jj = B[jj,:] / L[jj,jj]; rows i > jj: B[i,:] -= L[i,jj] B[jj,:] ----
+  // ---- B = L^-1 ----
 #pragma unroll
   for (int a = 0; a < CH_R; ++a)
 #pragma unroll
     for (int b = 0; b < CH_R; ++b) A[a][b] = (ty + CH_G * a == tx + CH_G * b) ? 1.0 : 0.0;
-  for (int jj = 0; jj < m; ++jj) {
-    double* row = cb + (jj & 1) * JACOBI_MAX;
-    const int aj = jj / CH_G, tj = jj % CH_G;
-    const double ild = 1.0 / Lp[pl(jj, jj)];
-    if (ty == tj) {
-#pragma unroll
-      for (int a = 0; a < CH_R; ++a) {
-        if (a != aj) continue;
-#pragma unroll
-        for (int b = 0; b < CH_R; ++b) {
-          const int j = tx + CH_G * b;
-          const double v = A[a][b] * ild;
-          A[a][b] = v;
-          if (j < m) row[j] = v;
-        }
-      }
-    }
-    __syncthreads();
-    // rows above jj are final and row jj of B is zero right of the diagonal: only blocks with
-    // a >= aj and b <= aj change (uniform branches)
-    double xr[CH_R];
-#pragma unroll
-    for (int b = 0; b < CH_R; ++b) {
-      const int j = tx + CH_G * b;
-      xr[b] = (b <= aj && j <= jj) ? row[j] : 0.0;
-    }
-#pragma unroll
-    for (int a = 0; a < CH_R; ++a) {
-      if (a < aj) continue;
-      const int i = ty + CH_G * a;
-      const double lij = (i > jj && i < m) ? Lp[pl(i, jj)] : 0.0;
-#pragma unroll
-      for (int b = 0; b < CH_R; ++b) {
-        if (b > aj) continue;
-        A[a][b] -= lij * xr[b];
-      }
-    }
-  }
+  inv_block<0>(A, m, sh, tx, ty);
+  inv_block<1>(A, m, sh, tx, ty);
+  inv_block<2>(A, m, sh, tx, ty);
+  inv_block<3>(A, m, sh, tx, ty);
+  inv_block<4>(A, m, sh, tx, ty);
+  inv_block<5>(A, m, sh, tx, ty);
   // Rinv(r, c) = B(c, r)
 #pragma unroll
   for (int a = 0; a < CH_R; ++a)
