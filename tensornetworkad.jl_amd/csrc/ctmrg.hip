@@ -287,7 +287,8 @@ static void orthonormalize(Ctx* c, double* Y, int n, int k, double* X, EigWork& 
   k_random_fill_cols(Y, n, k, w.nrm.as<double>(), 1e-13, seed, s);
   k_col_norms(Y, n, k, w.nrm.as<double>(), s);
   k_scale_cols(Y, n, k, w.nrm.as<double>(), 1, s);
-  // otherwise shifted CholeskyQR followed by two plain CholeskyQR passes (sCholQR3).  Columns that the
+  // otherwise CholeskyQR2 when the block is well conditioned, else shifted CholeskyQR followed by two
+  // plain CholeskyQR passes (sCholQR3).  Columns that the
   // shifted pass finds linearly dependent on earlier ones (pivot at the shift's level: the
   // block has lower rank than k, e.g. a rank-deficient corner matrix) are replaced by random
   // directions first, so the block always spans k dimensions.
@@ -326,6 +327,33 @@ static void orthonormalize(Ctx* c, double* Y, int n, int k, double* X, EigWork& 
   }
   const double u = 1.1102230246251565e-16;
   const double shift = 11.0 * ((double)n * k + (double)k * (k + 1)) * u * (double)k;
+  // a well-conditioned block takes two plain CholeskyQR passes (CholeskyQR2): when the Cholesky
+  // pivots of its Gram matrix (w.G, formed above) stay above 1e-4 (unit columns: the block's
+  // condition number is then around 1e4, far below u^-1/2 where CholeskyQR2 is orthogonal to
+  // working precision)
+  {
+    chol_inv(w.G.as<double>(), k, 0.0, w.Rinv.as<double>(), w.info.as<int>(), w.res.as<double>(), s);
+    int info = 0;
+    TNAD_HIP_TRY(hipMemcpyAsync(&info, w.info.ptr, sizeof(int), hipMemcpyDeviceToHost, s));
+    TNAD_HIP_TRY(hipMemcpyAsync(piv.data(), w.res.ptr, (size_t)k * 8, hipMemcpyDeviceToHost, s));
+    c->sync();
+    double pmin = HUGE_VAL;
+    for (int j = 0; j < k; ++j) pmin = std::min(pmin, piv[j]);
+    if (std::getenv("TNAD_DEBUG_EIG"))
+      std::fprintf(stderr, "[tnad eig] plain Cholesky info=%d min pivot=%.3e\n", info, pmin);
+    if (info == 0 && pmin > 1e-4) {
+      tc_launch(c, Prec::F64, gemm_desc('N', 'N', n, k, k, n, k, n), Y, w.Rinv.as<double>(), X, s);
+      tc_launch(c, Prec::F64, gemm_desc('T', 'N', k, k, n, n, n, k), X, X, w.G.as<double>(), s);
+      chol_inv(w.G.as<double>(), k, 0.0, w.Rinv.as<double>(), w.info.as<int>(), nullptr, s);
+      TNAD_HIP_TRY(hipMemcpyAsync(&info, w.info.ptr, sizeof(int), hipMemcpyDeviceToHost, s));
+      c->sync();
+      if (info != 0)   // breakdown (pivot <= 0): factor the shifted Gram instead
+        chol_inv(w.G.as<double>(), k, shift, w.Rinv.as<double>(), w.info.as<int>(), nullptr, s);
+      tc_launch(c, Prec::F64, gemm_desc('N', 'N', n, k, k, n, k, n), X, w.Rinv.as<double>(), Y, s);
+      k_copy(Y, X, (size_t)n * k, s);
+      return;
+    }
+  }
   std::vector<int> dead(k);
   // the Cholesky factor used below must belong to the block as it finally is: the last attempt
   // factors without refilling afterwards
@@ -412,24 +440,32 @@ int topk_eig(Ctx* c, const double* A, int n, int want, int k, const double* X0, 
     dbg_finite(c, "X", X, (size_t)n * k);
     dbg_finite(c, "H", w.H.as<double>(), (size_t)k * k);
     // Rayleigh-Ritz: once the block is nearly an invariant subspace (H nearly diagonal in the
-    // previous Ritz basis) second-order perturbation theory replaces the Jacobi sweeps; Jacobi on
-    // the first cold iteration, whenever H is not close enough to diagonal, and whenever the
-    // residual stopped falling (by 10% per iteration)
+    // previous Ritz basis) a few rotations of near-degenerate pairs plus second-order perturbation
+    // theory replace the Jacobi sweeps; Jacobi on the first cold iteration, whenever H is not close
+    // enough to diagonal, and whenever the residual stopped falling (by 10% per iteration)
     bool jac = force_jacobi || (it == 1 && !X0);
     if (!jac) {
-      ritz_perturb(w.H.as<double>(), k, k, want, 1e-3, theta, w.W.as<double>(), w.info.as<int>(), s);
+      ritz_select(w.H.as<double>(), k, k, want, 1e-3, theta, w.W.as<double>(), w.info.as<int>(), s);
       int flag = 1;
       TNAD_HIP_TRY(hipMemcpyAsync(&flag, w.info.ptr, sizeof(int), hipMemcpyDeviceToHost, s));
       c->sync();
       jac = flag != 0;
     }
     // far from convergence the Rayleigh-Ritz step only has to improve the basis (the span, which
-    // is what converges, does not depend on it): two Jacobi sweeps do; the sweeps run to
-    // completion once the residual is small enough for the result to be the answer
+    // is what converges, does not depend on it): two Jacobi sweeps do.  Closer in, three: Jacobi
+    // converges quadratically on a nearly diagonal H, and what is left is taken up by the next
+    // iteration's Rayleigh-Ritz (the residual test sees it).  The sweeps run to completion when
+    // the residual stopped falling or the iteration runs long.
     const bool rough = it == 1 || prev_rel > 1e-6;
-    if (jac) jacobi_eig_ws(w.H.as<double>(), k, k, theta, w.W.as<double>(), w.Vw.as<double>(), rough ? 2 : 60,
-                           nullptr, s);
+    const int sweeps = (force_jacobi || it > 8) ? 60 : (rough ? 2 : 3);
+    if (jac) jacobi_eig_ws(w.H.as<double>(), k, k, theta, w.W.as<double>(), w.Vw.as<double>(), sweeps,
+                           dbg ? w.sweeps.as<int>() : nullptr, s);
     n_jacobi += jac;
+    int sweeps_done = 0;
+    if (dbg && jac) {
+      TNAD_HIP_TRY(hipMemcpyAsync(&sweeps_done, w.sweeps.ptr, sizeof(int), hipMemcpyDeviceToHost, s));
+      c->sync();
+    }
     dbg_finite(c, "W", w.W.as<double>(), (size_t)k * k);
     dbg_finite(c, "theta", theta, (size_t)k);
     tc_launch(c, Prec::F64, gemm_desc('N', 'N', n, k, k, n, k, n), X, w.W.as<double>(), w.Xr.as<double>(), s);
@@ -444,8 +480,8 @@ int topk_eig(Ctx* c, const double* A, int n, int want, int k, const double* X0, 
     double mx = 0.0;
     for (int j = 0; j < want; ++j) mx = std::max(mx, hres[j]);
     rel = mx / std::max(std::fabs(hth[0]), 1e-300);
-    if (dbg) std::fprintf(stderr, "[tnad eig] n=%d k=%d want=%d it=%d rel_resid=%.3e theta0=%.6e rr=%s\n", n, k,
-                          want, it, rel, hth[0], jac ? "jacobi" : "perturb");
+    if (dbg) std::fprintf(stderr, "[tnad eig] n=%d k=%d want=%d it=%d rel_resid=%.3e theta0=%.6e rr=%s sweeps=%d\n",
+                          n, k, want, it, rel, hth[0], jac ? "jacobi" : "select", sweeps_done);
     force_jacobi = !(rel < 0.9 * prev_rel);
     prev_rel = rel;
     if (!(rel == rel)) throw Error(TNAD_ERR_NAN, "eigensolver: NaN in Ritz residual");

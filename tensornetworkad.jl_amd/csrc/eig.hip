@@ -7,6 +7,8 @@
 //    rotations are recorded and replayed on the identity by many workgroups.
 //    Used for the full spectrum when n <= 192 and for the Rayleigh-Ritz problem
 //    of the block subspace iteration.
+//  * ritz_select_kernel : Rayleigh-Ritz of a nearly diagonal matrix: a few Jacobi
+//    rotations for the strongly coupled pairs, perturbation theory for the rest.
 //  * chol_inv_kernel : Cholesky of a (shifted) Gram matrix followed by the
 //    triangular inverse, register-resident, for CholeskyQR orthonormalisation.
 #include "tnad_internal.h"
@@ -32,17 +34,47 @@ __device__ __forceinline__ void decode_tri(int b, int& i, int& j) {
   i = b - jj * (jj + 1) / 2;
 }
 
-__device__ __forceinline__ void rr_pair(int r_in, int t, int mm, int& p, int& q) {
-  // round-robin (circle) pairing of round r_in, slot t
-  if (t == 0) { p = r_in; q = mm - 1; }
-  else { p = (r_in + t) % (mm - 1); q = (r_in - t + (mm - 1)) % (mm - 1); }
+__device__ __forceinline__ void rr_pair(int r_in, int t, int M, int& p, int& q) {
+  // round-robin (circle) pairing of round r_in (0 <= r_in < M = mm - 1), slot t (0 <= t < mm / 2):
+  // slot 0 pairs r_in with the fixed index M, slot t > 0 pairs r_in + t with r_in - t (mod M)
+  if (t == 0) { p = r_in; q = M; }
+  else {
+    p = r_in + t;
+    if (p >= M) p -= M;
+    q = r_in - t;
+    if (q < 0) q += M;
+  }
   if (p > q) { int tmp = p; p = q; q = tmp; }
 }
 
-// Jacobi sweeps on the packed matrix in LDS.  The rotations of every round are
-// recorded (rot[(round * P + slot) * 2 + {0,1}] = {c, s}); the eigenvector
-// matrix is formed afterwards by jacobi_apply_kernel.  Writes the (unsorted)
+__device__ __forceinline__ int pk24(int lo, int hi) {  // pk for 0 <= lo <= hi < 2^12, 24-bit multiply
+  return (int)(__umul24((unsigned)hi, (unsigned)hi + 1u) >> 1) + lo;
+}
+
+// Jacobi sweeps on the packed matrix in LDS, stored in slot-position order.  Round r of the
+// round-robin ordering pairs, in slot t, the indices a = r + t and b = r - t (mod M = mm - 1; slot 0
+// pairs r with the fixed index M).  Position 2t holds a and 2t + 1 holds b, so every round works on
+// the same positions: a thread's 2x2 blocks (pairs of slots) sit at addresses fixed for the whole
+// run.  Between rounds the indices move by a fixed permutation of positions (a-members one slot
+// down, b-members one slot up, around the ends; see jpos_next), which the rotated blocks follow on
+// write-back; after a full sweep (M rounds) every index is back where it started.  An odd m is
+// padded to even mm with a zero row and column (never rotated).  The rotations of every round are
+// recorded for the index pair (p, q) = (min, max) of each slot (rot[round * P + slot] = {c, s});
+// the eigenvector matrix is formed afterwards by jacobi_apply_kernel.  Writes the (unsorted)
 // diagonal to diag and the number of rounds to *rounds_out.
+__device__ __forceinline__ int jpos_next(int pos, int P) {   // position of an index one round later
+  const int t = pos >> 1;
+  if ((pos & 1) == 0) return t == 0 ? 3 : (t == 1 ? 0 : pos - 2);   // a-members move down
+  if (t == 0) return 1;                                              // the fixed index M
+  return t == P - 1 ? pos - 1 : pos + 2;                             // b-members move up
+}
+__device__ __forceinline__ int jpos_index0(int pos, int M) {   // index at a position in round 0
+  const int t = pos >> 1;
+  if (pos == 0) return 0;
+  if (pos == 1) return M;
+  return (pos & 1) ? M - t : t;
+}
+
 __global__ __launch_bounds__(EIG_NT) void jacobi_rot_kernel(const double* __restrict__ H, int ldh, int m,
                                                             double* __restrict__ rot, int max_sweeps,
                                                             double* __restrict__ diag, int* rounds_out,
@@ -50,28 +82,24 @@ __global__ __launch_bounds__(EIG_NT) void jacobi_rot_kernel(const double* __rest
   extern __shared__ double smem[];
   const int mm = m + (m & 1);
   const int P = mm / 2;
-  double* a = smem;                                  // packed m x m upper
-  double* rc = a + (m * (m + 1) / 2);                // P cosines
-  double* rs = rc + P;                               // P sines
-  int* rp = reinterpret_cast<int*>(rs + P);          // P first index
-  int* rq = rp + P;                                  // P second index
+  const int M = mm - 1;
+  const int ntri = mm * (mm + 1) / 2;
+  double* a = smem;                              // packed mm x mm upper, position order
+  __shared__ double2 rcs[JACOBI_MAX / 2];        // (cos, sin) per slot (apart from a: no aliasing)
   __shared__ double red[EIG_NT / 64];
   __shared__ int nrot_s;
   __shared__ double fro_s;
   const int t = threadIdx.x;
 
-  // load and symmetrise (average both triangles so a slightly asymmetric input is still handled)
-  for (int b = t; b < m * (m + 1) / 2; b += EIG_NT) {
-    int i, j;
-    decode_tri(b, i, j);
-    a[b] = 0.5 * (H[i + (long long)j * ldh] + H[j + (long long)i * ldh]);
-  }
-  __syncthreads();
+  // load (position order of round 0) and symmetrise (average both triangles so a slightly
+  // asymmetric input is still handled)
   double part = 0.0;
-  for (int b = t; b < m * (m + 1) / 2; b += EIG_NT) {
+  for (int b = t; b < ntri; b += EIG_NT) {
     int i, j;
     decode_tri(b, i, j);
-    double v = a[b];
+    const int x = jpos_index0(i, M), y = jpos_index0(j, M);
+    const double v = (x < m && y < m) ? 0.5 * (H[x + (long long)y * ldh] + H[y + (long long)x * ldh]) : 0.0;
+    a[b] = v;
     part += (i == j ? 1.0 : 2.0) * v * v;
   }
   for (int o = 32; o > 0; o >>= 1) part += __shfl_down(part, o);
@@ -88,9 +116,11 @@ __global__ __launch_bounds__(EIG_NT) void jacobi_rot_kernel(const double* __rest
   // whose spectrum spans many decades.  Above that, the relative criterion resolves small
   // eigenvalues to their own scale.
   const double abs_floor = 4.5e-16 * fro_s + 1e-300;
-  const double rel_tol = 1e-15;
-  // unordered slot pairs (i <= j) handled by this thread, packed i | j << 16 (-1: none)
-  int bij[JB_MAX];
+  const double rel_tol2 = 1e-30;   // (1e-15)^2
+  // this thread's blocks: unordered slot pairs (i <= j), packed i | j << 16 (-1: none), with the
+  // addresses of their four entries (a_i a_j, a_i b_j, b_i a_j, b_i b_j) now and one round later,
+  // packed now | later << 16 (the packed triangle has fewer than 2^16 entries)
+  int bij[JB_MAX], adr[JB_MAX][4];
   {
     const int nblk = P * (P + 1) / 2;
 #pragma unroll
@@ -99,150 +129,193 @@ __global__ __launch_bounds__(EIG_NT) void jacobi_rot_kernel(const double* __rest
       int i = 0, j = 0;
       if (b < nblk) decode_tri(b, i, j);
       bij[u] = (b < nblk) ? (i | (j << 16)) : -1;
+      const int pr[2] = {2 * i, 2 * i + 1}, pc[2] = {2 * j, 2 * j + 1};
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int x = pr[e >> 1], y = pc[e & 1];
+        const int xn = jpos_next(x, P), yn = jpos_next(y, P);
+        adr[u][e] = pk(min(x, y), max(x, y)) | (pk(min(xn, yn), max(xn, yn)) << 16);
+      }
     }
   }
+  // rotation slot of this thread (t < P): its positions' addresses are fixed too
+  const int s_aa = pk(2 * t, 2 * t), s_ab = pk(2 * t, 2 * t + 1), s_bb = pk(2 * t + 1, 2 * t + 1);
 
   int round = 0;
   int sweep = 0;
   for (; sweep < max_sweeps; ++sweep) {
     if (t == 0) nrot_s = 0;
     __syncthreads();
-    for (int r = 0; r < mm - 1; ++r, ++round) {
+    for (int r = 0; r < M; ++r, ++round) {
       if (t < P) {
-        int p, q;
-        rr_pair(r, t, mm, p, q);
         double c = 1.0, sn = 0.0;
-        if (q < m) {
-          double apq = a[pk(p, q)];
-          double app = a[pk(p, p)], aqq = a[pk(q, q)];
-          if (fabs(apq) > abs_floor && fabs(apq) > rel_tol * sqrt(fabs(app * aqq))) {
-            double theta = (aqq - app) / (2.0 * apq);
-            double tt = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
-            c = 1.0 / sqrt(tt * tt + 1.0);
-            sn = tt * c;
-            atomicAdd(&nrot_s, 1);
-          }
+        const double aab = a[s_ab], aaa = a[s_aa], abb = a[s_bb];
+        if (fabs(aab) > abs_floor && aab * aab > rel_tol2 * fabs(aaa * abb)) {
+          const double theta = (abb - aaa) / (2.0 * aab);
+          const double tt = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
+          c = 1.0 / sqrt(tt * tt + 1.0);
+          sn = tt * c;
+          nrot_s = 1;   // every writer stores the same value
         }
-        rc[t] = c; rs[t] = sn; rp[t] = p; rq[t] = q;
-        rot[((size_t)round * P + t) * 2 + 0] = c;
-        rot[((size_t)round * P + t) * 2 + 1] = sn;
+        rcs[t] = make_double2(c, sn);
+        // the record is for the sorted index pair: (c, s) on (a, b) is (c, -s) on (b, a)
+        int ia, ib;
+        if (t == 0) { ia = r; ib = M; }
+        else {
+          ia = r + t;
+          if (ia >= M) ia -= M;
+          ib = r - t;
+          if (ib < 0) ib += M;
+        }
+        reinterpret_cast<double2*>(rot)[(size_t)round * P + t] = make_double2(c, ia < ib ? sn : -sn);
       }
       __syncthreads();
-      // 2x2 block updates A' = Ji^T A Jj over unordered pairs of rotation slots (this thread's
-      // slot pairs are fixed for the whole run and were decoded once, see bij)
+      // 2x2 block updates A' = Ji^T A Jj over unordered pairs of rotation slots.  Every entry belongs
+      // to exactly one block; all blocks are read, then (after a barrier, since the write-back goes
+      // to next round's positions) written.  A diagonal block (i == j) holds (a, b) twice: the same
+      // formula gives its rotated diagonal, and its off-diagonal entry is set to zero.
+      double x[JB_MAX][4];
+#pragma unroll
+      for (int u = 0; u < JB_MAX; ++u) {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) x[u][e] = a[adr[u][e] & 0xffff];
+      }
+#pragma unroll
+      for (int u = 0; u < JB_MAX; ++u) {
+        const int bb = bij[u] < 0 ? 0 : bij[u];
+        const double2 csi = rcs[bb & 0xffff], csj = rcs[bb >> 16];
+        const double ci = csi.x, si = csi.y, cj = csj.x, sj = csj.y;
+        const bool dgn = (bb & 0xffff) == (bb >> 16);
+        const double ypr = ci * x[u][0] - si * x[u][2], yps = ci * x[u][1] - si * x[u][3];
+        const double yqr = si * x[u][0] + ci * x[u][2], yqs = si * x[u][1] + ci * x[u][3];
+        x[u][0] = cj * ypr - sj * yps;
+        x[u][1] = dgn ? 0.0 : sj * ypr + cj * yps;
+        x[u][2] = dgn ? 0.0 : cj * yqr - sj * yqs;
+        x[u][3] = sj * yqr + cj * yqs;
+      }
+      __syncthreads();
 #pragma unroll
       for (int u = 0; u < JB_MAX; ++u) {
         if (bij[u] < 0) continue;
-        const int i = bij[u] & 0xffff, j = bij[u] >> 16;
-        const double ci = rc[i], si = rs[i], cj = rc[j], sj = rs[j];
-        if (si == 0.0 && sj == 0.0) continue;
-        const int p = rp[i], q = rq[i];
-        if (i == j) {
-          if (q >= m) continue;
-          double app = a[pk(p, p)], aqq = a[pk(q, q)], apq = a[pk(p, q)];
-          double c2 = ci * ci, s2 = si * si, cs = ci * si;
-          a[pk(p, p)] = c2 * app - 2.0 * cs * apq + s2 * aqq;
-          a[pk(q, q)] = s2 * app + 2.0 * cs * apq + c2 * aqq;
-          a[pk(p, q)] = 0.0;
-        } else {
-          const int r0 = rp[j], s0 = rq[j];
-          const bool qv = q < m, sv = s0 < m;
-          double xpr = a[pku(p, r0)];
-          double xps = sv ? a[pku(p, s0)] : 0.0;
-          double xqr = qv ? a[pku(q, r0)] : 0.0;
-          double xqs = (qv && sv) ? a[pku(q, s0)] : 0.0;
-          double ypr = ci * xpr - si * xqr, yps = ci * xps - si * xqs;
-          double yqr = si * xpr + ci * xqr, yqs = si * xps + ci * xqs;
-          double zpr = cj * ypr - sj * yps, zps = sj * ypr + cj * yps;
-          double zqr = cj * yqr - sj * yqs, zqs = sj * yqr + cj * yqs;
-          a[pku(p, r0)] = zpr;
-          if (sv) a[pku(p, s0)] = zps;
-          if (qv) a[pku(q, r0)] = zqr;
-          if (qv && sv) a[pku(q, s0)] = zqs;
-        }
+#pragma unroll
+        for (int e = 0; e < 4; ++e) a[(unsigned)adr[u][e] >> 16] = x[u][e];
       }
       __syncthreads();
     }
     if (nrot_s == 0) { ++sweep; break; }
     __syncthreads();
   }
-  for (int i = t; i < m; i += EIG_NT) diag[i] = a[pk(i, i)];
+  // whole sweeps ran: every index is back at its round-0 position
+  for (int i = t; i < m; i += EIG_NT) {
+    int pos;
+    if (i == 0) pos = 0;
+    else if (i == M) pos = 1;
+    else if (i < P) pos = 2 * i;
+    else pos = 2 * (M - i) + 1;
+    diag[i] = a[pk(pos, pos)];
+  }
   if (t == 0) {
     *rounds_out = round;
     if (sweeps_out) *sweeps_out = sweep;
   }
 }
 
-// Replays the recorded rotations on rows of V (V = product of all rotations,
-// starting from the identity).  Each workgroup owns JROWS rows in LDS; rows
-// are independent, rotations of one round touch disjoint column pairs.
-constexpr int JROWS = 8;
+// Replays the recorded rotations on rows of V (V = product of all rotations, starting from the
+// identity).  Rows are independent: each wavefront owns one row in LDS and walks through the rounds
+// on its own (the rotations of one round touch disjoint column pairs, and a wavefront's LDS accesses
+// complete in order), so workgroup barriers are only needed to stage rotation parameters.
+constexpr int JA_WAVES = 4;   // rows (wavefronts) per workgroup
 constexpr int JCHUNK = 16;    // rounds of rotation parameters staged in LDS at once
-__global__ __launch_bounds__(256) void jacobi_apply_kernel(const double* __restrict__ rot,
-                                                           const int* __restrict__ rounds_ptr, int m,
-                                                           double* __restrict__ Vrows) {
-  extern __shared__ double smem[];
+__global__ __launch_bounds__(64 * JA_WAVES) void jacobi_apply_kernel(const double* __restrict__ rot,
+                                                                    const int* __restrict__ rounds_ptr, int m,
+                                                                    double* __restrict__ Vrows) {
+  __shared__ double rows[JA_WAVES][JACOBI_MAX];
+  __shared__ double2 prm[JCHUNK * (JACOBI_MAX / 2)];
   const int mm = m + (m & 1);
   const int P = mm / 2;
-  double* rows = smem;                        // JROWS x m
-  double* prm = rows + JROWS * m;             // JCHUNK x P x 2
-  const int t = threadIdx.x;
-  const int row0 = blockIdx.x * JROWS;
-  const int nrows = min(JROWS, m - row0);
-  for (int e = t; e < JROWS * m; e += blockDim.x) {
-    int rr = e / m, cc = e % m;
-    rows[e] = (rr < nrows && row0 + rr == cc) ? 1.0 : 0.0;
-  }
-  const int R = *rounds_ptr;
-  __syncthreads();
-  for (int base = 0; base < R; base += JCHUNK) {
-    const int nr = min(JCHUNK, R - base);
-    for (int e = t; e < nr * P * 2; e += blockDim.x) prm[e] = rot[(size_t)base * P * 2 + e];
+  const int M = mm - 1;
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  const int row = blockIdx.x * JA_WAVES + wv;
+  const bool live = row < m;
+  double* R = rows[wv];
+  for (int c = lane; c < mm; c += 64) R[c] = (c == row) ? 1.0 : 0.0;
+  const int Rn = *rounds_ptr;
+  const double2* src = reinterpret_cast<const double2*>(rot);
+  for (int base = 0; base < Rn; base += JCHUNK) {
+    const int nr = min(JCHUNK, Rn - base);
+    __syncthreads();   // the previous chunk is consumed
+    for (int e = t; e < nr * P; e += 64 * JA_WAVES) prm[e] = src[(size_t)base * P + e];
     __syncthreads();
-    for (int rr = 0; rr < nr; ++rr) {
-      const int r_in = (base + rr) % (mm - 1);
-      const double* pr = prm + (size_t)rr * P * 2;
-      for (int e = t; e < nrows * P; e += blockDim.x) {
-        const int row = e / P, slot = e % P;
-        const double sn = pr[2 * slot + 1];
-        if (sn == 0.0) continue;
-        const double c = pr[2 * slot];
-        int p, q;
-        rr_pair(r_in, slot, mm, p, q);
-        double* R_ = rows + row * m;
-        double vp = R_[p], vq = R_[q];
-        R_[p] = c * vp - sn * vq;
-        R_[q] = sn * vp + c * vq;
+    if (live) {
+      int r_in = base % M;
+      // a lane turns slots lane and lane + 64 (P <= 96); the two pairs are disjoint, so both are
+      // read before either is written
+      const bool two = lane + 64 < P;
+      for (int rr = 0; rr < nr; ++rr) {
+        const double2 cs0 = prm[rr * P + lane];
+        const double2 cs1 = two ? prm[rr * P + lane + 64] : make_double2(1.0, 0.0);
+        int p0 = 0, q0 = 0, p1 = 0, q1 = 0;   // lanes without a slot read entry 0 and write nothing
+        if (lane < P) rr_pair(r_in, lane, M, p0, q0);
+        if (two) rr_pair(r_in, lane + 64, M, p1, q1);
+        const bool act0 = lane < P && cs0.y != 0.0, act1 = cs1.y != 0.0;
+        const double v0p = R[p0], v0q = R[q0], v1p = R[p1], v1q = R[q1];
+        if (act0) {
+          R[p0] = cs0.x * v0p - cs0.y * v0q;
+          R[q0] = cs0.y * v0p + cs0.x * v0q;
+        }
+        if (act1) {
+          R[p1] = cs1.x * v1p - cs1.y * v1q;
+          R[q1] = cs1.y * v1p + cs1.x * v1q;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        if (++r_in == M) r_in = 0;
       }
-      __syncthreads();
     }
   }
-  for (int e = t; e < nrows * m; e += blockDim.x) {
-    int rr = e / m, cc = e % m;
-    Vrows[(row0 + rr) + (size_t)m * cc] = rows[rr * m + cc];   // column-major V
-  }
+  if (live)
+    for (int c = lane; c < m; c += 64) Vrows[row + (size_t)m * c] = R[c];   // column-major V
 }
 
 // sort by |lambda| descending (stable), sign convention: largest |component| of each vector positive
-__global__ __launch_bounds__(EIG_NT) void jacobi_sort_kernel(const double* __restrict__ diag,
-                                                             const double* __restrict__ Vu, int m,
-                                                             double* __restrict__ evals, double* __restrict__ V) {
-  for (int i = threadIdx.x; i < m; i += EIG_NT) {
-    double li = diag[i];
-    double ai = fabs(li);
-    int rank = 0;
-    for (int j = 0; j < m; ++j) {
-      double aj = fabs(diag[j]);
-      if (aj > ai || (aj == ai && j < i)) ++rank;
-    }
-    evals[rank] = li;
-    double best = -1.0, sgn = 1.0;
-    for (int row = 0; row < m; ++row) {
-      double v = Vu[row + (long long)i * m];
-      if (fabs(v) > best) { best = fabs(v); sgn = v < 0 ? -1.0 : 1.0; }
-    }
-    for (int row = 0; row < m; ++row) V[row + (long long)rank * m] = sgn * Vu[row + (long long)i * m];
+// (the first such component when several tie).  One workgroup per (unsorted) eigenpair.
+constexpr int JS_NT = 256;
+__global__ __launch_bounds__(JS_NT) void jacobi_sort_kernel(const double* __restrict__ diag,
+                                                            const double* __restrict__ Vu, int m,
+                                                            double* __restrict__ evals, double* __restrict__ V) {
+  __shared__ int cnt_w[JS_NT / 64], row_w[JS_NT / 64];
+  __shared__ double best_w[JS_NT / 64];
+  const int i = blockIdx.x, t = threadIdx.x;
+  const double li = diag[i], ai = fabs(li);
+  const double* vi = Vu + (long long)i * m;
+  int cnt = 0;
+  double best = -1.0;
+  int brow = m;
+  for (int j = t; j < m; j += JS_NT) {
+    const double aj = fabs(diag[j]);
+    cnt += (aj > ai) || (aj == ai && j < i);
+    const double v = fabs(vi[j]);
+    if (v > best) { best = v; brow = j; }
   }
+  for (int o = 32; o > 0; o >>= 1) {
+    cnt += __shfl_down(cnt, o);
+    const double ob = __shfl_down(best, o);
+    const int orow = __shfl_down(brow, o);
+    if (ob > best || (ob == best && orow < brow)) { best = ob; brow = orow; }
+  }
+  if ((t & 63) == 0) { cnt_w[t >> 6] = cnt; best_w[t >> 6] = best; row_w[t >> 6] = brow; }
+  __syncthreads();
+  int rank = 0;
+  double bb = -1.0;
+  int br = m;
+  for (int w = 0; w < JS_NT / 64; ++w) {
+    rank += cnt_w[w];
+    if (best_w[w] > bb || (best_w[w] == bb && row_w[w] < br)) { bb = best_w[w]; br = row_w[w]; }
+  }
+  const double sgn = (br < m && vi[br] < 0) ? -1.0 : 1.0;
+  if (t == 0) evals[rank] = li;
+  for (int row = t; row < m; row += JS_NT) V[row + (long long)rank * m] = sgn * vi[row];
 }
 
 // Cholesky (with diagonal shift) + triangular inverse of an m x m SPD matrix, m <= 192,
@@ -409,63 +482,200 @@ __global__ __launch_bounds__(CH_G * CH_G) void chol_inv_kernel(const double* __r
 
 
 // Rayleigh-Ritz of a nearly diagonal symmetric H (m <= 192, one workgroup) for its first nh
-// ("kept") columns: with d = diag(H) and E the off-diagonal part restricted to pairs that involve
-// a kept column, when max |E_ij| / |d_i - d_j| over those pairs is <= thresh the eigenpairs are,
-// to second order in E,
-//   theta_j = d_j + sum_{i != j} E_ij^2 / (d_j - d_i),   v_j = (e_j + sum_{i != j} e_i E_ij / (d_j - d_i)) / norm,
-// written sorted by |theta| descending (stable).  Pairs among the further columns (i, j >= nh) are
-// left unrotated: those vectors are not converged and only widen the search space, which any
-// mixing among them leaves unchanged -- provided the further block cannot hold a direction that
-// belongs among the kept ones, which its Gershgorin bound (below the smallest kept |d_i| by 0.1%)
-// rules out.  Otherwise *flag = 1 and nothing else is written (the caller falls back to Jacobi).  The vectors are orthonormal to O((E/gap)^2); the subspace iteration's
-// residual test sees any remainder.
-__global__ __launch_bounds__(EIG_NT) void ritz_perturb_kernel(const double* __restrict__ H, int ldh, int m,
-                                                              int nh, double thresh, double* __restrict__ evals,
-                                                              double* __restrict__ V, int* __restrict__ flag) {
+// ("kept") columns.  Off-diagonal entries at or below the rounding level of H (4.5e-16 |H|_F, where
+// the Jacobi sweeps stop as well) count as zero.
+//  1. Pairs that involve a kept column and are strongly coupled, |H_ij| > thresh |H_ii - H_jj|
+//     (near-degenerate pairs, which perturbation theory cannot treat), are removed by Jacobi
+//     rotations on the packed triangle in LDS: each pass rotates a greedy set of disjoint strong
+//     pairs, until none is left (at most RS_PASSES passes, RS_MAXROT rotations, RS_LIST strong pairs
+//     per pass).  A rotation keeps the larger diagonal entry at the smaller index, so the kept
+//     positions keep the larger values.
+//  2. The rotated matrix, diagonal d plus weak couplings E (restricted to pairs with a kept column),
+//     has the eigenpairs, to second order in E,
+//        theta_j = d_j + sum_{i != j} E_ij^2 / (d_j - d_i),
+//        v_j = (e_j + sum_{i != j} e_i E_ij / (d_j - d_i)) / norm,
+//     and those of H are the rotations of step 1 applied to v_j.  Pairs among the further columns
+//     (i, j >= nh) are left unrotated: those vectors are not converged and only widen the search
+//     space, which any mixing among them leaves unchanged -- provided the further block cannot hold
+//     a direction that belongs among the kept ones, which its Gershgorin bound (below the smallest
+//     kept |d_i| by 0.1%) rules out.
+// The pairs are written sorted by |theta| descending (stable) and *flag = 0.  When a limit of step 1
+// is hit or the Gershgorin test fails, *flag = 1 and nothing else is written (the caller falls back
+// to Jacobi).  The vectors are orthonormal to O((E/gap)^2); the subspace iteration's residual test
+// sees any remainder.
+constexpr int RS_LIST = 256;
+constexpr int RS_PASSES = 12;
+constexpr int RS_MAXROT = JACOBI_MAX;
+
+__global__ __launch_bounds__(EIG_NT) void ritz_select_kernel(const double* __restrict__ H, int ldh, int m,
+                                                             int nh, double thresh, double* __restrict__ evals,
+                                                             double* __restrict__ V, int* __restrict__ flag) {
+  extern __shared__ double smem[];
+  double* a = smem;   // packed upper triangle, pk(i, j)
   __shared__ double dg[JACOBI_MAX], th[JACOBI_MAX], nr[JACOBI_MAX];
-  __shared__ int rk[JACOBI_MAX];
-  __shared__ double red[EIG_NT / 64];
-  __shared__ int bad;
+  __shared__ int rk[JACOBI_MAX], selid[JACOBI_MAX];
+  __shared__ int lst[RS_LIST], lsts[RS_LIST];
+  __shared__ double rc[RS_MAXROT], rsn[RS_MAXROT];
+  __shared__ int rpq[RS_MAXROT];
+  __shared__ double red[EIG_NT / 64], red2[EIG_NT / 64];
+  __shared__ int cnt_s, nsel_s, nrot_s, bad_s;
+  __shared__ double fro_s;
   const int t = threadIdx.x;
-  for (int i = t; i < m; i += EIG_NT) dg[i] = H[i + (long long)ldh * i];
-  __syncthreads();
-  double worst = 0.0;
-  for (int e = t; e < m * m; e += EIG_NT) {
-    const int i = e % m, j = e / m;
-    if (i >= j || i >= nh) continue;   // i < j: pairs with a kept column have i < nh
-    const double eij = 0.5 * (H[i + (long long)ldh * j] + H[j + (long long)ldh * i]);
-    const double gap = fabs(dg[i] - dg[j]);
-    const double r = (eij == 0.0) ? 0.0 : (gap > 0.0 ? fabs(eij) / gap : HUGE_VAL);
-    worst = fmax(worst, r);
+  const int ntri = m * (m + 1) / 2;
+  double part = 0.0;
+  for (int b = t; b < ntri; b += EIG_NT) {
+    int i, j;
+    decode_tri(b, i, j);
+    const double v = 0.5 * (H[i + (long long)ldh * j] + H[j + (long long)ldh * i]);
+    a[b] = v;
+    part += (i == j ? 1.0 : 2.0) * v * v;
   }
-  // the unrotated further columns must not hide a direction that belongs among the kept ones:
-  // Gershgorin bound of the further block below the smallest kept |diagonal| by a margin
-  double gmax = 0.0;
+  for (int o = 32; o > 0; o >>= 1) part += __shfl_down(part, o);
+  if ((t & 63) == 0) red[t >> 6] = part;
+  if (t == 0) { nrot_s = 0; bad_s = 0; }
+  __syncthreads();
+  if (t == 0) {
+    double acc = 0.0;
+    for (int w = 0; w < EIG_NT / 64; ++w) acc += red[w];
+    fro_s = sqrt(acc);
+  }
+  __syncthreads();
+  const double floor_abs = 4.5e-16 * fro_s + 1e-300;
+
+  // ---- 1. rotate the strongly coupled pairs away ----
+  for (int pass = 0;; ++pass) {
+    for (int i = t; i < m; i += EIG_NT) { dg[i] = a[pk(i, i)]; selid[i] = -1; }
+    if (t == 0) cnt_s = 0;
+    __syncthreads();
+    for (int b = t; b < ntri; b += EIG_NT) {
+      int i, j;
+      decode_tri(b, i, j);
+      if (i == j || i >= nh) continue;   // i < j: a pair with a kept column has i < nh
+      const double e = fabs(a[b]);
+      if (e <= floor_abs || e <= thresh * fabs(dg[i] - dg[j])) continue;
+      const int slot = atomicAdd(&cnt_s, 1);
+      if (slot < RS_LIST) lst[slot] = i | (j << 16);
+    }
+    __syncthreads();
+    const int cnt = cnt_s;
+    if (cnt == 0) break;
+    if (cnt > RS_LIST || pass == RS_PASSES) {
+      if (t == 0) bad_s = 1;
+      break;
+    }
+    // the list fills in arrival order: sort it (keys are distinct) so that the selection, and with it
+    // the result, does not depend on thread timing
+    for (int u = t; u < cnt; u += EIG_NT) {
+      const int key = lst[u];
+      int r = 0;
+      for (int w = 0; w < cnt; ++w) r += lst[w] < key;
+      lsts[r] = key;
+    }
+    __syncthreads();
+    if (t == 0) {   // greedy set of disjoint pairs, within the rotation budget
+      const int r0 = nrot_s;
+      int ns = 0;
+      for (int u = 0; u < cnt && r0 + ns < RS_MAXROT; ++u) {
+        const int i = lsts[u] & 0xffff, j = lsts[u] >> 16;
+        if (selid[i] < 0 && selid[j] < 0) {
+          selid[i] = ns;
+          selid[j] = ns;
+          rpq[r0 + ns] = i | (j << 16);
+          ++ns;
+        }
+      }
+      nsel_s = ns;
+      if (ns == 0) bad_s = 1;
+    }
+    __syncthreads();
+    if (bad_s) break;
+    const int ns = nsel_s, r0 = nrot_s;
+    if (t < ns) {
+      const int p = rpq[r0 + t] & 0xffff, q = rpq[r0 + t] >> 16;
+      const double app = dg[p], aqq = dg[q], apq = a[pk(p, q)];
+      const double theta = (aqq - app) / (2.0 * apq);
+      const double tt = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
+      double c = 1.0 / sqrt(tt * tt + 1.0), sn = tt * c;
+      if (fabs(app - tt * apq) < fabs(aqq + tt * apq)) {   // swap the two new directions
+        const double c2 = sn;
+        sn = -c;
+        c = c2;
+      }
+      rc[r0 + t] = c;
+      rsn[r0 + t] = sn;
+    }
+    __syncthreads();
+    // A <- J^T A J.  Work item (u, k): selected pair u = (p, q) with index k.  k outside every selected
+    // pair: the two entries (p, k), (q, k) turn by pair u alone; k = the first index of a selected pair
+    // v > u: the 2x2 block of pairs u and v turns on both sides; k = p: the diagonal block of pair u.
+    for (int e = t; e < ns * m; e += EIG_NT) {
+      const int u = e / m, k = e % m;
+      const int p = rpq[r0 + u] & 0xffff, q = rpq[r0 + u] >> 16;
+      const double ci = rc[r0 + u], si = rsn[r0 + u];
+      const int v = selid[k];
+      if (v < 0) {
+        const double xp = a[pku(p, k)], xq = a[pku(q, k)];
+        a[pku(p, k)] = ci * xp - si * xq;
+        a[pku(q, k)] = si * xp + ci * xq;
+      } else if (v == u) {
+        if (k != p) continue;
+        const double app = a[pk(p, p)], aqq = a[pk(q, q)], apq = a[pk(p, q)];
+        const double c2 = ci * ci, s2 = si * si, cs = ci * si;
+        a[pk(p, p)] = c2 * app - 2.0 * cs * apq + s2 * aqq;
+        a[pk(q, q)] = s2 * app + 2.0 * cs * apq + c2 * aqq;
+        a[pk(p, q)] = 0.0;
+      } else {
+        const int r = rpq[r0 + v] & 0xffff, s = rpq[r0 + v] >> 16;
+        if (v < u || k != r) continue;
+        const double cj = rc[r0 + v], sj = rsn[r0 + v];
+        const double xpr = a[pku(p, r)], xps = a[pku(p, s)], xqr = a[pku(q, r)], xqs = a[pku(q, s)];
+        const double ypr = ci * xpr - si * xqr, yps = ci * xps - si * xqs;
+        const double yqr = si * xpr + ci * xqr, yqs = si * xps + ci * xqs;
+        a[pku(p, r)] = cj * ypr - sj * yps;
+        a[pku(p, s)] = sj * ypr + cj * yps;
+        a[pku(q, r)] = cj * yqr - sj * yqs;
+        a[pku(q, s)] = sj * yqr + cj * yqs;
+      }
+    }
+    __syncthreads();
+    if (t == 0) nrot_s = r0 + ns;
+    __syncthreads();
+  }
+  __syncthreads();
+  if (bad_s) {
+    if (t == 0) *flag = 1;
+    return;
+  }
+  const int nrot = nrot_s;
+
+  // ---- 2. the further block must not hide a kept direction (Gershgorin) ----
+  double gmax = 0.0, hmin = HUGE_VAL;
   for (int i = nh + t; i < m; i += EIG_NT) {
     double rs = 0.0;
-    for (int j = nh; j < m; ++j) rs += fabs(i == j ? dg[i] : 0.5 * (H[i + (long long)ldh * j] + H[j + (long long)ldh * i]));
+    for (int j = nh; j < m; ++j) rs += fabs(a[pku(i, j)]);
     gmax = fmax(gmax, rs);
   }
-  double hmin = HUGE_VAL;
   for (int i = t; i < nh; i += EIG_NT) hmin = fmin(hmin, fabs(dg[i]));
   for (int o = 32; o > 0; o >>= 1) {
-    worst = fmax(worst, __shfl_down(worst, o));
     gmax = fmax(gmax, __shfl_down(gmax, o));
     hmin = fmin(hmin, __shfl_down(hmin, o));
   }
-  __shared__ double red2[EIG_NT / 64], red3[EIG_NT / 64];
-  if ((t & 63) == 0) { red[t >> 6] = worst; red2[t >> 6] = gmax; red3[t >> 6] = hmin; }
+  if ((t & 63) == 0) { red[t >> 6] = gmax; red2[t >> 6] = hmin; }
   __syncthreads();
   if (t == 0) {
-    double w = 0.0, g = 0.0, hm = HUGE_VAL;
-    for (int q = 0; q < EIG_NT / 64; ++q) { w = fmax(w, red[q]); g = fmax(g, red2[q]); hm = fmin(hm, red3[q]); }
-    bad = !(w <= thresh) || !(g < (1.0 - 1e-3) * hm);
-    *flag = bad;
+    double g = 0.0, hm = HUGE_VAL;
+    for (int w = 0; w < EIG_NT / 64; ++w) { g = fmax(g, red[w]); hm = fmin(hm, red2[w]); }
+    bad_s = !(g < (1.0 - 1e-3) * hm);
+    if (bad_s) *flag = 1;
   }
   __syncthreads();
-  if (bad) return;
-  auto E = [&](int i, int j) {
-    return (i >= nh && j >= nh) ? 0.0 : 0.5 * (H[i + (long long)ldh * j] + H[j + (long long)ldh * i]);
+  if (bad_s) return;
+
+  // ---- 3. second-order eigenpairs of the rotated matrix, rotated back ----
+  auto E = [&](int i, int j) -> double {
+    if (i >= nh && j >= nh) return 0.0;
+    const double e = a[pku(i, j)];
+    return fabs(e) <= floor_abs ? 0.0 : e;   // a nonzero E has |E| <= thresh |d_i - d_j|: the gap is > 0
   };
   for (int j = t; j < m; j += EIG_NT) {
     double acc = 0.0, n2 = 1.0;
@@ -492,24 +702,29 @@ __global__ __launch_bounds__(EIG_NT) void ritz_perturb_kernel(const double* __re
     evals[r] = th[j];
   }
   __syncthreads();
-  for (int e = t; e < m * m; e += EIG_NT) {
-    const int i = e % m, j = e / m;
-    double v;
-    if (i == j) {
-      v = 1.0;
-    } else {
-      const double eij = E(i, j);
-      v = (eij == 0.0) ? 0.0 : eij / (dg[j] - dg[i]);
+  // one thread per output column: it writes the column and turns it back through the rotations
+  // (last first: H's eigenvectors are J_1 ... J_n v), touching only its own column
+  for (int j = t; j < m; j += EIG_NT) {
+    double* v = V + (long long)m * rk[j];
+    for (int i = 0; i < m; ++i) {
+      const double eij = (i == j) ? 0.0 : E(i, j);
+      v[i] = (i == j ? 1.0 : (eij == 0.0 ? 0.0 : eij / (dg[j] - dg[i]))) * nr[j];
     }
-    V[i + (long long)m * rk[j]] = v * nr[j];
+    for (int u = nrot - 1; u >= 0; --u) {
+      const int p = rpq[u] & 0xffff, q = rpq[u] >> 16;
+      const double c = rc[u], sn = rsn[u];
+      const double xp = v[p], xq = v[q];
+      v[p] = c * xp + sn * xq;
+      v[q] = -sn * xp + c * xq;
+    }
   }
+  if (t == 0) *flag = 0;
 }
 }  // namespace
 
-static size_t jacobi_lds(int m) {
-  int mm = m + (m & 1);
-  int P = mm / 2;
-  return (size_t)m * (m + 1) / 2 * sizeof(double) + 2 * P * sizeof(double) + 2 * P * sizeof(int);
+static size_t jacobi_lds(int m) {   // packed mm x mm triangle
+  const int mm = m + (m & 1);
+  return (size_t)(mm * (mm + 1) / 2) * sizeof(double);
 }
 
 size_t jacobi_work_doubles(int m, int max_sweeps) {
@@ -534,17 +749,20 @@ void jacobi_eig_ws(const double* H, int ldh, int m, double* evals, double* V, do
   hipLaunchKernelGGL(jacobi_rot_kernel, dim3(1), dim3(EIG_NT), lds, s, H, ldh, m, rot, max_sweeps, diag, rounds,
                      sweeps_out);
   TNAD_HIP_TRY(hipGetLastError());
-  const size_t lds2 = ((size_t)JROWS * m + (size_t)JCHUNK * P * 2) * sizeof(double);
-  hipLaunchKernelGGL(jacobi_apply_kernel, dim3((m + JROWS - 1) / JROWS), dim3(256), lds2, s, rot, rounds, m, Vu);
+  hipLaunchKernelGGL(jacobi_apply_kernel, dim3((m + JA_WAVES - 1) / JA_WAVES), dim3(64 * JA_WAVES), 0, s, rot,
+                     rounds, m, Vu);
   TNAD_HIP_TRY(hipGetLastError());
-  hipLaunchKernelGGL(jacobi_sort_kernel, dim3(1), dim3(EIG_NT), 0, s, diag, Vu, m, evals, V);
+  hipLaunchKernelGGL(jacobi_sort_kernel, dim3(m), dim3(JS_NT), 0, s, diag, Vu, m, evals, V);
   TNAD_HIP_TRY(hipGetLastError());
 }
 
-void ritz_perturb(const double* H, int ldh, int m, int nh, double thresh, double* evals, double* V, int* flag,
-                  hipStream_t s) {
-  if (m < 1 || m > JACOBI_MAX || nh < 1 || nh > m) throw Error(TNAD_ERR_ARG, "ritz_perturb: size out of range");
-  hipLaunchKernelGGL(ritz_perturb_kernel, dim3(1), dim3(EIG_NT), 0, s, H, ldh, m, nh, thresh, evals, V, flag);
+void ritz_select(const double* H, int ldh, int m, int nh, double thresh, double* evals, double* V, int* flag,
+                 hipStream_t s) {
+  if (m < 1 || m > JACOBI_MAX || nh < 1 || nh > m) throw Error(TNAD_ERR_ARG, "ritz_select: size out of range");
+  const size_t lds = (size_t)m * (m + 1) / 2 * sizeof(double);
+  TNAD_HIP_TRY(hipFuncSetAttribute((const void*)ritz_select_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)lds));
+  hipLaunchKernelGGL(ritz_select_kernel, dim3(1), dim3(EIG_NT), lds, s, H, ldh, m, nh, thresh, evals, V, flag);
   TNAD_HIP_TRY(hipGetLastError());
 }
 

@@ -87,12 +87,14 @@ size_t jacobi_work_doubles(int m, int max_sweeps);
 void jacobi_eig_ws(const double* H, int ldh, int m, double* evals, double* V, double* Vwork,
                    int max_sweeps, int* sweeps_out, hipStream_t s);
 
-// Eigenpairs of a nearly diagonal symmetric m x m H by second-order perturbation theory for its
-// first nh columns (pairs among the others are not rotated), sorted by |lambda| descending, when
-// max |H_ij| / |H_ii - H_jj| over pairs with min(i, j) < nh is <= thresh; otherwise writes
-// *flag = 1 (device int) and leaves evals / V untouched.
-void ritz_perturb(const double* H, int ldh, int m, int nh, double thresh, double* evals, double* V, int* flag,
-                  hipStream_t s);
+// Eigenpairs of a nearly diagonal symmetric m x m H for its first nh columns (pairs among the
+// others are not rotated), sorted by |lambda| descending: pairs with min(i, j) < nh coupled more
+// strongly than |H_ij| > thresh |H_ii - H_jj| are rotated away by Jacobi rotations (a few passes),
+// the rest follows by second-order perturbation theory.  Writes *flag = 0 (device int) on success;
+// *flag = 1 leaves evals / V untouched (too many strong pairs, or the further block could hide a
+// kept direction).
+void ritz_select(const double* H, int ldh, int m, int nh, double thresh, double* evals, double* V, int* flag,
+                 hipStream_t s);
 
 // Cholesky of the m x m SPD matrix G (+ shift on the diagonal), returns Linv^T
 // (upper triangular, column-major) so that X = Y * Rinv with Rinv = L^-T.
