@@ -64,20 +64,26 @@ __global__ void ritz_scale_kernel(const double* __restrict__ Xr, const double* _
 }
 
 // *out = sum_ij (G_ij - delta_ij)^2 (one workgroup; G is k x k)
-__global__ void gram_dev_kernel(const double* __restrict__ G, int k, double* __restrict__ out) {
-  __shared__ double red[NT / 64];
+// |G - I|_F^2 of a k x k matrix (k <= 256): one workgroup of GD_NT threads, thread (row i, column
+// phase jo) sums rows i over columns jo, jo + GD_NT / 256, ... (coalesced along rows)
+constexpr int GD_NT = 1024;
+__global__ __launch_bounds__(GD_NT) void gram_dev_kernel(const double* __restrict__ G, int k,
+                                                         double* __restrict__ out) {
+  __shared__ double red[GD_NT / 64];
+  const int t = threadIdx.x, i = t & 255, jo = t >> 8;
   double acc = 0.0;
-  for (int e = threadIdx.x; e < k * k; e += NT) {
-    const double v = G[e] - ((e % k) == (e / k) ? 1.0 : 0.0);
-    acc += v * v;
-  }
+  if (i < k)
+    for (int j = jo; j < k; j += GD_NT / 256) {
+      const double v = G[i + (size_t)k * j] - (i == j ? 1.0 : 0.0);
+      acc += v * v;
+    }
   for (int o = 32; o > 0; o >>= 1) acc += __shfl_down(acc, o);
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+  if ((t & 63) == 0) red[t >> 6] = acc;
   __syncthreads();
-  if (threadIdx.x == 0) {
-    double t = 0.0;
-    for (int w = 0; w < NT / 64; ++w) t += red[w];
-    *out = t;
+  if (t == 0) {
+    double s = 0.0;
+    for (int w = 0; w < GD_NT / 64; ++w) s += red[w];
+    *out = s;
   }
 }
 
@@ -300,7 +306,7 @@ static void orthonormalize(Ctx* c, double* Y, int n, int k, double* X, EigWork& 
   // iteration come in.  Everything else goes through shifted CholeskyQR3 below.
   {
     tc_launch(c, Prec::F64, gemm_desc('T', 'N', k, k, n, n, n, k), Y, Y, w.G.as<double>(), s);
-    hipLaunchKernelGGL(gram_dev_kernel, dim3(1), dim3(NT), 0, s, w.G.as<double>(), k, w.res.as<double>());
+    hipLaunchKernelGGL(gram_dev_kernel, dim3(1), dim3(GD_NT), 0, s, w.G.as<double>(), k, w.res.as<double>());
     TNAD_HIP_TRY(hipGetLastError());
     double dev2 = 0.0;
     TNAD_HIP_TRY(hipMemcpyAsync(&dev2, w.res.ptr, 8, hipMemcpyDeviceToHost, s));
@@ -327,10 +333,10 @@ static void orthonormalize(Ctx* c, double* Y, int n, int k, double* X, EigWork& 
   }
   const double u = 1.1102230246251565e-16;
   const double shift = 11.0 * ((double)n * k + (double)k * (k + 1)) * u * (double)k;
-  // a well-conditioned block takes two plain CholeskyQR passes (CholeskyQR2): when the Cholesky
-  // pivots of its Gram matrix (w.G, formed above) stay above 1e-4 (unit columns: the block's
-  // condition number is then around 1e4, far below u^-1/2 where CholeskyQR2 is orthogonal to
-  // working precision)
+  // a well-conditioned block takes plain CholeskyQR passes (one, or two: CholeskyQR2) when the
+  // Cholesky pivots of its Gram matrix (w.G, formed above) stay above 1e-4 (unit columns: the
+  // block's condition number is then around 1e4, far below u^-1/2 where CholeskyQR2 is orthogonal
+  // to working precision)
   {
     chol_inv(w.G.as<double>(), k, 0.0, w.Rinv.as<double>(), w.info.as<int>(), w.res.as<double>(), s);
     int info = 0;
@@ -344,6 +350,14 @@ static void orthonormalize(Ctx* c, double* Y, int n, int k, double* X, EigWork& 
     if (info == 0 && pmin > 1e-4) {
       tc_launch(c, Prec::F64, gemm_desc('N', 'N', n, k, k, n, k, n), Y, w.Rinv.as<double>(), X, s);
       tc_launch(c, Prec::F64, gemm_desc('T', 'N', k, k, n, n, n, k), X, X, w.G.as<double>(), s);
+      // one pass leaves an error of O(kappa^2 u): a nearly orthonormal block (the usual case once
+      // the iteration converges) is done after it
+      hipLaunchKernelGGL(gram_dev_kernel, dim3(1), dim3(GD_NT), 0, s, w.G.as<double>(), k, w.res.as<double>());
+      TNAD_HIP_TRY(hipGetLastError());
+      double dev2 = 0.0;
+      TNAD_HIP_TRY(hipMemcpyAsync(&dev2, w.res.ptr, 8, hipMemcpyDeviceToHost, s));
+      c->sync();
+      if (std::sqrt(dev2) <= 1e-12) return;
       chol_inv(w.G.as<double>(), k, 0.0, w.Rinv.as<double>(), w.info.as<int>(), nullptr, s);
       TNAD_HIP_TRY(hipMemcpyAsync(&info, w.info.ptr, sizeof(int), hipMemcpyDeviceToHost, s));
       c->sync();

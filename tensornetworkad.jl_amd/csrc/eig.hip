@@ -151,10 +151,13 @@ __global__ __launch_bounds__(EIG_NT) void jacobi_rot_kernel(const double* __rest
         double c = 1.0, sn = 0.0;
         const double aab = a[s_ab], aaa = a[s_aa], abb = a[s_bb];
         if (fabs(aab) > abs_floor && aab * aab > rel_tol2 * fabs(aaa * abb)) {
-          const double theta = (abb - aaa) / (2.0 * aab);
-          const double tt = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
-          c = 1.0 / sqrt(tt * tt + 1.0);
-          sn = tt * c;
+          // tan of the rotation angle (|angle| <= pi/4): tt = sign(d) e / (|d| + hypot(d, e)) with
+          // d = abb - aaa, e = 2 aab; then c = 1 / sqrt(1 + tt^2), s = tt c, formed without tt
+          const double d = abb - aaa, e = 2.0 * aab;
+          const double den = fabs(d) + sqrt(d * d + e * e);
+          const double inv = 1.0 / sqrt(den * den + e * e);
+          c = den * inv;
+          sn = (d >= 0.0 ? e : -e) * inv;
           nrot_s = 1;   // every writer stores the same value
         }
         rcs[t] = make_double2(c, sn);
