@@ -45,12 +45,32 @@ TcDesc gemm_desc(char ta, char tb, int M, int N, int K, long long lda, long long
 namespace {
 
 constexpr int BM = 64, BN = 64, NT = 256;
+constexpr int TC_BK = 16;   // K-tile of the FP32 / FP64 kernel (32 measured slower: occupancy)
 
 __device__ __forceinline__ long long off2(int i, int I0, long long s0, long long s1) {
   int i0 = i % I0;
   int i1 = i / I0;
   return (long long)i0 * s0 + (long long)i1 * s1;
 }
+
+// Position of one loader along the contraction index k = k0 + K0 * k1: (k0, offset), advanced by a
+// K-tile per step without dividing (k0 wraps, carrying into k1 via delta = s1 - K0 * s0).
+struct KPos {
+  int k0;
+  long long off;
+  __device__ __forceinline__ void init(int k, int K0, long long s0, long long s1) {
+    k0 = k % K0;
+    off = (long long)k0 * s0 + (long long)(k / K0) * s1;
+  }
+  __device__ __forceinline__ void advance(int step, int K0, long long s0, long long delta) {
+    k0 += step;
+    off += (long long)step * s0;
+    while (k0 >= K0) {
+      k0 -= K0;
+      off += delta;
+    }
+  }
+};
 
 template <typename T> struct MfmaOp;
 // Accumulator lane layout of the 16x16 output block (lane l, accumulator register r):
@@ -75,12 +95,13 @@ template <> struct MfmaOp<float> {
 // ---------------------------------------------------------------------------
 // F64 / F32 kernel (k-major LDS tiles, one element per lane per MFMA)
 // ---------------------------------------------------------------------------
-template <typename T>
+template <typename T, int BK>
 __global__ __launch_bounds__(NT) void tc_kernel(TcDesc d, const T* __restrict__ A,
                                                 const T* __restrict__ B, T* __restrict__ C,
                                                 T* __restrict__ Wsp) {
-  constexpr int BK = 16;
   constexpr int PAD = 16;
+  constexpr int NR = BK / 4;   // loads per thread and operand per K-tile (64 x BK over NT threads)
+  static_assert(NR * NT == 64 * BK, "loader coverage");
   __shared__ T As[BK][BM + PAD];
   __shared__ T Bs[BK][BN + PAD];
 
@@ -100,43 +121,53 @@ __global__ __launch_bounds__(NT) void tc_kernel(TcDesc d, const T* __restrict__ 
 
   // A loader mapping
   const bool a_mfast = (d.am0 == 1 && d.M0 > 1) || d.ak0 != 1;
-  int a_mi[4], a_ki[4];
-  long long a_moff[4];
-  bool a_mok[4];
+  int a_mi[NR], a_ki[NR];
+  long long a_moff[NR];
+  bool a_mok[NR];
 #pragma unroll
-  for (int r = 0; r < 4; ++r) {
+  for (int r = 0; r < NR; ++r) {
     if (a_mfast) { a_mi[r] = t & 63; a_ki[r] = (t >> 6) + 4 * r; }
-    else         { a_ki[r] = t & 15; a_mi[r] = (t >> 4) + 16 * r; }
+    else         { a_ki[r] = t % BK; a_mi[r] = t / BK + (NT / BK) * r; }
     int m = tile_m + a_mi[r];
     a_mok[r] = m < M;
     a_moff[r] = a_mok[r] ? off2(m, d.M0, d.am0, d.am1) : 0;
   }
   // B loader mapping
   const bool b_nfast = (d.bn0 == 1 && d.N0 > 1) || d.bk0 != 1;
-  int b_ni[4], b_ki[4];
-  long long b_noff[4];
-  bool b_nok[4];
+  int b_ni[NR], b_ki[NR];
+  long long b_noff[NR];
+  bool b_nok[NR];
 #pragma unroll
-  for (int r = 0; r < 4; ++r) {
+  for (int r = 0; r < NR; ++r) {
     if (b_nfast) { b_ni[r] = t & 63; b_ki[r] = (t >> 6) + 4 * r; }
-    else         { b_ki[r] = t & 15; b_ni[r] = (t >> 4) + 16 * r; }
+    else         { b_ki[r] = t % BK; b_ni[r] = t / BK + (NT / BK) * r; }
     int n = tile_n + b_ni[r];
     b_nok[r] = n < N;
     b_noff[r] = b_nok[r] ? off2(n, d.N0, d.bn0, d.bn1) : 0;
   }
 
-  T ra[4], rb[4];
+  T ra[NR], rb[NR];
+  KPos a_kp[NR], b_kp[NR];
+  const long long a_dk = d.ak1 - (long long)d.K0 * d.ak0, b_dk = d.bk1 - (long long)d.K0 * d.bk0;
+#pragma unroll
+  for (int r = 0; r < NR; ++r) {
+    a_kp[r].init(kbeg + a_ki[r], d.K0, d.ak0, d.ak1);
+    b_kp[r].init(kbeg + b_ki[r], d.K0, d.bk0, d.bk1);
+  }
+  // loads the K-tile starting at k0, then moves the loaders one tile on
   auto gload = [&](int k0) {
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
+    for (int r = 0; r < NR; ++r) {
       int k = k0 + a_ki[r];
       T v = T(0);
-      if (a_mok[r] && k < K) v = A[a_moff[r] + off2(k, d.K0, d.ak0, d.ak1)];
+      if (a_mok[r] && k < K) v = A[a_moff[r] + a_kp[r].off];
       ra[r] = v;
       int kb = k0 + b_ki[r];
       T w = T(0);
-      if (b_nok[r] && kb < K) w = B[b_noff[r] + off2(kb, d.K0, d.bk0, d.bk1)];
+      if (b_nok[r] && kb < K) w = B[b_noff[r] + b_kp[r].off];
       rb[r] = w;
+      a_kp[r].advance(BK, d.K0, d.ak0, a_dk);
+      b_kp[r].advance(BK, d.K0, d.bk0, b_dk);
     }
   };
 
@@ -151,7 +182,7 @@ __global__ __launch_bounds__(NT) void tc_kernel(TcDesc d, const T* __restrict__ 
   for (int k0 = kbeg; k0 < K; k0 += BK) {
     __syncthreads();
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
+    for (int r = 0; r < NR; ++r) {
       As[a_ki[r]][a_mi[r]] = ra[r];
       Bs[b_ki[r]][b_ni[r]] = rb[r];
     }
@@ -275,17 +306,27 @@ __global__ __launch_bounds__(NT) void tc_kernel_bf16(TcDesc d, const float* __re
     b_noff[r] = b_nok[r] ? off2(n, d.N0, d.bn0, d.bn1) : 0;
   }
   float ra[8], rb[8];
+  KPos a_kp[8], b_kp[8];
+  const long long a_dk = d.ak1 - (long long)d.K0 * d.ak0, b_dk = d.bk1 - (long long)d.K0 * d.bk0;
+#pragma unroll
+  for (int r = 0; r < 8; ++r) {
+    a_kp[r].init(kbeg + a_ki[r], d.K0, d.ak0, d.ak1);
+    b_kp[r].init(kbeg + b_ki[r], d.K0, d.bk0, d.bk1);
+  }
+  // loads the K-tile starting at k0, then moves the loaders one tile on
   auto gload = [&](int k0) {
 #pragma unroll
     for (int r = 0; r < 8; ++r) {
       int k = k0 + a_ki[r];
       float v = 0.f;
-      if (a_mok[r] && k < K) v = A[a_moff[r] + off2(k, d.K0, d.ak0, d.ak1)];
+      if (a_mok[r] && k < K) v = A[a_moff[r] + a_kp[r].off];
       ra[r] = v;
       int kb = k0 + b_ki[r];
       float w = 0.f;
-      if (b_nok[r] && kb < K) w = B[b_noff[r] + off2(kb, d.K0, d.bk0, d.bk1)];
+      if (b_nok[r] && kb < K) w = B[b_noff[r] + b_kp[r].off];
       rb[r] = w;
+      a_kp[r].advance(BK, d.K0, d.ak0, a_dk);
+      b_kp[r].advance(BK, d.K0, d.bk0, b_dk);
     }
   };
 
@@ -395,11 +436,11 @@ void tc_launch(Ctx* c, Prec p, const TcDesc& d, const void* A, const void* B, vo
   void* W = nsplit > 1 ? wsp.ptr : nullptr;
   switch (p) {
     case Prec::F64:
-      hipLaunchKernelGGL(tc_kernel<double>, grid, dim3(NT), 0, s, dd, (const double*)A,
+      hipLaunchKernelGGL((tc_kernel<double, TC_BK>), grid, dim3(NT), 0, s, dd, (const double*)A,
                          (const double*)B, (double*)C, (double*)W);
       break;
     case Prec::F32:
-      hipLaunchKernelGGL(tc_kernel<float>, grid, dim3(NT), 0, s, dd, (const float*)A,
+      hipLaunchKernelGGL((tc_kernel<float, TC_BK>), grid, dim3(NT), 0, s, dd, (const float*)A,
                          (const float*)B, (float*)C, (float*)W);
       break;
     case Prec::BF16:
