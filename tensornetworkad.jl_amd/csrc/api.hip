@@ -70,7 +70,54 @@ void Ctx::trim() {
 Ctx::~Ctx() {
   if (stream) (void)hipStreamSynchronize(stream);
   for (auto& kv : free_blocks) (void)hipFree(kv.second);
+  if (pinned_) (void)hipHostFree(pinned_);
   if (stream) (void)hipStreamDestroy(stream);
+}
+
+// ---------------------------------------------------------------------------
+// pinned staging of small host <-> device transfers
+// ---------------------------------------------------------------------------
+constexpr size_t kPinnedBytes = size_t(1) << 20;
+char* Ctx::stage(size_t bytes, size_t& off) {
+  if (bytes > kPinnedBytes / 2) return nullptr;
+  if (!pinned_) {
+    void* p = nullptr;
+    TNAD_HIP_TRY(hipHostMalloc(&p, kPinnedBytes, hipHostMallocDefault));
+    pinned_ = static_cast<char*>(p);
+  }
+  size_t need = (bytes + 15) & ~size_t(15);
+  if (pinned_used_ + need > kPinnedBytes) sync();   // drains the queued transfers, frees the area
+  off = pinned_used_;
+  pinned_used_ += need;
+  return pinned_ + off;
+}
+void Ctx::read_async(void* dst_host, const void* src_dev, size_t bytes) {
+  if (bytes == 0) return;
+  size_t off = 0;
+  char* st = stage(bytes, off);
+  if (!st) {   // large: straight into the destination (complete at the next sync as well)
+    TNAD_HIP_TRY(hipMemcpyAsync(dst_host, src_dev, bytes, hipMemcpyDeviceToHost, stream));
+    return;
+  }
+  TNAD_HIP_TRY(hipMemcpyAsync(st, src_dev, bytes, hipMemcpyDeviceToHost, stream));
+  pending_.push_back({dst_host, off, bytes});
+}
+void Ctx::write_async(void* dst_dev, const void* src_host, size_t bytes) {
+  if (bytes == 0) return;
+  size_t off = 0;
+  char* st = stage(bytes, off);
+  if (!st) {
+    TNAD_HIP_TRY(hipMemcpyAsync(dst_dev, src_host, bytes, hipMemcpyHostToDevice, stream));
+    return;
+  }
+  std::memcpy(st, src_host, bytes);
+  TNAD_HIP_TRY(hipMemcpyAsync(dst_dev, st, bytes, hipMemcpyHostToDevice, stream));
+}
+void Ctx::sync() {
+  TNAD_HIP_TRY(hipStreamSynchronize(stream));
+  for (const auto& r : pending_) std::memcpy(r.dst, pinned_ + r.off, r.bytes);
+  pending_.clear();
+  pinned_used_ = 0;
 }
 
 DevBuf::DevBuf(Ctx* c, size_t b) : ptr(c->alloc(b)), bytes(b), owner(c) {}

@@ -55,7 +55,7 @@ struct EigBackStats {
 };
 
 struct EigWork {
-  DevBuf Y, Xr, Yr, H, W, Vw, G, Rinv, nrm, res, info, sweeps;
+  DevBuf Y, Xr, Yr, H, W, Vw, G, Rinv, nrm, res, info, sweeps, scal;
   EigWork(Ctx* c, int n, int k);
 };
 

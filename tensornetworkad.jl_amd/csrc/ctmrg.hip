@@ -87,13 +87,6 @@ __global__ __launch_bounds__(GD_NT) void gram_dev_kernel(const double* __restric
   }
 }
 
-// T = (3 I - G) / 2  (one Newton-Schulz step towards G^-1/2)
-__global__ void ns_step_kernel(const double* __restrict__ G, int k, double* __restrict__ T) {
-  int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= k * k) return;
-  T[e] = 0.5 * (((e % k) == (e / k) ? 3.0 : 0.0) - G[e]);
-}
-
 // make the largest-|.| entry of every column positive (first occurrence wins);
 // the same sign is applied to the companion block Y (A*X) when given.
 __global__ void canon_sign_kernel(double* __restrict__ X, double* __restrict__ Y, int n) {
@@ -255,7 +248,7 @@ static void debug_dump(Ctx* c, const char* name, const double* dev, size_t n) {
   const char* dir = std::getenv("TNAD_DEBUG_DUMP");
   if (!dir || !*dir) return;
   std::vector<double> h(n);
-  TNAD_HIP_TRY(hipMemcpyAsync(h.data(), dev, n * 8, hipMemcpyDeviceToHost, c->stream));
+  c->read_async(h.data(), dev, n * 8);
   c->sync();
   std::string path = std::string(dir) + "/" + name + ".bin";
   if (FILE* f = std::fopen(path.c_str(), "wb")) {
@@ -272,7 +265,7 @@ static void dbg_finite(Ctx* c, const char* what, const double* dev, size_t n) {
   }();
   if (!on) return;
   std::vector<double> h(n);
-  TNAD_HIP_TRY(hipMemcpyAsync(h.data(), dev, n * 8, hipMemcpyDeviceToHost, c->stream));
+  c->read_async(h.data(), dev, n * 8);
   c->sync();
   size_t bad = 0;
   double mx = 0.0;
@@ -293,73 +286,38 @@ static void orthonormalize(Ctx* c, double* Y, int n, int k, double* X, EigWork& 
   k_random_fill_cols(Y, n, k, w.nrm.as<double>(), 1e-13, seed, s);
   k_col_norms(Y, n, k, w.nrm.as<double>(), s);
   k_scale_cols(Y, n, k, w.nrm.as<double>(), 1, s);
-  // otherwise CholeskyQR2 when the block is well conditioned, else shifted CholeskyQR followed by two
-  // plain CholeskyQR passes (sCholQR3).  Columns that the
-  // shifted pass finds linearly dependent on earlier ones (pivot at the shift's level: the
-  // block has lower rank than k, e.g. a rank-deficient corner matrix) are replaced by random
-  // directions first, so the block always spans k dimensions.
+  // A well-conditioned block takes plain CholeskyQR passes: one when the result is orthonormal to
+  // working precision (|X^T X - I|_F <= 1e-12, the usual case once the iteration converges), else a
+  // second (CholeskyQR2).  That holds when the Cholesky pivots of the Gram matrix stay above 1e-4 (unit
+  // columns: condition number around 1e4, far below u^-1/2).  The first pass and the test run before
+  // the host looks at the pivots, so that one round trip serves both.  Otherwise: shifted CholeskyQR
+  // followed by two plain passes (sCholQR3); columns that the shifted pass finds linearly dependent
+  // on earlier ones (pivot at the shift's level: the block has lower rank than k, e.g. a
+  // rank-deficient corner matrix) are replaced by random directions first, so the block always spans
+  // k dimensions.
   std::vector<double> piv(k);
-  // fast path: a block whose unit columns are already nearly orthonormal, |Y^T Y - I|_F <= 1/2 (so
-  // every singular value lies in [0.7, 1.3]), is brought to orthonormality by Newton-Schulz steps
-  // Y <- Y (3 I - Y^T Y) / 2 (symmetric orthogonalisation; the error e falls as 3 e^2 / 4), which
-  // are matrix-core products only -- the usual case once the Ritz-scaled blocks of a converging
-  // iteration come in.  Everything else goes through shifted CholeskyQR3 below.
-  {
-    tc_launch(c, Prec::F64, gemm_desc('T', 'N', k, k, n, n, n, k), Y, Y, w.G.as<double>(), s);
-    hipLaunchKernelGGL(gram_dev_kernel, dim3(1), dim3(GD_NT), 0, s, w.G.as<double>(), k, w.res.as<double>());
-    TNAD_HIP_TRY(hipGetLastError());
-    double dev2 = 0.0;
-    TNAD_HIP_TRY(hipMemcpyAsync(&dev2, w.res.ptr, 8, hipMemcpyDeviceToHost, s));
-    c->sync();
-    const double e0 = std::sqrt(dev2);
-    if (std::getenv("TNAD_DEBUG_EIG"))
-      std::fprintf(stderr, "[tnad eig] orthonormalize n=%d k=%d |Y^T Y - I|_F=%.3e\n", n, k, e0);
-    if (e0 <= 0.5) {
-      int passes = 0;
-      for (double e = e0; e > 1e-17 && passes < 8; e = 0.75 * e * e + 1e-18) ++passes;
-      double* src = Y;
-      double* dst = X;
-      for (int p = 0; p < passes; ++p) {
-        if (p > 0) tc_launch(c, Prec::F64, gemm_desc('T', 'N', k, k, n, n, n, k), src, src, w.G.as<double>(), s);
-        hipLaunchKernelGGL(ns_step_kernel, dim3((k * k + NT - 1) / NT), dim3(NT), 0, s, w.G.as<double>(), k,
-                           w.Rinv.as<double>());
-        TNAD_HIP_TRY(hipGetLastError());
-        tc_launch(c, Prec::F64, gemm_desc('N', 'N', n, k, k, n, k, n), src, w.Rinv.as<double>(), dst, s);
-        std::swap(src, dst);
-      }
-      if (src != X) k_copy(src, X, (size_t)n * k, s);
-      return;
-    }
-  }
   const double u = 1.1102230246251565e-16;
   const double shift = 11.0 * ((double)n * k + (double)k * (k + 1)) * u * (double)k;
-  // a well-conditioned block takes plain CholeskyQR passes (one, or two: CholeskyQR2) when the
-  // Cholesky pivots of its Gram matrix (w.G, formed above) stay above 1e-4 (unit columns: the
-  // block's condition number is then around 1e4, far below u^-1/2 where CholeskyQR2 is orthogonal
-  // to working precision)
   {
-    chol_inv(w.G.as<double>(), k, 0.0, w.Rinv.as<double>(), w.info.as<int>(), w.res.as<double>(), s);
+    double* scal = w.scal.as<double>();   // [0] smallest pivot, [1] |X^T X - I|_F^2
+    tc_launch(c, Prec::F64, gemm_desc('T', 'N', k, k, n, n, n, k), Y, Y, w.G.as<double>(), s);
+    chol_inv(w.G.as<double>(), k, 0.0, w.Rinv.as<double>(), w.info.as<int>(), nullptr, s, scal);
+    tc_launch(c, Prec::F64, gemm_desc('N', 'N', n, k, k, n, k, n), Y, w.Rinv.as<double>(), X, s);
+    tc_launch(c, Prec::F64, gemm_desc('T', 'N', k, k, n, n, n, k), X, X, w.G.as<double>(), s);
+    hipLaunchKernelGGL(gram_dev_kernel, dim3(1), dim3(GD_NT), 0, s, w.G.as<double>(), k, scal + 1);
+    TNAD_HIP_TRY(hipGetLastError());
     int info = 0;
-    TNAD_HIP_TRY(hipMemcpyAsync(&info, w.info.ptr, sizeof(int), hipMemcpyDeviceToHost, s));
-    TNAD_HIP_TRY(hipMemcpyAsync(piv.data(), w.res.ptr, (size_t)k * 8, hipMemcpyDeviceToHost, s));
+    double hs[2] = {0.0, 0.0};
+    c->read_async(&info, w.info.ptr, sizeof(int));
+    c->read_async(hs, scal, 2 * sizeof(double));
     c->sync();
-    double pmin = HUGE_VAL;
-    for (int j = 0; j < k; ++j) pmin = std::min(pmin, piv[j]);
     if (std::getenv("TNAD_DEBUG_EIG"))
-      std::fprintf(stderr, "[tnad eig] plain Cholesky info=%d min pivot=%.3e\n", info, pmin);
-    if (info == 0 && pmin > 1e-4) {
-      tc_launch(c, Prec::F64, gemm_desc('N', 'N', n, k, k, n, k, n), Y, w.Rinv.as<double>(), X, s);
-      tc_launch(c, Prec::F64, gemm_desc('T', 'N', k, k, n, n, n, k), X, X, w.G.as<double>(), s);
-      // one pass leaves an error of O(kappa^2 u): a nearly orthonormal block (the usual case once
-      // the iteration converges) is done after it
-      hipLaunchKernelGGL(gram_dev_kernel, dim3(1), dim3(GD_NT), 0, s, w.G.as<double>(), k, w.res.as<double>());
-      TNAD_HIP_TRY(hipGetLastError());
-      double dev2 = 0.0;
-      TNAD_HIP_TRY(hipMemcpyAsync(&dev2, w.res.ptr, 8, hipMemcpyDeviceToHost, s));
-      c->sync();
-      if (std::sqrt(dev2) <= 1e-12) return;
+      std::fprintf(stderr, "[tnad eig] plain Cholesky info=%d min pivot=%.3e |X^T X - I|_F=%.3e\n", info, hs[0],
+                   std::sqrt(hs[1]));
+    if (info == 0 && hs[0] > 1e-4) {
+      if (std::sqrt(hs[1]) <= 1e-12) return;
       chol_inv(w.G.as<double>(), k, 0.0, w.Rinv.as<double>(), w.info.as<int>(), nullptr, s);
-      TNAD_HIP_TRY(hipMemcpyAsync(&info, w.info.ptr, sizeof(int), hipMemcpyDeviceToHost, s));
+      c->read_async(&info, w.info.ptr, sizeof(int));
       c->sync();
       if (info != 0)   // breakdown (pivot <= 0): factor the shifted Gram instead
         chol_inv(w.G.as<double>(), k, shift, w.Rinv.as<double>(), w.info.as<int>(), nullptr, s);
@@ -374,7 +332,7 @@ static void orthonormalize(Ctx* c, double* Y, int n, int k, double* X, EigWork& 
   for (int attempt = 0;; ++attempt) {
     tc_launch(c, Prec::F64, gemm_desc('T', 'N', k, k, n, n, n, k), Y, Y, w.G.as<double>(), s);
     chol_inv(w.G.as<double>(), k, shift, w.Rinv.as<double>(), w.info.as<int>(), w.res.as<double>(), s);
-    TNAD_HIP_TRY(hipMemcpyAsync(piv.data(), w.res.ptr, (size_t)k * 8, hipMemcpyDeviceToHost, s));
+    c->read_async(piv.data(), w.res.ptr, (size_t)k * 8);
     c->sync();
     // a column with less than 1% of its (unit) norm outside the span of the earlier ones is
     // replaced: it adds next to no new direction, and keeping it would leave the block so
@@ -388,7 +346,7 @@ static void orthonormalize(Ctx* c, double* Y, int n, int k, double* X, EigWork& 
     if (std::getenv("TNAD_DEBUG_EIG"))
       std::fprintf(stderr, "[tnad eig] shifted CholeskyQR attempt %d: %d dependent columns\n", attempt, ndead);
     if (ndead == 0 || attempt == 3) break;
-    TNAD_HIP_TRY(hipMemcpyAsync(w.info.ptr, dead.data(), (size_t)k * sizeof(int), hipMemcpyHostToDevice, s));
+    c->write_async(w.info.ptr, dead.data(), (size_t)k * sizeof(int));
     k_random_fill_flagged(Y, n, k, w.info.as<int>(), seed + 7919u * (attempt + 1), s);
     k_col_norms(Y, n, k, w.nrm.as<double>(), s);
     k_scale_cols(Y, n, k, w.nrm.as<double>(), 1, s);
@@ -401,7 +359,7 @@ static void orthonormalize(Ctx* c, double* Y, int n, int k, double* X, EigWork& 
     tc_launch(c, Prec::F64, gemm_desc('T', 'N', k, k, n, n, n, k), src, src, w.G.as<double>(), s);
     chol_inv(w.G.as<double>(), k, 0.0, w.Rinv.as<double>(), w.info.as<int>(), nullptr, s);
     int info = 0;
-    TNAD_HIP_TRY(hipMemcpyAsync(&info, w.info.ptr, sizeof(int), hipMemcpyDeviceToHost, s));
+    c->read_async(&info, w.info.ptr, sizeof(int));
     c->sync();
     if (info != 0)   // breakdown (pivot <= 0): factor the shifted Gram instead
       chol_inv(w.G.as<double>(), k, shift, w.Rinv.as<double>(), w.info.as<int>(), nullptr, s);
@@ -415,7 +373,8 @@ static void orthonormalize(Ctx* c, double* Y, int n, int k, double* X, EigWork& 
 EigWork::EigWork(Ctx* c, int n, int k)
     : Y(c, (size_t)n * k * 8), Xr(c, (size_t)n * k * 8), Yr(c, (size_t)n * k * 8), H(c, (size_t)k * k * 8),
       W(c, (size_t)k * k * 8), Vw(c, jacobi_work_doubles(k, 60) * 8), G(c, (size_t)k * k * 8), Rinv(c, (size_t)k * k * 8),
-      nrm(c, (size_t)k * 8), res(c, (size_t)k * 8), info(c, (size_t)k * sizeof(int) + 16), sweeps(c, 16) {}
+      nrm(c, (size_t)k * 8), res(c, (size_t)k * 8), info(c, (size_t)k * sizeof(int) + 16), sweeps(c, 16),
+      scal(c, 16 * 8) {}
 
 int topk_eig(Ctx* c, const double* A, int n, int want, int k, const double* X0, int maxit, double tol,
              double* X, double* AX, double* theta, EigStats* st) {
@@ -457,14 +416,10 @@ int topk_eig(Ctx* c, const double* A, int n, int want, int k, const double* X0, 
     // previous Ritz basis) a few rotations of near-degenerate pairs plus second-order perturbation
     // theory replace the Jacobi sweeps; Jacobi on the first cold iteration, whenever H is not close
     // enough to diagonal, and whenever the residual stopped falling (by 10% per iteration)
-    bool jac = force_jacobi || (it == 1 && !X0);
-    if (!jac) {
-      ritz_select(w.H.as<double>(), k, k, want, 1e-3, theta, w.W.as<double>(), w.info.as<int>(), s);
-      int flag = 1;
-      TNAD_HIP_TRY(hipMemcpyAsync(&flag, w.info.ptr, sizeof(int), hipMemcpyDeviceToHost, s));
-      c->sync();
-      jac = flag != 0;
-    }
+    // (ritz_select's flag gates the Jacobi kernels on the device: no host round trip)
+    const bool jac_forced = force_jacobi || (it == 1 && !X0);
+    int* gate = w.info.as<int>();
+    if (!jac_forced) ritz_select(w.H.as<double>(), k, k, want, 1e-3, theta, w.W.as<double>(), gate, s);
     // far from convergence the Rayleigh-Ritz step only has to improve the basis (the span, which
     // is what converges, does not depend on it): two Jacobi sweeps do.  Closer in, three: Jacobi
     // converges quadratically on a nearly diagonal H, and what is left is taken up by the next
@@ -472,14 +427,8 @@ int topk_eig(Ctx* c, const double* A, int n, int want, int k, const double* X0, 
     // the residual stopped falling or the iteration runs long.
     const bool rough = it == 1 || prev_rel > 1e-6;
     const int sweeps = (force_jacobi || it > 8) ? 60 : (rough ? 2 : 3);
-    if (jac) jacobi_eig_ws(w.H.as<double>(), k, k, theta, w.W.as<double>(), w.Vw.as<double>(), sweeps,
-                           dbg ? w.sweeps.as<int>() : nullptr, s);
-    n_jacobi += jac;
-    int sweeps_done = 0;
-    if (dbg && jac) {
-      TNAD_HIP_TRY(hipMemcpyAsync(&sweeps_done, w.sweeps.ptr, sizeof(int), hipMemcpyDeviceToHost, s));
-      c->sync();
-    }
+    jacobi_eig_ws(w.H.as<double>(), k, k, theta, w.W.as<double>(), w.Vw.as<double>(), sweeps,
+                  dbg ? w.sweeps.as<int>() : nullptr, s, jac_forced ? nullptr : gate);
     dbg_finite(c, "W", w.W.as<double>(), (size_t)k * k);
     dbg_finite(c, "theta", theta, (size_t)k);
     tc_launch(c, Prec::F64, gemm_desc('N', 'N', n, k, k, n, k, n), X, w.W.as<double>(), w.Xr.as<double>(), s);
@@ -488,14 +437,19 @@ int topk_eig(Ctx* c, const double* A, int n, int want, int k, const double* X0, 
     hipLaunchKernelGGL(resid_norms_kernel, dim3(want), dim3(NT), 0, s, w.Xr.as<double>(), w.Yr.as<double>(),
                        theta, n, w.res.as<double>());
     TNAD_HIP_TRY(hipGetLastError());
-    TNAD_HIP_TRY(hipMemcpyAsync(hres.data(), w.res.ptr, want * 8, hipMemcpyDeviceToHost, s));
-    TNAD_HIP_TRY(hipMemcpyAsync(hth.data(), theta, 8, hipMemcpyDeviceToHost, s));
+    c->read_async(hres.data(), w.res.ptr, want * 8);
+    c->read_async(hth.data(), theta, 8);
+    int flag = 1, sweeps_done = 0;
+    if (!jac_forced) c->read_async(&flag, gate, sizeof(int));
+    if (dbg) c->read_async(&sweeps_done, w.sweeps.ptr, sizeof(int));
     c->sync();
+    const bool jac = flag != 0;   // Jacobi ran (forced, or the selective Rayleigh-Ritz declined)
+    n_jacobi += jac;
     double mx = 0.0;
     for (int j = 0; j < want; ++j) mx = std::max(mx, hres[j]);
     rel = mx / std::max(std::fabs(hth[0]), 1e-300);
     if (dbg) std::fprintf(stderr, "[tnad eig] n=%d k=%d want=%d it=%d rel_resid=%.3e theta0=%.6e rr=%s sweeps=%d\n",
-                          n, k, want, it, rel, hth[0], jac ? "jacobi" : "select", sweeps_done);
+                          n, k, want, it, rel, hth[0], jac ? "jacobi" : "select", jac ? sweeps_done : 0);
     force_jacobi = !(rel < 0.9 * prev_rel);
     prev_rel = rel;
     if (!(rel == rel)) throw Error(TNAD_ERR_NAN, "eigensolver: NaN in Ritz residual");
@@ -629,7 +583,7 @@ void ctmrg_step(Ctx* c, const StepDims& d, const StepOpts& o, const double* a64,
   // kept part carries 98-99% of the norm of the change of the full vector near convergence.
   const int nv = (n <= JACOBI_MAX) ? k : std::min(k, chi);
   std::vector<double> th(nv);
-  TNAD_HIP_TRY(hipMemcpyAsync(th.data(), theta, (size_t)nv * 8, hipMemcpyDeviceToHost, s));
+  c->read_async(th.data(), theta, (size_t)nv * 8);
   c->sync();
   vals.assign(n, 0.0);
   double s0 = std::fabs(th[0]);
@@ -861,7 +815,7 @@ void expectation_fwd(Ctx* c, int chi, int D, int ph, const double* h, const doub
                      ev.scal.as<double>() + 1, ev.scal.as<double>() + 5);
   TNAD_HIP_TRY(hipGetLastError());
   if (e_host) {
-    TNAD_HIP_TRY(hipMemcpyAsync(e_host, ev.scal.as<double>() + 5, 8, hipMemcpyDeviceToHost, s));
+    c->read_async(e_host, ev.scal.as<double>() + 5, 8);
     c->sync();
   }
 }
@@ -888,7 +842,7 @@ void expectation_back(Ctx* c, int chi, int D, int ph, const double* h, const dou
     // 2 * nnbar = -2 e / nn^2
     DevBuf cf(c, 8);
     std::vector<double> sc(8);
-    TNAD_HIP_TRY(hipMemcpyAsync(sc.data(), ev.scal.ptr, 8 * 8, hipMemcpyDeviceToHost, s));
+    c->read_async(sc.data(), ev.scal.ptr, 8 * 8);
     c->sync();
     double coef = 2.0 * sc[7];
     hipLaunchKernelGGL(add_trace_diag_kernel, dim3((m3 * ph + NT - 1) / NT), dim3(NT), 0, s, lbar.as<double>(),

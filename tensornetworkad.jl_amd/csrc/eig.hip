@@ -78,7 +78,8 @@ __device__ __forceinline__ int jpos_index0(int pos, int M) {   // index at a pos
 __global__ __launch_bounds__(EIG_NT) void jacobi_rot_kernel(const double* __restrict__ H, int ldh, int m,
                                                             double* __restrict__ rot, int max_sweeps,
                                                             double* __restrict__ diag, int* rounds_out,
-                                                            int* sweeps_out) {
+                                                            int* sweeps_out, const int* __restrict__ gate) {
+  if (gate && *gate == 0) return;   // the caller's result stands (see jacobi_eig_ws)
   extern __shared__ double smem[];
   const int mm = m + (m & 1);
   const int P = mm / 2;
@@ -231,7 +232,9 @@ constexpr int JA_WAVES = 4;   // rows (wavefronts) per workgroup
 constexpr int JCHUNK = 16;    // rounds of rotation parameters staged in LDS at once
 __global__ __launch_bounds__(64 * JA_WAVES) void jacobi_apply_kernel(const double* __restrict__ rot,
                                                                     const int* __restrict__ rounds_ptr, int m,
-                                                                    double* __restrict__ Vrows) {
+                                                                    double* __restrict__ Vrows,
+                                                                    const int* __restrict__ gate) {
+  if (gate && *gate == 0) return;
   __shared__ double rows[JA_WAVES][JACOBI_MAX];
   __shared__ double2 prm[JCHUNK * (JACOBI_MAX / 2)];
   const int mm = m + (m & 1);
@@ -286,7 +289,9 @@ __global__ __launch_bounds__(64 * JA_WAVES) void jacobi_apply_kernel(const doubl
 constexpr int JS_NT = 256;
 __global__ __launch_bounds__(JS_NT) void jacobi_sort_kernel(const double* __restrict__ diag,
                                                             const double* __restrict__ Vu, int m,
-                                                            double* __restrict__ evals, double* __restrict__ V) {
+                                                            double* __restrict__ evals, double* __restrict__ V,
+                                                            const int* __restrict__ gate) {
+  if (gate && *gate == 0) return;
   __shared__ int cnt_w[JS_NT / 64], row_w[JS_NT / 64];
   __shared__ double best_w[JS_NT / 64];
   const int i = blockIdx.x, t = threadIdx.x;
@@ -337,9 +342,10 @@ constexpr int CH_R = JACOBI_MAX / CH_G;   // elements per thread per dimension
 static_assert(CH_R * CH_G == JACOBI_MAX, "Cholesky tiling");
 
 struct CholShared {
-  double* Lp;   // packed lower L, column-major
-  double* cb;   // 2 x JACOBI_MAX broadcast buffers
+  double* Lp;     // packed lower L, column-major
+  double* cb;     // 2 x JACOBI_MAX broadcast buffers
   int* bad;
+  double* pmin;   // smallest pivot so far (thread 0 only)
 };
 
 __device__ __forceinline__ int chol_pl(int m, int i, int j) { return j * m - j * (j - 1) / 2 + (i - j); }
@@ -367,7 +373,10 @@ __device__ __forceinline__ void chol_block(double (&A)[CH_R][CH_R], int m, const
     }
     const double sd = sqrt(d);
     const double isd = 1.0 / sd;
-    if (t == 0 && piv) piv[jj] = sd;
+    if (t == 0) {
+      if (piv) piv[jj] = sd;
+      *sh.pmin = fmin(*sh.pmin, sd);
+    }
     double li[CH_R], lj[CH_R];
 #pragma unroll
     for (int a = BJ; a < CH_R; ++a) {
@@ -427,10 +436,12 @@ __device__ __forceinline__ void inv_block(double (&A)[CH_R][CH_R], int m, const 
 
 __global__ __launch_bounds__(CH_G * CH_G) void chol_inv_kernel(const double* __restrict__ G, int m,
                                                                double shift, double* __restrict__ Rinv,
-                                                               int* info, double* __restrict__ piv) {
+                                                               int* info, double* __restrict__ piv,
+                                                               double* __restrict__ pmin_out) {
   extern __shared__ double smem[];
   __shared__ int bad;
-  CholShared sh{smem, smem + (size_t)m * (m + 1) / 2, &bad};
+  __shared__ double pmin_s;
+  CholShared sh{smem, smem + (size_t)m * (m + 1) / 2, &bad, &pmin_s};
   const int t = threadIdx.x, tx = t % CH_G, ty = t / CH_G;
   double A[CH_R][CH_R];
 #pragma unroll
@@ -443,7 +454,7 @@ __global__ __launch_bounds__(CH_G * CH_G) void chol_inv_kernel(const double* __r
       A[a][b] = v;
     }
   }
-  if (t == 0) bad = 0;
+  if (t == 0) { bad = 0; pmin_s = HUGE_VAL; }
   __syncthreads();
   // ---- right-looking Cholesky, six 32-column blocks ----
   chol_block<0>(A, m, sh, tx, ty, t, piv);
@@ -480,7 +491,10 @@ __global__ __launch_bounds__(CH_G * CH_G) void chol_inv_kernel(const double* __r
       const int i = ty + CH_G * a, j = tx + CH_G * b;
       if (i < m && j < m) Rinv[j + (size_t)m * i] = (j <= i) ? A[a][b] : 0.0;
     }
-  if (t == 0 && info) *info = bad;
+  if (t == 0) {
+    if (info) *info = bad;
+    if (pmin_out) *pmin_out = pmin_s;
+  }
 }
 
 
@@ -738,7 +752,7 @@ size_t jacobi_work_doubles(int m, int max_sweeps) {
 }
 
 void jacobi_eig_ws(const double* H, int ldh, int m, double* evals, double* V, double* Vwork, int max_sweeps,
-                   int* sweeps_out, hipStream_t s) {
+                   int* sweeps_out, hipStream_t s, const int* gate) {
   if (m < 1 || m > JACOBI_MAX) throw Error(TNAD_ERR_ARG, "jacobi_eig: size out of range");
   const int mm = m + (m & 1);
   const size_t P = mm / 2;
@@ -750,12 +764,12 @@ void jacobi_eig_ws(const double* H, int ldh, int m, double* evals, double* V, do
   TNAD_HIP_TRY(hipFuncSetAttribute((const void*)jacobi_rot_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
                                    (int)lds));
   hipLaunchKernelGGL(jacobi_rot_kernel, dim3(1), dim3(EIG_NT), lds, s, H, ldh, m, rot, max_sweeps, diag, rounds,
-                     sweeps_out);
+                     sweeps_out, gate);
   TNAD_HIP_TRY(hipGetLastError());
   hipLaunchKernelGGL(jacobi_apply_kernel, dim3((m + JA_WAVES - 1) / JA_WAVES), dim3(64 * JA_WAVES), 0, s, rot,
-                     rounds, m, Vu);
+                     rounds, m, Vu, gate);
   TNAD_HIP_TRY(hipGetLastError());
-  hipLaunchKernelGGL(jacobi_sort_kernel, dim3(m), dim3(JS_NT), 0, s, diag, Vu, m, evals, V);
+  hipLaunchKernelGGL(jacobi_sort_kernel, dim3(m), dim3(JS_NT), 0, s, diag, Vu, m, evals, V, gate);
   TNAD_HIP_TRY(hipGetLastError());
 }
 
@@ -769,12 +783,13 @@ void ritz_select(const double* H, int ldh, int m, int nh, double thresh, double*
   TNAD_HIP_TRY(hipGetLastError());
 }
 
-void chol_inv(const double* G, int m, double shift, double* Rinv, int* info, double* piv, hipStream_t s) {
+void chol_inv(const double* G, int m, double shift, double* Rinv, int* info, double* piv, hipStream_t s,
+              double* pmin) {
   if (m < 1 || m > JACOBI_MAX) throw Error(TNAD_ERR_ARG, "chol_inv: size out of range");
   size_t lds = ((size_t)m * (m + 1) / 2 + 2 * JACOBI_MAX) * sizeof(double);
   TNAD_HIP_TRY(hipFuncSetAttribute((const void*)chol_inv_kernel,
                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-  hipLaunchKernelGGL(chol_inv_kernel, dim3(1), dim3(CH_G * CH_G), lds, s, G, m, shift, Rinv, info, piv);
+  hipLaunchKernelGGL(chol_inv_kernel, dim3(1), dim3(CH_G * CH_G), lds, s, G, m, shift, Rinv, info, piv, pmin);
   TNAD_HIP_TRY(hipGetLastError());
 }
 

@@ -224,7 +224,7 @@ int resolvent_pcg(Ctx* c, int rows, int r, int k, const double* W, const double*
   const double tol2 = tol * tol;
   int it = 0;
   for (;; ++it) {
-    TNAD_HIP_TRY(hipMemcpyAsync(hs.data(), sc.ptr, (size_t)8 * r * 8, hipMemcpyDeviceToHost, s));
+    c->read_async(hs.data(), sc.ptr, (size_t)8 * r * 8);
     c->sync();
     int nact = 0;
     for (int j = 0; j < r; ++j) {
@@ -236,7 +236,7 @@ int resolvent_pcg(Ctx* c, int rows, int r, int k, const double* W, const double*
       nact += h_act[j] != 0.0;
     }
     if (nact == 0 || it >= maxit) break;
-    TNAD_HIP_TRY(hipMemcpyAsync(active, h_act.data(), (size_t)r * 8, hipMemcpyHostToDevice, s));
+    c->write_async(active, h_act.data(), (size_t)r * 8);
     op(p.as<double>(), q.as<double>());
     coldot(p.as<double>(), q.as<double>(), pq);
     hipLaunchKernelGGL(cg_alpha_kernel, dim3(tr), dim3(NT), 0, s, rz, pq, active, r, alpha);

@@ -82,10 +82,12 @@ constexpr int JACOBI_MAX = 192;   // packed FP64 upper triangle of 192x192 fits 
 // Full symmetric eigendecomposition of an m x m matrix (m <= JACOBI_MAX).
 // Output eigenpairs sorted by |lambda| descending; V column-major (ld = m).
 // Vwork: jacobi_work_doubles(m, max_sweeps) doubles of scratch (rotation record
-// and unsorted eigenvectors).  sweeps_out: device int (nullable).
+// and unsorted eigenvectors).  sweeps_out: device int (nullable).  gate (nullable device int): when
+// it holds 0 at run time the kernels return at once and evals / V keep what they hold (a fallback
+// that the device decides, without a host round trip).
 size_t jacobi_work_doubles(int m, int max_sweeps);
 void jacobi_eig_ws(const double* H, int ldh, int m, double* evals, double* V, double* Vwork,
-                   int max_sweeps, int* sweeps_out, hipStream_t s);
+                   int max_sweeps, int* sweeps_out, hipStream_t s, const int* gate = nullptr);
 
 // Eigenpairs of a nearly diagonal symmetric m x m H for its first nh columns (pairs among the
 // others are not rotated), sorted by |lambda| descending: pairs with min(i, j) < nh coupled more
@@ -98,8 +100,10 @@ void ritz_select(const double* H, int ldh, int m, int nh, double thresh, double*
 
 // Cholesky of the m x m SPD matrix G (+ shift on the diagonal), returns Linv^T
 // (upper triangular, column-major) so that X = Y * Rinv with Rinv = L^-T.
-// piv (nullable, m doubles) receives the Cholesky pivots diag(L).
-void chol_inv(const double* G, int m, double shift, double* Rinv, int* info, double* piv, hipStream_t s);
+// piv (nullable, m doubles) receives the Cholesky pivots diag(L); pmin (nullable, device double) the
+// smallest of them.
+void chol_inv(const double* G, int m, double shift, double* Rinv, int* info, double* piv, hipStream_t s,
+              double* pmin = nullptr);
 
 // ---------------------------------------------------------------------------
 // Elementwise / reduction kernels (elem.hip)
@@ -147,8 +151,21 @@ struct Ctx {
   void trim();
   DevBuf buf(size_t bytes) { return DevBuf(this, bytes); }
   template <typename T> DevBuf bufn(size_t n) { return DevBuf(this, n * sizeof(T)); }
-  void sync() { TNAD_HIP_TRY(hipStreamSynchronize(stream)); }
+  // Small host <-> device transfers on the stream go through a pinned host staging area (a transfer
+  // from or to pageable memory costs tens of microseconds of idle GPU each).  read_async queues a
+  // device -> host copy whose data reaches dst_host at the next sync(); write_async copies the host
+  // data into the staging area at once and queues the host -> device copy.
+  void read_async(void* dst_host, const void* src_dev, size_t bytes);
+  void write_async(void* dst_dev, const void* src_host, size_t bytes);
+  void sync();   // stream synchronise, then deliver the queued reads
   ~Ctx();
+ private:
+  char* pinned_ = nullptr;
+  size_t pinned_used_ = 0;
+  struct PendingRead { void* dst; size_t off, bytes; };
+  std::vector<PendingRead> pending_;
+  char* stage(size_t bytes, size_t& off);   // staging space, or nullptr when the transfer is too large
+ public:
 };
 
 // host <-> device helpers that accept either host or device pointers

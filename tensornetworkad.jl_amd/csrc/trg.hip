@@ -292,7 +292,7 @@ static void svd_split(Ctx* c, const double* t, int d1, int d2, int d3, int d4, i
   keig = topk_eig(c, B.as<double>(), N, std::min(want, keig), keig, nullptr, 300, 1e-13, X.as<double>(),
                   AX.as<double>(), theta.as<double>(), nullptr);
   std::vector<double> th(keig);
-  TNAD_HIP_TRY(hipMemcpyAsync(th.data(), theta.ptr, (size_t)keig * 8, hipMemcpyDeviceToHost, s));
+  c->read_async(th.data(), theta.ptr, (size_t)keig * 8);
   c->sync();
   // singular values = the positive eigenvalues in descending order; values at the eigensolver's
   // rounding level (relative 1e-14) are not resolved as +-s pairs and count as exact zeros
@@ -319,9 +319,9 @@ static void svd_split(Ctx* c, const double* t, int d1, int d2, int d3, int d4, i
   tp.V = DevBuf(c, (size_t)mp * Kc * 8);
   k_fill(tp.sq.as<double>(), (size_t)std::max(k, Kc), 0.0, s);
   if (K > 0) {
-    TNAD_HIP_TRY(hipMemcpyAsync(cols.ptr, pos.data(), (size_t)K * sizeof(int), hipMemcpyHostToDevice, s));
-    TNAD_HIP_TRY(hipMemcpyAsync(tp.sv.ptr, tp.hs.data(), (size_t)K * 8, hipMemcpyHostToDevice, s));
-    TNAD_HIP_TRY(hipMemcpyAsync(tp.sq.ptr, hsq.data(), (size_t)K * 8, hipMemcpyHostToDevice, s));
+    c->write_async(cols.ptr, pos.data(), (size_t)K * sizeof(int));
+    c->write_async(tp.sv.ptr, tp.hs.data(), (size_t)K * 8);
+    c->write_async(tp.sq.ptr, hsq.data(), (size_t)K * 8);
     hipLaunchKernelGGL(split_vectors_kernel, dim3(nb((size_t)N * K)), dim3(NT), 0, s, X.as<double>(), N, m,
                        cols.as<int>(), K, tp.U.as<double>(), tp.V.as<double>());
   }
@@ -447,15 +447,15 @@ void trg_run(Ctx* c, const double* a_host, int q, int chi, int niter, double tol
     TNAD_HIP_TRY(hipGetLastError());
     double hv = 0.0;
     int hi = 0;
-    TNAD_HIP_TRY(hipMemcpyAsync(&hv, mv.ptr, 8, hipMemcpyDeviceToHost, s));
-    TNAD_HIP_TRY(hipMemcpyAsync(&hi, mi.ptr, sizeof(int), hipMemcpyDeviceToHost, s));
+    c->read_async(&hv, mv.ptr, 8);
+    c->read_async(&hi, mi.ptr, sizeof(int));
     c->sync();
     if (!(hv > 0.0)) throw Error(TNAD_ERR_NAN, "trg: tensor vanished or is not finite");
     if (grad) {
       T->a_in = DevBuf(c, na * 8);
       k_copy(a.as<double>(), T->a_in.as<double>(), na, s);
       double v = 0.0;
-      TNAD_HIP_TRY(hipMemcpyAsync(&v, a.as<double>() + hi, 8, hipMemcpyDeviceToHost, s));
+      c->read_async(&v, a.as<double>() + hi, 8);
       c->sync();
       T->sgn = v < 0 ? -1.0 : 1.0;
     }
@@ -500,7 +500,7 @@ void trg_run(Ctx* c, const double* a_host, int q, int chi, int niter, double tol
   hipLaunchKernelGGL(trace_kernel, dim3(1), dim3(256), 0, s, a.as<double>(), du, dr, tr.as<double>());
   TNAD_HIP_TRY(hipGetLastError());
   double htr = 0.0;
-  TNAD_HIP_TRY(hipMemcpyAsync(&htr, tr.ptr, 8, hipMemcpyDeviceToHost, s));
+  c->read_async(&htr, tr.ptr, 8);
   c->sync();
   if (!(htr > 0.0)) throw Error(TNAD_ERR_NAN, "trg: non-positive trace");
   lnz += std::log(htr) / std::pow(2.0, niter);
@@ -555,7 +555,7 @@ void trg_run(Ctx* c, const double* a_host, int q, int chi, int niter, double tol
     DevBuf red(c, 300 * 8);
     k_dot(anb.as<double>(), T.a_in.as<double>(), na, red.as<double>(), s);
     double dotv = 0.0;
-    TNAD_HIP_TRY(hipMemcpyAsync(&dotv, red.ptr, 8, hipMemcpyDeviceToHost, s));
+    c->read_async(&dotv, red.ptr, 8);
     c->sync();
     const double mv_ = T.maxval;
     const double mbar = std::pow(2.0, 1 - it) / mv_ - dotv / (mv_ * mv_);
@@ -563,10 +563,10 @@ void trg_run(Ctx* c, const double* a_host, int q, int chi, int niter, double tol
     k_axpby(1.0 / mv_, anb.as<double>(), 0.0, ain.as<double>(), na, s);
     // maximum(abs.(a)) sends its cotangent to the first maximiser with the sign of that entry
     std::vector<double> one(1);
-    TNAD_HIP_TRY(hipMemcpyAsync(one.data(), ain.as<double>() + T.idx, 8, hipMemcpyDeviceToHost, s));
+    c->read_async(one.data(), ain.as<double>() + T.idx, 8);
     c->sync();
     one[0] += mbar * T.sgn;
-    TNAD_HIP_TRY(hipMemcpyAsync(ain.as<double>() + T.idx, one.data(), 8, hipMemcpyHostToDevice, s));
+    c->write_async(ain.as<double>() + T.idx, one.data(), 8);
     abar = std::move(ain);
   }
   copy_out(c, abar_out, abar.ptr, (size_t)q * q * q * q * 8);
@@ -585,8 +585,8 @@ void trg_svd_run(Ctx* c, const double* t, int d1, int d2, int d3, int d4, int dm
   const int k = tp.k;
   // outputs are written with the caller's capacity kmax: u (m x kmax), v (kmax x mp)
   std::vector<double> hu((size_t)m * k), hv((size_t)k * mp);
-  TNAD_HIP_TRY(hipMemcpyAsync(hu.data(), bu.ptr, (size_t)m * k * 8, hipMemcpyDeviceToHost, c->stream));
-  TNAD_HIP_TRY(hipMemcpyAsync(hv.data(), bv.ptr, (size_t)k * mp * 8, hipMemcpyDeviceToHost, c->stream));
+  c->read_async(hu.data(), bu.ptr, (size_t)m * k * 8);
+  c->read_async(hv.data(), bv.ptr, (size_t)k * mp * 8);
   c->sync();
   std::vector<double> ou((size_t)m * kmax, 0.0), ov((size_t)kmax * mp, 0.0);
   for (size_t i = 0; i < (size_t)m * k; ++i) ou[i] = hu[i];

@@ -496,7 +496,7 @@ int tnad_env_from_features(tnad_ctx* ctx, const float* feat, int chi, int D, dou
     k_sumsq(bT.as<double>(), nT, red.as<double>(), s);
     k_copy(red.as<double>(), nrm.as<double>() + 1, 1, s);
     std::vector<double> h(2);
-    TNAD_HIP_TRY(hipMemcpyAsync(h.data(), nrm.ptr, 16, hipMemcpyDeviceToHost, s));
+    c->read_async(h.data(), nrm.ptr, 16);
     c->sync();
     if (!(h[0] > 0.0) || !(h[1] > 0.0)) throw Error(TNAD_ERR_NAN, "env_from_features: zero or NaN environment");
     k_axpby(1.0 / std::sqrt(h[0]), bC.as<double>(), 0.0, bC.as<double>(), nC, s);
