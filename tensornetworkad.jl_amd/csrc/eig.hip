@@ -344,6 +344,7 @@ static_assert(CH_R * CH_G == JACOBI_MAX, "Cholesky tiling");
 struct CholShared {
   double* Lp;     // packed lower L, column-major
   double* cb;     // 2 x JACOBI_MAX broadcast buffers
+  double* invd;   // 1 / diag(L) (JACOBI_MAX)
   int* bad;
   double* pmin;   // smallest pivot so far (thread 0 only)
 };
@@ -351,6 +352,18 @@ struct CholShared {
 __device__ __forceinline__ int chol_pl(int m, int i, int j) { return j * m - j * (j - 1) / 2 + (i - j); }
 
 // elimination steps jj = 32 BJ .. min(32 BJ + 31, m - 1)
+// 1 / sqrt(d) for a positive normal d: the hardware estimate refined by two Newton steps
+__device__ __forceinline__ double rsqrt_refined(double d) {
+  double y = __builtin_amdgcn_rsq(d);
+  y = y * (1.5 - 0.5 * d * y * y);
+  y = y * (1.5 - 0.5 * d * y * y);
+  return y;
+}
+
+// elimination steps jj = 32 BJ .. min(32 BJ + 31, m - 1).  The owner column broadcasts the whole
+// column jj (zeros above the diagonal and in the padding), so the update needs no bounds tests: its
+// only stray products land in row jj above the diagonal (the unused upper triangle) and in column
+// jj, which the owner rewrites.
 template <int BJ>
 __device__ __forceinline__ void chol_block(double (&A)[CH_R][CH_R], int m, const CholShared& sh, int tx, int ty,
                                            int t, double* __restrict__ piv) {
@@ -362,7 +375,7 @@ __device__ __forceinline__ void chol_block(double (&A)[CH_R][CH_R], int m, const
 #pragma unroll
       for (int a = BJ; a < CH_R; ++a) {
         const int i = ty + CH_G * a;
-        if (i >= jj && i < m) col[i] = A[a][BJ];
+        col[i] = (i >= jj && i < m) ? A[a][BJ] : 0.0;
       }
     }
     __syncthreads();
@@ -371,8 +384,8 @@ __device__ __forceinline__ void chol_block(double (&A)[CH_R][CH_R], int m, const
       if (t == 0 && *sh.bad == 0) *sh.bad = jj + 1;
       d = 1e-300;
     }
-    const double sd = sqrt(d);
-    const double isd = 1.0 / sd;
+    const double isd = rsqrt_refined(d);
+    const double sd = d * isd;
     if (t == 0) {
       if (piv) piv[jj] = sd;
       *sh.pmin = fmin(*sh.pmin, sd);
@@ -380,10 +393,8 @@ __device__ __forceinline__ void chol_block(double (&A)[CH_R][CH_R], int m, const
     double li[CH_R], lj[CH_R];
 #pragma unroll
     for (int a = BJ; a < CH_R; ++a) {
-      const int i = ty + CH_G * a;
-      li[a] = (i > jj && i < m) ? col[i] * isd : 0.0;
-      const int j = tx + CH_G * a;
-      lj[a] = (j > jj && j < m) ? col[j] * isd : 0.0;
+      li[a] = col[ty + CH_G * a] * isd;
+      lj[a] = col[tx + CH_G * a] * isd;
     }
     // trailing lower blocks BJ <= b <= a
 #pragma unroll
@@ -401,16 +412,18 @@ __device__ __forceinline__ void chol_block(double (&A)[CH_R][CH_R], int m, const
 }
 
 // forward-substitution steps jj = 32 AJ .. min(32 AJ + 31, m - 1) of B = L^-1:
-// row jj = B[jj,:] / L[jj,jj] ; rows i > jj: B[i,:] -= L[i,jj] B[jj,:]
+// row jj = B[jj,:] / L[jj,jj] ; rows i > jj: B[i,:] -= L[i,jj] B[jj,:].  Row jj of B is zero right of
+// its diagonal, so its broadcast needs no mask (padding columns j >= m may pick up stale values; they
+// are never written out).
 template <int AJ>
 __device__ __forceinline__ void inv_block(double (&A)[CH_R][CH_R], int m, const CholShared& sh, int tx, int ty) {
   const int j0 = AJ * CH_G, j1 = min(m, j0 + CH_G);
   for (int jj = j0; jj < j1; ++jj) {
     double* row = sh.cb + (jj & 1) * JACOBI_MAX;
-    const double ild = 1.0 / sh.Lp[chol_pl(m, jj, jj)];
+    const double ild = sh.invd[jj];
     if (ty == jj - j0) {
 #pragma unroll
-      for (int b = 0; b <= AJ; ++b) {   // row jj of B is zero right of the diagonal
+      for (int b = 0; b <= AJ; ++b) {
         const int j = tx + CH_G * b;
         const double v = A[AJ][b] * ild;
         A[AJ][b] = v;
@@ -420,14 +433,12 @@ __device__ __forceinline__ void inv_block(double (&A)[CH_R][CH_R], int m, const 
     __syncthreads();
     double xr[CH_R];
 #pragma unroll
-    for (int b = 0; b <= AJ; ++b) {
-      const int j = tx + CH_G * b;
-      xr[b] = (j <= jj) ? row[j] : 0.0;
-    }
+    for (int b = 0; b <= AJ; ++b) xr[b] = row[tx + CH_G * b];
+    const int colbase = jj * (m - 1) - jj * (jj - 1) / 2;   // chol_pl(m, i, jj) = colbase + i
 #pragma unroll
     for (int a = AJ; a < CH_R; ++a) {
       const int i = ty + CH_G * a;
-      const double lij = (i > jj && i < m) ? sh.Lp[chol_pl(m, i, jj)] : 0.0;
+      const double lij = (i > jj && i < m) ? sh.Lp[colbase + i] : 0.0;
 #pragma unroll
       for (int b = 0; b <= AJ; ++b) A[a][b] -= lij * xr[b];
     }
@@ -441,7 +452,8 @@ __global__ __launch_bounds__(CH_G * CH_G) void chol_inv_kernel(const double* __r
   extern __shared__ double smem[];
   __shared__ int bad;
   __shared__ double pmin_s;
-  CholShared sh{smem, smem + (size_t)m * (m + 1) / 2, &bad, &pmin_s};
+  CholShared sh{smem, smem + (size_t)m * (m + 1) / 2, smem + (size_t)m * (m + 1) / 2 + 2 * JACOBI_MAX, &bad,
+                &pmin_s};
   const int t = threadIdx.x, tx = t % CH_G, ty = t / CH_G;
   double A[CH_R][CH_R];
 #pragma unroll
@@ -471,6 +483,8 @@ __global__ __launch_bounds__(CH_G * CH_G) void chol_inv_kernel(const double* __r
       const int i = ty + CH_G * a, j = tx + CH_G * b;
       if (i < m && j <= i) sh.Lp[chol_pl(m, i, j)] = A[a][b];
     }
+  __syncthreads();
+  for (int i = t; i < m; i += CH_G * CH_G) sh.invd[i] = 1.0 / sh.Lp[chol_pl(m, i, i)];
   __syncthreads();
   // ---- B = L^-1 ----
 #pragma unroll
@@ -786,7 +800,7 @@ void ritz_select(const double* H, int ldh, int m, int nh, double thresh, double*
 void chol_inv(const double* G, int m, double shift, double* Rinv, int* info, double* piv, hipStream_t s,
               double* pmin) {
   if (m < 1 || m > JACOBI_MAX) throw Error(TNAD_ERR_ARG, "chol_inv: size out of range");
-  size_t lds = ((size_t)m * (m + 1) / 2 + 2 * JACOBI_MAX) * sizeof(double);
+  size_t lds = ((size_t)m * (m + 1) / 2 + 3 * JACOBI_MAX) * sizeof(double);
   TNAD_HIP_TRY(hipFuncSetAttribute((const void*)chol_inv_kernel,
                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   hipLaunchKernelGGL(chol_inv_kernel, dim3(1), dim3(CH_G * CH_G), lds, s, G, m, shift, Rinv, info, piv, pmin);
