@@ -17,8 +17,10 @@ namespace tnad {
 namespace {
 
 constexpr int EIG_NT = 1024;
-// 2x2 blocks per thread per Jacobi round: ceil(P (P + 1) / 2 / EIG_NT) for P = JACOBI_MAX / 2
-constexpr int JB_MAX = ((JACOBI_MAX / 2) * (JACOBI_MAX / 2 + 1) / 2 + EIG_NT - 1) / EIG_NT;
+// Jacobi rotation kernel: JR_NT threads, each holding up to JB_MAX 2x2 blocks (and their fixed
+// addresses) in registers: ceil(P (P + 1) / 2 / JR_NT) for P = JACOBI_MAX / 2.
+constexpr int JR_NT = 1024;
+constexpr int JB_MAX = ((JACOBI_MAX / 2) * (JACOBI_MAX / 2 + 1) / 2 + JR_NT - 1) / JR_NT;
 
 __device__ __forceinline__ int pk(int i, int j) {  // packed upper, i <= j, column-major
   return j * (j + 1) / 2 + i;
@@ -75,7 +77,7 @@ __device__ __forceinline__ int jpos_index0(int pos, int M) {   // index at a pos
   return (pos & 1) ? M - t : t;
 }
 
-__global__ __launch_bounds__(EIG_NT) void jacobi_rot_kernel(const double* __restrict__ H, int ldh, int m,
+__global__ __launch_bounds__(JR_NT) void jacobi_rot_kernel(const double* __restrict__ H, int ldh, int m,
                                                             double* __restrict__ rot, int max_sweeps,
                                                             double* __restrict__ diag, int* rounds_out,
                                                             int* sweeps_out, const int* __restrict__ gate) {
@@ -87,7 +89,7 @@ __global__ __launch_bounds__(EIG_NT) void jacobi_rot_kernel(const double* __rest
   const int ntri = mm * (mm + 1) / 2;
   double* a = smem;                              // packed mm x mm upper, position order
   __shared__ double2 rcs[JACOBI_MAX / 2];        // (cos, sin) per slot (apart from a: no aliasing)
-  __shared__ double red[EIG_NT / 64];
+  __shared__ double red[JR_NT / 64];
   __shared__ int nrot_s;
   __shared__ double fro_s;
   const int t = threadIdx.x;
@@ -95,7 +97,7 @@ __global__ __launch_bounds__(EIG_NT) void jacobi_rot_kernel(const double* __rest
   // load (position order of round 0) and symmetrise (average both triangles so a slightly
   // asymmetric input is still handled)
   double part = 0.0;
-  for (int b = t; b < ntri; b += EIG_NT) {
+  for (int b = t; b < ntri; b += JR_NT) {
     int i, j;
     decode_tri(b, i, j);
     const int x = jpos_index0(i, M), y = jpos_index0(j, M);
@@ -108,7 +110,7 @@ __global__ __launch_bounds__(EIG_NT) void jacobi_rot_kernel(const double* __rest
   __syncthreads();
   if (t == 0) {
     double sacc = 0.0;
-    for (int w = 0; w < EIG_NT / 64; ++w) sacc += red[w];
+    for (int w = 0; w < JR_NT / 64; ++w) sacc += red[w];
     fro_s = sqrt(sacc);
   }
   __syncthreads();
@@ -126,7 +128,7 @@ __global__ __launch_bounds__(EIG_NT) void jacobi_rot_kernel(const double* __rest
     const int nblk = P * (P + 1) / 2;
 #pragma unroll
     for (int u = 0; u < JB_MAX; ++u) {
-      const int b = t + u * EIG_NT;
+      const int b = t + u * JR_NT;
       int i = 0, j = 0;
       if (b < nblk) decode_tri(b, i, j);
       bij[u] = (b < nblk) ? (i | (j << 16)) : -1;
@@ -210,7 +212,7 @@ __global__ __launch_bounds__(EIG_NT) void jacobi_rot_kernel(const double* __rest
     __syncthreads();
   }
   // whole sweeps ran: every index is back at its round-0 position
-  for (int i = t; i < m; i += EIG_NT) {
+  for (int i = t; i < m; i += JR_NT) {
     int pos;
     if (i == 0) pos = 0;
     else if (i == M) pos = 1;
@@ -777,7 +779,7 @@ void jacobi_eig_ws(const double* H, int ldh, int m, double* evals, double* V, do
   size_t lds = jacobi_lds(m);
   TNAD_HIP_TRY(hipFuncSetAttribute((const void*)jacobi_rot_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
                                    (int)lds));
-  hipLaunchKernelGGL(jacobi_rot_kernel, dim3(1), dim3(EIG_NT), lds, s, H, ldh, m, rot, max_sweeps, diag, rounds,
+  hipLaunchKernelGGL(jacobi_rot_kernel, dim3(1), dim3(JR_NT), lds, s, H, ldh, m, rot, max_sweeps, diag, rounds,
                      sweeps_out, gate);
   TNAD_HIP_TRY(hipGetLastError());
   hipLaunchKernelGGL(jacobi_apply_kernel, dim3((m + JA_WAVES - 1) / JA_WAVES), dim3(64 * JA_WAVES), 0, s, rot,

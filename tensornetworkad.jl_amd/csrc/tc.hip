@@ -45,7 +45,10 @@ TcDesc gemm_desc(char ta, char tb, int M, int N, int K, long long lda, long long
 namespace {
 
 constexpr int BM = 64, BN = 64, NT = 256;
-constexpr int TC_BK = 16;   // K-tile of the FP32 / FP64 kernel (32 measured slower: occupancy)
+// K-tile of the FP32 / FP64 kernel: 16 for grids that fill the chip (occupancy matters there), 32 for
+// grids of fewer than 256 workgroups, which wait on global-load latency once per K-tile
+constexpr int TC_BK = 16, TC_BK_DEEP = 32, TC_DEEP_BELOW = 256;
+constexpr int TC_BIG_MIN_WG = 1024;  // 128 x 128 tiles only at 4+ workgroups per CU (slower at 1 per CU)
 
 __device__ __forceinline__ long long off2(int i, int I0, long long s0, long long s1) {
   int i0 = i % I0;
@@ -93,20 +96,23 @@ template <> struct MfmaOp<float> {
 };
 
 // ---------------------------------------------------------------------------
-// F64 / F32 kernel (k-major LDS tiles, one element per lane per MFMA)
+// F64 / F32 kernel (k-major LDS tiles, one element per lane per MFMA).  TILE x TILE output tile per
+// workgroup of 4 waves in 2 x 2; each wave owns a (TILE/2)^2 block of (TILE/32)^2 MFMA 16x16 blocks,
+// so TILE = 128 does twice the MFMA work per LDS fragment read of TILE = 64.
 // ---------------------------------------------------------------------------
-template <typename T, int BK>
+template <typename T, int BK, int TILE>
 __global__ __launch_bounds__(NT) void tc_kernel(TcDesc d, const T* __restrict__ A,
                                                 const T* __restrict__ B, T* __restrict__ C,
                                                 T* __restrict__ Wsp) {
   constexpr int PAD = 16;
-  constexpr int NR = BK / 4;   // loads per thread and operand per K-tile (64 x BK over NT threads)
-  static_assert(NR * NT == 64 * BK, "loader coverage");
-  __shared__ T As[BK][BM + PAD];
-  __shared__ T Bs[BK][BN + PAD];
+  constexpr int NR = TILE * BK / NT;   // loads per thread and operand per K-tile
+  constexpr int FB = TILE / 32;        // MFMA blocks per wave and dimension
+  static_assert(NR * NT == TILE * BK && NT % TILE == 0 && NT % BK == 0, "loader coverage");
+  __shared__ T As[BK][TILE + PAD];
+  __shared__ T Bs[BK][TILE + PAD];
 
   const int M = d.M0 * d.M1, N = d.N0 * d.N1;
-  const int tile_m = blockIdx.x * BM, tile_n = blockIdx.y * BN;
+  const int tile_m = blockIdx.x * TILE, tile_n = blockIdx.y * TILE;
   // grid z: batch index, or split-K slice (batch 1) when a workspace is given
   const int bz = Wsp ? 0 : blockIdx.z;
   const int kbeg = Wsp ? blockIdx.z * d.kchunk : 0;
@@ -119,14 +125,14 @@ __global__ __launch_bounds__(NT) void tc_kernel(TcDesc d, const T* __restrict__ 
   const int lane = t & 63, wave = t >> 6;
   const int wm = wave & 1, wn = wave >> 1;
 
-  // A loader mapping
+  // A loader mapping: consecutive threads along m (m contiguous in memory) or along k
   const bool a_mfast = (d.am0 == 1 && d.M0 > 1) || d.ak0 != 1;
   int a_mi[NR], a_ki[NR];
   long long a_moff[NR];
   bool a_mok[NR];
 #pragma unroll
   for (int r = 0; r < NR; ++r) {
-    if (a_mfast) { a_mi[r] = t & 63; a_ki[r] = (t >> 6) + 4 * r; }
+    if (a_mfast) { a_mi[r] = t % TILE; a_ki[r] = t / TILE + (NT / TILE) * r; }
     else         { a_ki[r] = t % BK; a_mi[r] = t / BK + (NT / BK) * r; }
     int m = tile_m + a_mi[r];
     a_mok[r] = m < M;
@@ -139,7 +145,7 @@ __global__ __launch_bounds__(NT) void tc_kernel(TcDesc d, const T* __restrict__ 
   bool b_nok[NR];
 #pragma unroll
   for (int r = 0; r < NR; ++r) {
-    if (b_nfast) { b_ni[r] = t & 63; b_ki[r] = (t >> 6) + 4 * r; }
+    if (b_nfast) { b_ni[r] = t % TILE; b_ki[r] = t / TILE + (NT / TILE) * r; }
     else         { b_ki[r] = t % BK; b_ni[r] = t / BK + (NT / BK) * r; }
     int n = tile_n + b_ni[r];
     b_nok[r] = n < N;
@@ -172,11 +178,11 @@ __global__ __launch_bounds__(NT) void tc_kernel(TcDesc d, const T* __restrict__ 
   };
 
   typedef typename MfmaOp<T>::acc_t acc_t;
-  acc_t acc[2][2];
+  acc_t acc[FB][FB];
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
+  for (int i = 0; i < FB; ++i)
 #pragma unroll
-    for (int j = 0; j < 2; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
+    for (int j = 0; j < FB; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
 
   gload(kbeg);
   for (int k0 = kbeg; k0 < K; k0 += BK) {
@@ -190,15 +196,15 @@ __global__ __launch_bounds__(NT) void tc_kernel(TcDesc d, const T* __restrict__ 
     if (k0 + BK < K) gload(k0 + BK);
 #pragma unroll
     for (int kk = 0; kk < BK; kk += 4) {
-      T af[2], bf[2];
+      T af[FB], bf[FB];
 #pragma unroll
-      for (int i = 0; i < 2; ++i) af[i] = As[kk + (lane >> 4)][wm * 32 + i * 16 + (lane & 15)];
+      for (int i = 0; i < FB; ++i) af[i] = As[kk + (lane >> 4)][wm * (TILE / 2) + i * 16 + (lane & 15)];
 #pragma unroll
-      for (int j = 0; j < 2; ++j) bf[j] = Bs[kk + (lane >> 4)][wn * 32 + j * 16 + (lane & 15)];
+      for (int j = 0; j < FB; ++j) bf[j] = Bs[kk + (lane >> 4)][wn * (TILE / 2) + j * 16 + (lane & 15)];
 #pragma unroll
-      for (int i = 0; i < 2; ++i)
+      for (int i = 0; i < FB; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j) acc[i][j] = MfmaOp<T>::run(af[i], bf[j], acc[i][j]);
+        for (int j = 0; j < FB; ++j) acc[i][j] = MfmaOp<T>::run(af[i], bf[j], acc[i][j]);
     }
   }
 
@@ -206,15 +212,15 @@ __global__ __launch_bounds__(NT) void tc_kernel(TcDesc d, const T* __restrict__ 
   const T alpha = (T)d.alpha, beta = (T)d.beta;
   T* Wz = Wsp ? Wsp + (size_t)blockIdx.z * M * N : nullptr;   // this slice's M x N partial
 #pragma unroll
-  for (int j = 0; j < 2; ++j) {
-    int n = tile_n + wn * 32 + j * 16 + (lane & 15);
+  for (int j = 0; j < FB; ++j) {
+    int n = tile_n + wn * (TILE / 2) + j * 16 + (lane & 15);
     if (n >= N) continue;
     long long noff = off2(n, d.N0, d.cn0, d.cn1);
 #pragma unroll
-    for (int i = 0; i < 2; ++i) {
+    for (int i = 0; i < FB; ++i) {
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        int m = tile_m + wm * 32 + i * 16 + MfmaOp<T>::row(lane, r);
+        int m = tile_m + wm * (TILE / 2) + i * 16 + MfmaOp<T>::row(lane, r);
         if (m >= M) continue;
         if (Wz) {
           Wz[m + (size_t)M * n] = (T)acc[i][j][r];
@@ -434,14 +440,32 @@ void tc_launch(Ctx* c, Prec p, const TcDesc& d, const void* A, const void* B, vo
     grid.z = (unsigned)nsplit;
   }
   void* W = nsplit > 1 ? wsp.ptr : nullptr;
+  // 128 x 128 tiles for large outputs (at least TC_BIG_MIN_WG of them); 64 x 64 otherwise
+  const dim3 grid_big((unsigned)((M + 127) / 128), (unsigned)((N + 127) / 128), grid.z);
+  const bool big = nsplit == 1 && (long long)grid_big.x * grid_big.y * grid_big.z >= TC_BIG_MIN_WG;
+  const bool deep = !big && (long long)grid.x * grid.y * grid.z < TC_DEEP_BELOW;
   switch (p) {
     case Prec::F64:
-      hipLaunchKernelGGL((tc_kernel<double, TC_BK>), grid, dim3(NT), 0, s, dd, (const double*)A,
-                         (const double*)B, (double*)C, (double*)W);
+      if (big)
+        hipLaunchKernelGGL((tc_kernel<double, TC_BK, 128>), grid_big, dim3(NT), 0, s, dd, (const double*)A,
+                           (const double*)B, (double*)C, (double*)W);
+      else if (deep)
+        hipLaunchKernelGGL((tc_kernel<double, TC_BK_DEEP, 64>), grid, dim3(NT), 0, s, dd, (const double*)A,
+                           (const double*)B, (double*)C, (double*)W);
+      else
+        hipLaunchKernelGGL((tc_kernel<double, TC_BK, 64>), grid, dim3(NT), 0, s, dd, (const double*)A,
+                           (const double*)B, (double*)C, (double*)W);
       break;
     case Prec::F32:
-      hipLaunchKernelGGL((tc_kernel<float, TC_BK>), grid, dim3(NT), 0, s, dd, (const float*)A,
-                         (const float*)B, (float*)C, (float*)W);
+      if (big)
+        hipLaunchKernelGGL((tc_kernel<float, TC_BK, 128>), grid_big, dim3(NT), 0, s, dd, (const float*)A,
+                           (const float*)B, (float*)C, (float*)W);
+      else if (deep)
+        hipLaunchKernelGGL((tc_kernel<float, TC_BK_DEEP, 64>), grid, dim3(NT), 0, s, dd, (const float*)A,
+                           (const float*)B, (float*)C, (float*)W);
+      else
+        hipLaunchKernelGGL((tc_kernel<float, TC_BK, 64>), grid, dim3(NT), 0, s, dd, (const float*)A,
+                           (const float*)B, (float*)C, (float*)W);
       break;
     case Prec::BF16:
       hipLaunchKernelGGL(tc_kernel_bf16, grid, dim3(NT), 0, s, dd, (const float*)A,
