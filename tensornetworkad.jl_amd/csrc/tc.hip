@@ -49,6 +49,7 @@ constexpr int BM = 64, BN = 64, NT = 256;
 // grids of fewer than 256 workgroups, which wait on global-load latency once per K-tile
 constexpr int TC_BK = 16, TC_BK_DEEP = 32, TC_DEEP_BELOW = 256;
 constexpr int TC_BIG_MIN_WG = 1024;  // 128 x 128 tiles only at 4+ workgroups per CU (slower at 1 per CU)
+constexpr int TC_SMALL_BELOW = 128;  // 32 x 32 tiles below this many 64 x 64 workgroups
 
 __device__ __forceinline__ long long off2(int i, int I0, long long s0, long long s1) {
   int i0 = i % I0;
@@ -443,10 +444,16 @@ void tc_launch(Ctx* c, Prec p, const TcDesc& d, const void* A, const void* B, vo
   // 128 x 128 tiles for large outputs (at least TC_BIG_MIN_WG of them); 64 x 64 otherwise
   const dim3 grid_big((unsigned)((M + 127) / 128), (unsigned)((N + 127) / 128), grid.z);
   const bool big = nsplit == 1 && (long long)grid_big.x * grid_big.y * grid_big.z >= TC_BIG_MIN_WG;
-  const bool deep = !big && (long long)grid.x * grid.y * grid.z < TC_DEEP_BELOW;
+  // 32 x 32 tiles (4x the workgroups) when 64 x 64 tiles would leave most CUs idle
+  const dim3 grid_small((unsigned)((M + 31) / 32), (unsigned)((N + 31) / 32), grid.z);
+  const bool small = !big && (long long)grid.x * grid.y * grid.z < TC_SMALL_BELOW && grid_small.y <= 65535;
+  const bool deep = !big && !small && (long long)grid.x * grid.y * grid.z < TC_DEEP_BELOW;
   switch (p) {
     case Prec::F64:
-      if (big)
+      if (small)
+        hipLaunchKernelGGL((tc_kernel<double, TC_BK, 32>), grid_small, dim3(NT), 0, s, dd, (const double*)A,
+                           (const double*)B, (double*)C, (double*)W);
+      else if (big)
         hipLaunchKernelGGL((tc_kernel<double, TC_BK, 128>), grid_big, dim3(NT), 0, s, dd, (const double*)A,
                            (const double*)B, (double*)C, (double*)W);
       else if (deep)
@@ -457,7 +464,10 @@ void tc_launch(Ctx* c, Prec p, const TcDesc& d, const void* A, const void* B, vo
                            (const double*)B, (double*)C, (double*)W);
       break;
     case Prec::F32:
-      if (big)
+      if (small)
+        hipLaunchKernelGGL((tc_kernel<float, TC_BK, 32>), grid_small, dim3(NT), 0, s, dd, (const float*)A,
+                           (const float*)B, (float*)C, (float*)W);
+      else if (big)
         hipLaunchKernelGGL((tc_kernel<float, TC_BK, 128>), grid_big, dim3(NT), 0, s, dd, (const float*)A,
                            (const float*)B, (float*)C, (float*)W);
       else if (deep)
