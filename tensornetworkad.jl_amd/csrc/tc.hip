@@ -421,13 +421,20 @@ void tc_launch(Ctx* c, Prec p, const TcDesc& d, const void* A, const void* B, vo
     throw Error(TNAD_ERR_ARG, "tc_launch: extents out of range");
   dim3 grid((unsigned)((M + BM - 1) / BM), (unsigned)((N + BN - 1) / BN), (unsigned)d.batch);
   if (grid.y > 65535) throw Error(TNAD_ERR_ARG, "tc_launch: N too large");
+  const long long K = (long long)d.K0 * d.K1;
+  const long long tiles64 = (long long)grid.x * grid.y;
+  // 32 x 32 tiles (4x the workgroups) when 64 x 64 tiles would leave most CUs idle: short K, or an
+  // output of at most 16 tiles (the k x k Gram products, cut along K below as well)
+  const dim3 grid32((unsigned)((M + 31) / 32), (unsigned)((N + 31) / 32), (unsigned)d.batch);
+  const bool small = (long long)grid.x * grid.y * grid.z < TC_SMALL_BELOW && grid32.y <= 65535 &&
+                     (K < 256 || tiles64 <= 16);
+  if (small) grid = grid32;
   // split-K when the output tiles cannot fill the chip and K is long: about 512 workgroups,
   // slices of at least 128 K elements (a multiple of both kernels' K-tile)
-  const long long K = (long long)d.K0 * d.K1;
   const long long tiles = (long long)grid.x * grid.y;
   TcDesc dd = d;
   int nsplit = 1;
-  if (d.batch == 1 && tiles < 128 && K >= 256) {
+  if (d.batch == 1 && tiles < (small ? 512 : 128) && K >= 256) {
     long long want = std::min<long long>((512 + tiles - 1) / tiles, (K + 127) / 128);
     long long chunk = (K + want - 1) / want;
     chunk = (chunk + 31) / 32 * 32;
@@ -443,15 +450,12 @@ void tc_launch(Ctx* c, Prec p, const TcDesc& d, const void* A, const void* B, vo
   void* W = nsplit > 1 ? wsp.ptr : nullptr;
   // 128 x 128 tiles for large outputs (at least TC_BIG_MIN_WG of them); 64 x 64 otherwise
   const dim3 grid_big((unsigned)((M + 127) / 128), (unsigned)((N + 127) / 128), grid.z);
-  const bool big = nsplit == 1 && (long long)grid_big.x * grid_big.y * grid_big.z >= TC_BIG_MIN_WG;
-  // 32 x 32 tiles (4x the workgroups) when 64 x 64 tiles would leave most CUs idle
-  const dim3 grid_small((unsigned)((M + 31) / 32), (unsigned)((N + 31) / 32), grid.z);
-  const bool small = !big && (long long)grid.x * grid.y * grid.z < TC_SMALL_BELOW && grid_small.y <= 65535;
+  const bool big = !small && nsplit == 1 && (long long)grid_big.x * grid_big.y * grid_big.z >= TC_BIG_MIN_WG;
   const bool deep = !big && !small && (long long)grid.x * grid.y * grid.z < TC_DEEP_BELOW;
   switch (p) {
     case Prec::F64:
       if (small)
-        hipLaunchKernelGGL((tc_kernel<double, TC_BK, 32>), grid_small, dim3(NT), 0, s, dd, (const double*)A,
+        hipLaunchKernelGGL((tc_kernel<double, TC_BK, 32>), grid, dim3(NT), 0, s, dd, (const double*)A,
                            (const double*)B, (double*)C, (double*)W);
       else if (big)
         hipLaunchKernelGGL((tc_kernel<double, TC_BK, 128>), grid_big, dim3(NT), 0, s, dd, (const double*)A,
@@ -465,7 +469,7 @@ void tc_launch(Ctx* c, Prec p, const TcDesc& d, const void* A, const void* B, vo
       break;
     case Prec::F32:
       if (small)
-        hipLaunchKernelGGL((tc_kernel<float, TC_BK, 32>), grid_small, dim3(NT), 0, s, dd, (const float*)A,
+        hipLaunchKernelGGL((tc_kernel<float, TC_BK, 32>), grid, dim3(NT), 0, s, dd, (const float*)A,
                            (const float*)B, (float*)C, (float*)W);
       else if (big)
         hipLaunchKernelGGL((tc_kernel<float, TC_BK, 128>), grid_big, dim3(NT), 0, s, dd, (const float*)A,
