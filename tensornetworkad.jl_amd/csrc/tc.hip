@@ -424,10 +424,11 @@ void tc_launch(Ctx* c, Prec p, const TcDesc& d, const void* A, const void* B, vo
   const long long K = (long long)d.K0 * d.K1;
   const long long tiles64 = (long long)grid.x * grid.y;
   // 32 x 32 tiles (4x the workgroups) when 64 x 64 tiles would leave most CUs idle: short K, or an
-  // output of at most 16 tiles (the k x k Gram products, cut along K below as well)
+  // output of at most 16 tiles with K up to 4096 (the k x k Gram products of the eigensolver, cut along
+  // K below as well; for longer K the 64 x 64 tile's higher MFMA rate per slice wins)
   const dim3 grid32((unsigned)((M + 31) / 32), (unsigned)((N + 31) / 32), (unsigned)d.batch);
   const bool small = (long long)grid.x * grid.y * grid.z < TC_SMALL_BELOW && grid32.y <= 65535 &&
-                     (K < 256 || tiles64 <= 16);
+                     (K < 256 || (tiles64 <= 16 && K <= 4096));
   if (small) grid = grid32;
   // split-K when the output tiles cannot fill the chip and K is long: about 512 workgroups,
   // slices of at least 128 K elements (a multiple of both kernels' K-tile)
