@@ -184,7 +184,7 @@ def test_energy_gradient_full_spectrum(ctx, seed, chi, maxit):
 
 
 def test_energy_gradient_truncated_eigensolver(ctx):
-    # chi*D = 216 > 192: block subspace iteration + Neumann remainder in the adjoint
+    # chi*D = 216 > 192: block subspace iteration + resolvent CG for the rest of the spectrum in the adjoint
     import tnad
     _say("case: energy grad truncated D=3 chi=24")
     h = O.hamiltonian_heisenberg()
