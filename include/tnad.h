@@ -53,7 +53,7 @@ typedef struct tnad_opts {
   int precision;       /* contraction precision of the low-precision phase (TNAD_PREC_*) */
   double switch_tol;   /* precision != FP64: switch to FP64 once |vals - oldvals| <= switch_tol */
   int bwd_precision;   /* contraction precision of the reverse sweep */
-  int eig_block;       /* subspace block size k for chi*D > 192 (0 = 1.5 chi, capped at 192) */
+  int eig_block;       /* subspace block size k for chi*D > 192 (0 = 2.25 chi, capped at 192) */
   int eig_max_iters;   /* cap on subspace iterations per step, cold start (stops at eig_tol) */
   int eig_warm;        /* 1: start each step's eigensolver from the previous step's Ritz vectors */
   int eig_warm_iters;  /* cap on subspace iterations per step, warm start (stops at eig_tol) */

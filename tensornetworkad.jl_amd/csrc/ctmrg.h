@@ -66,9 +66,13 @@ struct EigWork {
 int topk_eig(Ctx* c, const double* A, int n, int want, int k, const double* X0, int maxit, double tol,
              double* X, double* AX, double* theta, EigStats* st);
 
+// Subspace block of a step: n (full spectrum) for n <= 192; else the chi kept columns plus a guard of
+// 1.25 chi (2.25 chi in all, capped at 192), or eig_block when set.  The guard sets the convergence
+// rate |lambda_(k+1) / lambda_chi| per iteration: on random normal D=3 bulks at chi=64 a block of 96
+// (1.5 chi) took ~40 iterations per step, 144 about 2.5x fewer and 25% less time.
 inline int step_k(const StepDims& d, const StepOpts& o) {
   if (d.n <= JACOBI_MAX) return d.n;
-  int k = o.eig_block > 0 ? o.eig_block : d.chi + d.chi / 2;
+  int k = o.eig_block > 0 ? o.eig_block : 2 * d.chi + d.chi / 4;
   if (k < d.chi) k = d.chi;
   if (k > JACOBI_MAX) k = JACOBI_MAX;
   if (k > d.n) k = d.n;
