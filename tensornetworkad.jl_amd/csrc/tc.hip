@@ -109,8 +109,8 @@ __global__ __launch_bounds__(NT) void tc_kernel(TcDesc d, const T* __restrict__ 
   constexpr int NR = TILE * BK / NT;   // loads per thread and operand per K-tile
   constexpr int FB = TILE / 32;        // MFMA blocks per wave and dimension
   static_assert(NR * NT == TILE * BK && NT % TILE == 0 && NT % BK == 0, "loader coverage");
-  __shared__ T As[BK][TILE + PAD];
-  __shared__ T Bs[BK][TILE + PAD];
+  __shared__ T As[2][BK][TILE + PAD];   // double-buffered: K-tile t in buffer t % 2
+  __shared__ T Bs[2][BK][TILE + PAD];
 
   const int M = d.M0 * d.M1, N = d.N0 * d.N1;
   const int tile_m = blockIdx.x * TILE, tile_n = blockIdx.y * TILE;
@@ -185,28 +185,40 @@ __global__ __launch_bounds__(NT) void tc_kernel(TcDesc d, const T* __restrict__ 
 #pragma unroll
     for (int j = 0; j < FB; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
 
-  gload(kbeg);
-  for (int k0 = kbeg; k0 < K; k0 += BK) {
-    __syncthreads();
+  // Software pipeline over K-tiles with two LDS buffers: while tile t is multiplied out of buffer t % 2,
+  // tile t + 1 (loaded into registers during the previous step) is stored into the other buffer and the
+  // loads of tile t + 2 are issued.  One barrier per K-tile: it publishes tile t + 1 and retires the reads
+  // of tile t, whose buffer the next step refills.
+  auto stage = [&](int buf) {
 #pragma unroll
     for (int r = 0; r < NR; ++r) {
-      As[a_ki[r]][a_mi[r]] = ra[r];
-      Bs[b_ki[r]][b_ni[r]] = rb[r];
+      As[buf][a_ki[r]][a_mi[r]] = ra[r];
+      Bs[buf][b_ki[r]][b_ni[r]] = rb[r];
     }
-    __syncthreads();
-    if (k0 + BK < K) gload(k0 + BK);
+  };
+  gload(kbeg);
+  stage(0);
+  if (kbeg + BK < K) gload(kbeg + BK);
+  __syncthreads();
+  int buf = 0;
+  for (int k0 = kbeg; k0 < K; k0 += BK, buf ^= 1) {
 #pragma unroll
     for (int kk = 0; kk < BK; kk += 4) {
       T af[FB], bf[FB];
 #pragma unroll
-      for (int i = 0; i < FB; ++i) af[i] = As[kk + (lane >> 4)][wm * (TILE / 2) + i * 16 + (lane & 15)];
+      for (int i = 0; i < FB; ++i) af[i] = As[buf][kk + (lane >> 4)][wm * (TILE / 2) + i * 16 + (lane & 15)];
 #pragma unroll
-      for (int j = 0; j < FB; ++j) bf[j] = Bs[kk + (lane >> 4)][wn * (TILE / 2) + j * 16 + (lane & 15)];
+      for (int j = 0; j < FB; ++j) bf[j] = Bs[buf][kk + (lane >> 4)][wn * (TILE / 2) + j * 16 + (lane & 15)];
 #pragma unroll
       for (int i = 0; i < FB; ++i)
 #pragma unroll
         for (int j = 0; j < FB; ++j) acc[i][j] = MfmaOp<T>::run(af[i], bf[j], acc[i][j]);
     }
+    if (k0 + BK < K) {
+      stage(buf ^ 1);
+      if (k0 + 2 * BK < K) gload(k0 + 2 * BK);
+    }
+    __syncthreads();
   }
 
   // epilogue: column lane%16 of each 16x16 block, rows per MfmaOp<T>::row
