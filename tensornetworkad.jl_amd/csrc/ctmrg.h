@@ -61,7 +61,8 @@ struct EigWork {
 
 // Top-k eigenpairs (by |lambda|) of the symmetric n x n FP64 matrix A.
 // n <= 192: full Jacobi (k = n).  Otherwise block subspace iteration with
-// Rayleigh-Ritz and sCholQR3 orthonormalisation, optionally warm-started from X0.
+// Rayleigh-Ritz and CholeskyQR orthonormalisation (one pass, CholeskyQR2 or shifted CholeskyQR3 as the
+// block needs), optionally warm-started from X0.
 // Outputs X (n x k), AX = A X, theta (k).  Returns k.
 int topk_eig(Ctx* c, const double* A, int n, int want, int k, const double* X0, int maxit, double tol,
              double* X, double* AX, double* theta, EigStats* st);

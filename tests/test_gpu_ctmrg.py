@@ -251,3 +251,19 @@ def test_mixed_fp32_energy_within_1e7(ctx):
     assert abs(e32 - e64) < 1e-7
     e_o = O.energy(h, b, 24, 1e-10, 100)
     assert abs(e64 - e_o) < 1e-9
+
+
+def test_mixed_bf16_energy_and_gradient_within_budget(ctx):
+    # BF16 operands on MFMA (FP32 accumulation) while the singular values change by more than 1e-2, then
+    # FP64 steps to tol; the FP32 reverse sweep.  Energy within 1e-7 and gradient within 1e-5 (relative)
+    # of the plain FP64 path, the mixed-precision error budget.
+    import tnad
+    _say("case: mixed bf16 D=3 chi=24")
+    h = O.hamiltonian_heisenberg()
+    b = _bulk(13, d=3)
+    e64, g64 = tnad.energy_and_grad(h, b, 24, 1e-10, 100, ctx=ctx)
+    e16, g16, info = tnad.energy_and_grad(h, b, 24, 1e-10, 100, ctx=ctx, return_info=True,
+                                          bwd_precision="fp32", **tnad.MIXED_BF16)
+    assert info["low_prec_iters"] > 0
+    assert abs(e16 - e64) < 1e-7
+    assert np.linalg.norm(g16 - g64) / np.linalg.norm(g64) < 1e-5
