@@ -53,18 +53,34 @@ __device__ __forceinline__ int pk24(int lo, int hi) {  // pk for 0 <= lo <= hi <
   return (int)(__umul24((unsigned)hi, (unsigned)hi + 1u) >> 1) + lo;
 }
 
-// Jacobi sweeps on the packed matrix in LDS, stored in slot-position order.  Round r of the
-// round-robin ordering pairs, in slot t, the indices a = r + t and b = r - t (mod M = mm - 1; slot 0
-// pairs r with the fixed index M).  Position 2t holds a and 2t + 1 holds b, so every round works on
-// the same positions: a thread's 2x2 blocks (pairs of slots) sit at addresses fixed for the whole
-// run.  Between rounds the indices move by a fixed permutation of positions (a-members one slot
-// down, b-members one slot up, around the ends; see jpos_next), which the rotated blocks follow on
+// Jacobi sweeps on the matrix in LDS, stored in slot-position order.  Round r of the round-robin
+// ordering pairs, in slot t, the indices a = r + t and b = r - t (mod M = mm - 1; slot 0 pairs r with
+// the fixed index M).  Position 2t holds a and 2t + 1 holds b, so every round works on the same
+// positions: a thread's 2x2 blocks (pairs of slots) sit at addresses fixed for the whole run.
+// Between rounds the indices move by a fixed permutation of positions (a-members one slot down,
+// b-members one slot up, around the ends; see jpos_next), which the rotated blocks follow on
 // write-back; after a full sweep (M rounds) every index is back where it started.  An odd m is
-// padded to even mm with a zero row and column (never rotated).  The rotations of every round are
-// recorded for the index pair (p, q) = (min, max) of each slot (rot[round * P + slot] = {c, s});
-// the eigenvector matrix is formed afterwards by jacobi_apply_kernel.  Writes the (unsorted)
-// diagonal to diag and the number of rounds to *rounds_out.
+// padded to even mm with a zero row and column (never rotated).
+//
+// Layout: the upper triangle of 2x2 blocks (slot i <= slot j, block number jblk(i, j) =
+// j (j + 1) / 2 + i) in four planes of P (P + 1) / 2 doubles, one per block element (a_i a_j, a_i b_j,
+// b_i a_j, b_i b_j).  Threads own consecutive blocks, so each plane is read with consecutive 8-byte
+// words across a wavefront, and most write-backs (to blocks (i -+ 1, j -+ 1)) stay consecutive too.
+// A diagonal block keeps its off-diagonal entry in the a_i b_j plane only; its b_i a_j word is
+// never written and never read.
+//
+// The rotations of every round are recorded for the index pair (p, q) = (min, max) of each slot
+// (rot[round * P + slot] = {c, s}); the eigenvector matrix is formed afterwards by
+// jacobi_apply_kernel.  Writes the (unsorted) diagonal to diag and the number of rounds to
+// *rounds_out.
+__device__ __forceinline__ int jblk(int i, int j) { return j * (j + 1) / 2 + i; }
+// plane address of the entry at positions (x, y): symmetric, so the pair is ordered by (slot, member)
+__device__ __forceinline__ int jplane_adr(int x, int y, int nblk) {
+  if (x > y) { const int tmp = x; x = y; y = tmp; }
+  return (2 * (x & 1) + (y & 1)) * nblk + jblk(x >> 1, y >> 1);
+}
 __device__ __forceinline__ int jpos_next(int pos, int P) {   // position of an index one round later
+  if (P == 1) return pos;                                    // one slot: the pair never changes
   const int t = pos >> 1;
   if ((pos & 1) == 0) return t == 0 ? 3 : (t == 1 ? 0 : pos - 2);   // a-members move down
   if (t == 0) return 1;                                              // the fixed index M
@@ -86,8 +102,8 @@ __global__ __launch_bounds__(JR_NT) void jacobi_rot_kernel(const double* __restr
   const int mm = m + (m & 1);
   const int P = mm / 2;
   const int M = mm - 1;
-  const int ntri = mm * (mm + 1) / 2;
-  double* a = smem;                              // packed mm x mm upper, position order
+  const int nblk = P * (P + 1) / 2;
+  double* a = smem;                              // 4 planes of nblk blocks, position order
   __shared__ double2 rcs[JACOBI_MAX / 2];        // (cos, sin) per slot (apart from a: no aliasing)
   __shared__ double red[JR_NT / 64];
   __shared__ int nrot_s;
@@ -97,13 +113,17 @@ __global__ __launch_bounds__(JR_NT) void jacobi_rot_kernel(const double* __restr
   // load (position order of round 0) and symmetrise (average both triangles so a slightly
   // asymmetric input is still handled)
   double part = 0.0;
-  for (int b = t; b < ntri; b += JR_NT) {
+  for (int q = t; q < 4 * nblk; q += JR_NT) {
+    const int e = q / nblk, b = q - e * nblk;
     int i, j;
     decode_tri(b, i, j);
-    const int x = jpos_index0(i, M), y = jpos_index0(j, M);
+    const int x = jpos_index0(2 * i + (e >> 1), M), y = jpos_index0(2 * j + (e & 1), M);
     const double v = (x < m && y < m) ? 0.5 * (H[x + (long long)y * ldh] + H[y + (long long)x * ldh]) : 0.0;
-    a[b] = v;
-    part += (i == j ? 1.0 : 2.0) * v * v;
+    a[q] = v;
+    // weight of the entry in |H|_F^2: an off-diagonal block holds four entries of the upper triangle;
+    // a diagonal block two diagonal entries, one off-diagonal entry and its duplicate
+    const double w = i != j ? 2.0 : (e == 1 ? 2.0 : (e == 2 ? 0.0 : 1.0));
+    part += w * v * v;
   }
   for (int o = 32; o > 0; o >>= 1) part += __shfl_down(part, o);
   if ((t & 63) == 0) red[t >> 6] = part;
@@ -120,29 +140,25 @@ __global__ __launch_bounds__(JR_NT) void jacobi_rot_kernel(const double* __restr
   // eigenvalues to their own scale.
   const double abs_floor = 4.5e-16 * fro_s + 1e-300;
   const double rel_tol2 = 1e-30;   // (1e-15)^2
-  // this thread's blocks: unordered slot pairs (i <= j), packed i | j << 16 (-1: none), with the
-  // addresses of their four entries (a_i a_j, a_i b_j, b_i a_j, b_i b_j) now and one round later,
-  // packed now | later << 16 (the packed triangle has fewer than 2^16 entries)
-  int bij[JB_MAX], adr[JB_MAX][4];
-  {
-    const int nblk = P * (P + 1) / 2;
+  // this thread's blocks b = t + u JR_NT: unordered slot pairs (i <= j), packed i | j << 16 (-1: none);
+  // their entries are read from a[e nblk + b] and written one round later to the plane addresses of
+  // the moved positions, packed two to an int (4 nblk < 2^16)
+  int bij[JB_MAX], wadr[JB_MAX][2];
 #pragma unroll
-    for (int u = 0; u < JB_MAX; ++u) {
-      const int b = t + u * JR_NT;
-      int i = 0, j = 0;
-      if (b < nblk) decode_tri(b, i, j);
-      bij[u] = (b < nblk) ? (i | (j << 16)) : -1;
-      const int pr[2] = {2 * i, 2 * i + 1}, pc[2] = {2 * j, 2 * j + 1};
+  for (int u = 0; u < JB_MAX; ++u) {
+    const int b = t + u * JR_NT;
+    int i = 0, j = 0;
+    if (b < nblk) decode_tri(b, i, j);
+    bij[u] = (b < nblk) ? (i | (j << 16)) : -1;
+    int w[4];
 #pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        const int x = pr[e >> 1], y = pc[e & 1];
-        const int xn = jpos_next(x, P), yn = jpos_next(y, P);
-        adr[u][e] = pk(min(x, y), max(x, y)) | (pk(min(xn, yn), max(xn, yn)) << 16);
-      }
-    }
+    for (int e = 0; e < 4; ++e)
+      w[e] = jplane_adr(jpos_next(2 * i + (e >> 1), P), jpos_next(2 * j + (e & 1), P), nblk);
+    wadr[u][0] = w[0] | (w[1] << 16);
+    wadr[u][1] = w[2] | (w[3] << 16);
   }
-  // rotation slot of this thread (t < P): its positions' addresses are fixed too
-  const int s_aa = pk(2 * t, 2 * t), s_ab = pk(2 * t, 2 * t + 1), s_bb = pk(2 * t + 1, 2 * t + 1);
+  // rotation slot of this thread (t < P): its diagonal block's addresses are fixed too
+  const int s_aa = jblk(t, t), s_ab = nblk + jblk(t, t), s_bb = 3 * nblk + jblk(t, t);
 
   int round = 0;
   int sweep = 0;
@@ -183,8 +199,9 @@ __global__ __launch_bounds__(JR_NT) void jacobi_rot_kernel(const double* __restr
       double x[JB_MAX][4];
 #pragma unroll
       for (int u = 0; u < JB_MAX; ++u) {
+        const int b = bij[u] < 0 ? 0 : t + u * JR_NT;
 #pragma unroll
-        for (int e = 0; e < 4; ++e) x[u][e] = a[adr[u][e] & 0xffff];
+        for (int e = 0; e < 4; ++e) x[u][e] = a[e * nblk + b];
       }
 #pragma unroll
       for (int u = 0; u < JB_MAX; ++u) {
@@ -192,8 +209,9 @@ __global__ __launch_bounds__(JR_NT) void jacobi_rot_kernel(const double* __restr
         const double2 csi = rcs[bb & 0xffff], csj = rcs[bb >> 16];
         const double ci = csi.x, si = csi.y, cj = csj.x, sj = csj.y;
         const bool dgn = (bb & 0xffff) == (bb >> 16);
-        const double ypr = ci * x[u][0] - si * x[u][2], yps = ci * x[u][1] - si * x[u][3];
-        const double yqr = si * x[u][0] + ci * x[u][2], yqs = si * x[u][1] + ci * x[u][3];
+        const double x2 = dgn ? x[u][1] : x[u][2];   // a diagonal block's b a word is not kept
+        const double ypr = ci * x[u][0] - si * x2, yps = ci * x[u][1] - si * x[u][3];
+        const double yqr = si * x[u][0] + ci * x2, yqs = si * x[u][1] + ci * x[u][3];
         x[u][0] = cj * ypr - sj * yps;
         x[u][1] = dgn ? 0.0 : sj * ypr + cj * yps;
         x[u][2] = dgn ? 0.0 : cj * yqr - sj * yqs;
@@ -203,8 +221,12 @@ __global__ __launch_bounds__(JR_NT) void jacobi_rot_kernel(const double* __restr
 #pragma unroll
       for (int u = 0; u < JB_MAX; ++u) {
         if (bij[u] < 0) continue;
-#pragma unroll
-        for (int e = 0; e < 4; ++e) a[(unsigned)adr[u][e] >> 16] = x[u][e];
+        // a diagonal block's two off-diagonal words are the same entry (zero): write it once
+        const bool dgn = (bij[u] & 0xffff) == (bij[u] >> 16);
+        a[wadr[u][0] & 0xffff] = x[u][0];
+        a[(unsigned)wadr[u][0] >> 16] = x[u][1];
+        if (!dgn) a[wadr[u][1] & 0xffff] = x[u][2];
+        a[(unsigned)wadr[u][1] >> 16] = x[u][3];
       }
       __syncthreads();
     }
@@ -218,7 +240,7 @@ __global__ __launch_bounds__(JR_NT) void jacobi_rot_kernel(const double* __restr
     else if (i == M) pos = 1;
     else if (i < P) pos = 2 * i;
     else pos = 2 * (M - i) + 1;
-    diag[i] = a[pk(pos, pos)];
+    diag[i] = a[((pos & 1) ? 3 * nblk : 0) + jblk(pos >> 1, pos >> 1)];
   }
   if (t == 0) {
     *rounds_out = round;
@@ -755,9 +777,9 @@ __global__ __launch_bounds__(EIG_NT) void ritz_select_kernel(const double* __res
 }
 }  // namespace
 
-static size_t jacobi_lds(int m) {   // packed mm x mm triangle
-  const int mm = m + (m & 1);
-  return (size_t)(mm * (mm + 1) / 2) * sizeof(double);
+static size_t jacobi_lds(int m) {   // four planes of P (P + 1) / 2 blocks, P = ceil(m / 2)
+  const int P = (m + 1) / 2;
+  return (size_t)(4 * (P * (P + 1) / 2)) * sizeof(double);
 }
 
 size_t jacobi_work_doubles(int m, int max_sweeps) {

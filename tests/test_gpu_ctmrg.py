@@ -43,7 +43,8 @@ def test_dimension_mismatch_is_an_error(ctx):
 
 
 # --- eigensolver ---------------------------------------------------------------------
-@pytest.mark.parametrize("n", [7, 80, 192])
+# n <= 2 is the one-slot case of the round-robin ordering; odd n is padded with a zero row and column
+@pytest.mark.parametrize("n", [1, 2, 3, 4, 7, 80, 191, 192])
 def test_symeig_full_matches_lapack(ctx, n):
     _say(f"case: symeig full n={n}")
     rng = np.random.default_rng(n)
