@@ -461,10 +461,11 @@ static void ctmrg_back_impl(tnad_ctx* ctx, tnad_tape* t, const double* Cbar_dev,
   const int ns = (int)t->steps.size();
   const int first = (t->opts.bwd_last > 0 && t->opts.bwd_last < ns) ? ns - t->opts.bwd_last : 0;
   const Prec bp = to_prec_enum(t->opts.bwd_precision);
+  double sweep_ref = 0.0;   // largest resolvent right-hand side of the sweep so far
   for (int i = ns - 1; i >= first; --i) {
     const PBuf& ap = ap_at(c, *t, bp);
     ctmrg_step_back(c, d, bp, *t->steps[i], t->a64.as<double>(), ap, Cb.as<double>(), Tb.as<double>(),
-                    Cb2.as<double>(), Tb2.as<double>(), abar_dev);
+                    Cb2.as<double>(), Tb2.as<double>(), abar_dev, &sweep_ref);
     std::swap(Cb, Cb2);
     std::swap(Tb, Tb2);
   }

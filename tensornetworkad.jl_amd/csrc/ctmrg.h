@@ -98,11 +98,15 @@ void ctmrg_step(Ctx* c, const StepDims& d, const StepOpts& o, const double* a64,
                 const double* C, const double* T, const double* X0, double* X_keep, double* Cout, double* Tout,
                 std::vector<double>& vals, StepTape* tape, StepState& state);
 
+// sweep_ref (nullable): running reference of the reverse sweep for the resolvent solve (see
+// resolvent_pcg); a step computed at a lower contraction precision is solved to the same absolute
+// accuracy as the sweep's largest right-hand side.
 void eig_adjoint(Ctx* c, const StepDims& d, const StepTape& tp, const double* G, const double* C1bar,
-                 double* cpbar, EigBackStats* st);
+                 double* cpbar, EigBackStats* st, double* sweep_ref = nullptr);
 
 void ctmrg_step_back(Ctx* c, const StepDims& d, Prec p, const StepTape& tp, const double* a64, const PBuf& ap,
-                     const double* C3bar, const double* T3bar, double* Cbar, double* Tbar, double* abar);
+                     const double* C3bar, const double* T3bar, double* Cbar, double* Tbar, double* abar,
+                     double* sweep_ref = nullptr);
 
 struct EvCache {
   DevBuf apn, E1, E2, E3, E4, l, tr, scal;

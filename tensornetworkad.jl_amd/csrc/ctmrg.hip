@@ -296,9 +296,36 @@ static void orthonormalize(Ctx* c, double* Y, int n, int k, double* X, EigWork& 
   // rank-deficient corner matrix) are replaced by random directions first, so the block always spans
   // k dimensions.
   std::vector<double> piv(k);
+  std::vector<int> dead(k);
   const double u = 1.1102230246251565e-16;
   const double shift = 11.0 * ((double)n * k + (double)k * (k + 1)) * u * (double)k;
-  {
+  // a column with less than 1% of its (unit) norm outside the span of the earlier ones is replaced:
+  // it adds next to no new direction, and keeping it would leave the block so ill-conditioned that
+  // plain passes could break down
+  const double dead_thresh = std::max(2.0 * std::sqrt(shift), 1e-2);
+  // shifted Cholesky of the Gram matrix of Y: its pivots flag the dependent columns (returns how many)
+  auto find_dead = [&](unsigned fill_seed, bool refill) {
+    tc_launch(c, Prec::F64, gemm_desc('T', 'N', k, k, n, n, n, k), Y, Y, w.G.as<double>(), s);
+    chol_inv(w.G.as<double>(), k, shift, w.Rinv.as<double>(), w.info.as<int>(), w.res.as<double>(), s);
+    c->read_async(piv.data(), w.res.ptr, (size_t)k * 8);
+    c->sync();
+    int ndead = 0;
+    for (int j = 0; j < k; ++j) {
+      dead[j] = !(piv[j] > dead_thresh);
+      ndead += dead[j];
+    }
+    if (ndead > 0 && refill) {
+      c->write_async(w.info.ptr, dead.data(), (size_t)k * sizeof(int));
+      k_random_fill_flagged(Y, n, k, w.info.as<int>(), fill_seed, s);
+      k_col_norms(Y, n, k, w.nrm.as<double>(), s);
+      k_scale_cols(Y, n, k, w.nrm.as<double>(), 1, s);
+    }
+    return ndead;
+  };
+  // Plain CholeskyQR as described above; when its pivots are too small, the shifted Cholesky finds
+  // the dependent columns, they are refilled, and the plain passes are tried once more (a block whose
+  // only defect was those columns is now well-conditioned).  What still fails takes sCholQR3 below.
+  for (int round = 0; round < 2; ++round) {
     double* scal = w.scal.as<double>();   // [0] smallest pivot, [1] |X^T X - I|_F^2
     tc_launch(c, Prec::F64, gemm_desc('T', 'N', k, k, n, n, n, k), Y, Y, w.G.as<double>(), s);
     chol_inv(w.G.as<double>(), k, 0.0, w.Rinv.as<double>(), w.info.as<int>(), nullptr, s, scal);
@@ -325,31 +352,19 @@ static void orthonormalize(Ctx* c, double* Y, int n, int k, double* X, EigWork& 
       k_copy(Y, X, (size_t)n * k, s);
       return;
     }
+    if (round == 1) break;
+    const int ndead = find_dead(seed + 7919u, true);
+    if (std::getenv("TNAD_DEBUG_EIG"))
+      std::fprintf(stderr, "[tnad eig] shifted CholeskyQR: %d dependent columns refilled\n", ndead);
+    if (ndead == 0) break;   // ill-conditioned without dependent columns: sCholQR3
   }
-  std::vector<int> dead(k);
   // the Cholesky factor used below must belong to the block as it finally is: the last attempt
   // factors without refilling afterwards
   for (int attempt = 0;; ++attempt) {
-    tc_launch(c, Prec::F64, gemm_desc('T', 'N', k, k, n, n, n, k), Y, Y, w.G.as<double>(), s);
-    chol_inv(w.G.as<double>(), k, shift, w.Rinv.as<double>(), w.info.as<int>(), w.res.as<double>(), s);
-    c->read_async(piv.data(), w.res.ptr, (size_t)k * 8);
-    c->sync();
-    // a column with less than 1% of its (unit) norm outside the span of the earlier ones is
-    // replaced: it adds next to no new direction, and keeping it would leave the block so
-    // ill-conditioned that the plain passes below could break down
-    const double thresh = std::max(2.0 * std::sqrt(shift), 1e-2);
-    int ndead = 0;
-    for (int j = 0; j < k; ++j) {
-      dead[j] = !(piv[j] > thresh);
-      ndead += dead[j];
-    }
+    const int ndead = find_dead(seed + 7919u * (attempt + 2), attempt < 3);
     if (std::getenv("TNAD_DEBUG_EIG"))
       std::fprintf(stderr, "[tnad eig] shifted CholeskyQR attempt %d: %d dependent columns\n", attempt, ndead);
     if (ndead == 0 || attempt == 3) break;
-    c->write_async(w.info.ptr, dead.data(), (size_t)k * sizeof(int));
-    k_random_fill_flagged(Y, n, k, w.info.as<int>(), seed + 7919u * (attempt + 1), s);
-    k_col_norms(Y, n, k, w.nrm.as<double>(), s);
-    k_scale_cols(Y, n, k, w.nrm.as<double>(), 1, s);
   }
   dbg_finite(c, "shifted Rinv", w.Rinv.as<double>(), (size_t)k * k);
   tc_launch(c, Prec::F64, gemm_desc('N', 'N', n, k, k, n, k, n), Y, w.Rinv.as<double>(), X, s);
@@ -609,7 +624,7 @@ void ctmrg_step(Ctx* c, const StepDims& d, const StepOpts& o, const double* a64,
 // inverse and the solve ends after its first step.
 // ==========================================================================
 void eig_adjoint(Ctx* c, const StepDims& d, const StepTape& tp, const double* G, const double* C1bar,
-                 double* cpbar, EigBackStats* st) {
+                 double* cpbar, EigBackStats* st, double* sweep_ref) {
   hipStream_t s = c->stream;
   const int chi = d.chi, n = d.n, k = tp.k;
   const double* X = tp.X.as<double>();
@@ -632,8 +647,14 @@ void eig_adjoint(Ctx* c, const StepDims& d, const StepTape& tp, const double* G,
       tc_launch(c, Prec::F64, gemm_desc('N', 'N', n, chi, n, n, n, n), M, V, tmp.as<double>(), s);
       tc_launch(c, Prec::F64, gemm_desc('N', 'N', n, chi, n, n, n, n), M, tmp.as<double>(), out, s);
     };
+    // A step computed with low-precision contractions resolved its eigenpairs only to the relative
+    // residual its contractions can support (1e-8 and above), so this adjoint is no more accurate
+    // than that whatever the solve does: its complement part is solved to 1e-10 instead of 1e-13, and
+    // no further than 1e-13 of the reverse sweep's largest right-hand side (the early steps of a
+    // converging CTMRG receive cotangents many orders smaller than the last ones).
+    const bool low = tp.prec != Prec::F64;
     iters = resolvent_pcg(c, n, chi, k, X, sq.as<double>(), sq.as<double>() + chi, applyA2, G, Xc.as<double>(),
-                          1e-13, 200);
+                          low ? 1e-10 : 1e-13, 200, 1e-40, sweep_ref, low ? 1e-13 : 0.0);
     // W += X_c Lambda1 + A X_c
     tc_launch(c, Prec::F64, gemm_desc('N', 'N', n, chi, n, n, n, n), M, Xc.as<double>(), AXc.as<double>(), s);
     hipLaunchKernelGGL(add_lam_cols_kernel, dim3(nb((size_t)n * chi)), dim3(NT), 0, s, W.as<double>(),
@@ -652,7 +673,8 @@ void eig_adjoint(Ctx* c, const StepDims& d, const StepTape& tp, const double* G,
 // bulk contribution added into abar.
 // ==========================================================================
 void ctmrg_step_back(Ctx* c, const StepDims& d, Prec p, const StepTape& tp, const double* a64, const PBuf& ap,
-                     const double* C3bar, const double* T3bar, double* Cbar, double* Tbar, double* abar) {
+                     const double* C3bar, const double* T3bar, double* Cbar, double* Tbar, double* abar,
+                     double* sweep_ref) {
   hipStream_t s = c->stream;
   const int chi = d.chi, D = d.D, n = d.n;
   const long long lc = chi, lD = D;
@@ -727,7 +749,7 @@ void ctmrg_step_back(Ctx* c, const StepDims& d, Prec p, const StepTape& tp, cons
 
   // --- eigen adjoint -> cpbar (n x n, FP64) ---------------------------------
   DevBuf cpbar(c, (size_t)n * n * 8);
-  eig_adjoint(c, d, tp, zbar.as<double>(), C1bar.as<double>(), cpbar.as<double>(), nullptr);
+  eig_adjoint(c, d, tp, zbar.as<double>(), C1bar.as<double>(), cpbar.as<double>(), nullptr, sweep_ref);
 
   // --- cp chain adjoint -----------------------------------------------------
   PBuf cpb(c, p, (size_t)n * n), X2b(c, p, (size_t)n * n), X1(c, p, (size_t)n * chi), X1b(c, p, (size_t)n * chi);

@@ -1,10 +1,11 @@
 // Small dense FP64 kernels for the eigensolver, one workgroup each (m <= 192).
 //
 //  * jacobi_rot_kernel / jacobi_apply_kernel : cyclic two-sided Jacobi with the
-//    round-robin (circle) ordering on a packed triangle in LDS (192x192 FP64 =
-//    148 KB of the 160 KB a CU offers), so each round applies m/2 disjoint
-//    rotations in parallel and every 2x2 block is updated by one thread; the
-//    rotations are recorded and replayed on the identity by many workgroups.
+//    round-robin (circle) ordering on the upper triangle of 2x2 blocks in LDS
+//    (192x192 FP64 = 149 KB of the 160 KB a CU offers), so each round applies
+//    m/2 disjoint rotations in parallel and every 2x2 block is updated by one
+//    thread; the rotations are recorded and replayed on the identity by many
+//    workgroups.
 //    Used for the full spectrum when n <= 192 and for the Rayleigh-Ritz problem
 //    of the block subspace iteration.
 //  * ritz_select_kernel : Rayleigh-Ritz of a nearly diagonal matrix: a few Jacobi
@@ -92,7 +93,65 @@ __device__ __forceinline__ int jpos_index0(int pos, int M) {   // index at a pos
   if (pos == 1) return M;
   return (pos & 1) ? M - t : t;
 }
+__device__ __forceinline__ int jpos_prev(int pos, int P) {   // position of an index one round earlier
+  if (P == 1) return pos;
+  if ((pos & 1) == 0) return pos == 0 ? 2 : (pos == 2 * P - 2 ? 2 * P - 1 : pos + 2);
+  return pos == 1 ? 1 : (pos == 3 ? 0 : pos - 2);
+}
 
+constexpr int JR_MAX_SWEEPS = 64;
+
+// Rotation of one slot from its 2x2 diagonal block; returns false (c = 1, s = 0) below the thresholds.
+// tan of the rotation angle (|angle| <= pi/4): tt = sign(d) e / (|d| + hypot(d, e)) with d = abb - aaa,
+// e = 2 aab; then c = 1 / sqrt(1 + tt^2), s = tt c, formed without tt.
+__device__ __forceinline__ bool jrot_cs(double aaa, double aab, double abb, double abs_floor, double& c,
+                                        double& sn) {
+  const double rel_tol2 = 1e-30;   // (1e-15)^2
+  c = 1.0;
+  sn = 0.0;
+  if (!(fabs(aab) > abs_floor && aab * aab > rel_tol2 * fabs(aaa * abb))) return false;
+  const double d = abb - aaa, e = 2.0 * aab;
+  const double den = fabs(d) + sqrt(d * d + e * e);
+  const double inv = 1.0 / sqrt(den * den + e * e);
+  c = den * inv;
+  sn = (d >= 0.0 ? e : -e) * inv;
+  return true;
+}
+
+// A' = Ji^T A Jj on a 2x2 block (a_i a_j, a_i b_j, b_i a_j, b_i b_j).  A diagonal block (i == j)
+// holds its off-diagonal entry in x[1] only; its rotated off-diagonal entry is set to zero.
+__device__ __forceinline__ void jblock_update(double (&x)[4], double ci, double si, double cj, double sj,
+                                              bool dgn) {
+  const double x2 = dgn ? x[1] : x[2];
+  const double ypr = ci * x[0] - si * x2, yps = ci * x[1] - si * x[3];
+  const double yqr = si * x[0] + ci * x2, yqs = si * x[1] + ci * x[3];
+  x[0] = cj * ypr - sj * yps;
+  x[1] = dgn ? 0.0 : sj * ypr + cj * yps;
+  x[2] = dgn ? 0.0 : cj * yqr - sj * yqs;
+  x[3] = sj * yqr + cj * yqs;
+}
+
+// record of slot t's rotation in global round R, for the sorted index pair: (c, s) on (a, b) is
+// (c, -s) on (b, a)
+__device__ __forceinline__ void jrot_record(double* rot, int R, int t, int M, int P, double c, double sn) {
+  const int r = R % M;
+  int ia, ib;
+  if (t == 0) { ia = r; ib = M; }
+  else {
+    ia = r + t;
+    if (ia >= M) ia -= M;
+    ib = r - t;
+    if (ib < 0) ib += M;
+  }
+  reinterpret_cast<double2*>(rot)[(size_t)R * P + t] = make_double2(c, ia < ib ? sn : -sn);
+}
+
+// NEXT: round R + 1's rotations are formed during round R (two barriers per round) by threads that own
+// one block fewer than the busiest; otherwise (three barriers per round) the first P threads form each
+// round's rotations before its block updates.  NEXT pays where the slowest threads own JB_MAX blocks
+// and the rotation threads JB_MAX - 1 (P (P + 1) / 2 > (JB_MAX - 1) JR_NT), so the rotation work hides
+// behind their block updates.
+template <bool NEXT>
 __global__ __launch_bounds__(JR_NT) void jacobi_rot_kernel(const double* __restrict__ H, int ldh, int m,
                                                             double* __restrict__ rot, int max_sweeps,
                                                             double* __restrict__ diag, int* rounds_out,
@@ -104,11 +163,13 @@ __global__ __launch_bounds__(JR_NT) void jacobi_rot_kernel(const double* __restr
   const int M = mm - 1;
   const int nblk = P * (P + 1) / 2;
   double* a = smem;                              // 4 planes of nblk blocks, position order
-  __shared__ double2 rcs[JACOBI_MAX / 2];        // (cos, sin) per slot (apart from a: no aliasing)
+  __shared__ double2 rcs[2][JACOBI_MAX / 2];     // (cos, sin) per slot, this round and the next
   __shared__ double red[JR_NT / 64];
-  __shared__ int nrot_s;
+  __shared__ int nrot_sw[JR_MAX_SWEEPS];         // 1 if sweep k rotates at all
+  __shared__ int4 pent_s[JACOBI_MAX / 2];        // next-round diagonal entries per rotation slot
   __shared__ double fro_s;
   const int t = threadIdx.x;
+  for (int k = t; k < JR_MAX_SWEEPS; k += JR_NT) nrot_sw[k] = 0;
 
   // load (position order of round 0) and symmetrise (average both triangles so a slightly
   // asymmetric input is still handled)
@@ -139,7 +200,6 @@ __global__ __launch_bounds__(JR_NT) void jacobi_rot_kernel(const double* __restr
   // whose spectrum spans many decades.  Above that, the relative criterion resolves small
   // eigenvalues to their own scale.
   const double abs_floor = 4.5e-16 * fro_s + 1e-300;
-  const double rel_tol2 = 1e-30;   // (1e-15)^2
   // this thread's blocks b = t + u JR_NT: unordered slot pairs (i <= j), packed i | j << 16 (-1: none);
   // their entries are read from a[e nblk + b] and written one round later to the plane addresses of
   // the moved positions, packed two to an int (4 nblk < 2^16)
@@ -157,45 +217,79 @@ __global__ __launch_bounds__(JR_NT) void jacobi_rot_kernel(const double* __restr
     wadr[u][0] = w[0] | (w[1] << 16);
     wadr[u][1] = w[2] | (w[3] << 16);
   }
-  // rotation slot of this thread (t < P): its diagonal block's addresses are fixed too
-  const int s_aa = jblk(t, t), s_ab = nblk + jblk(t, t), s_bb = 3 * nblk + jblk(t, t);
+  // Rotations of round R + 1 are formed during round R by the last P threads (which own at most
+  // JB_MAX - 1 blocks), from round R's blocks and rotations: slot s of round R + 1 pairs the indices
+  // now at positions xo = jpos_prev(2 s) and yo = jpos_prev(2 s + 1), so its diagonal block is the
+  // rotated entries (xo, xo), (xo, yo), (yo, yo) of this round.  Each entry is packed as block number
+  // | element << 13 | slot of its row << 15 | slot of its column << 22 | diagonal block << 29 |
+  // always zero << 30 (the off-diagonal entry of a diagonal block).
+  const int ts = NEXT ? t - (JR_NT - P) : -1;   // rotation slot of this thread, or negative
+  if (ts >= 0) {
+    int pent[3];
+    const int xo = jpos_prev(2 * ts, P), yo = jpos_prev(2 * ts + 1, P);
+    const int pq[3][2] = {{xo, xo}, {xo, yo}, {yo, yo}};
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      int x = pq[k][0], y = pq[k][1];
+      if (x > y) { const int tmp = x; x = y; y = tmp; }
+      const int i = x >> 1, j = y >> 1;
+      const int zero = (i == j && x != y) ? 1 : 0;
+      pent[k] = jblk(i, j) | ((2 * (x & 1) + (y & 1)) << 13) | (i << 15) | (j << 22) | ((i == j) << 29) |
+                (zero << 30);
+    }
+    pent_s[ts] = make_int4(pent[0], pent[1], pent[2], 0);
+  }
 
+  // round 0's rotations
+  if (NEXT && t < P) {
+    const int bd = jblk(t, t);
+    double c, sn;
+    if (jrot_cs(a[bd], a[nblk + bd], a[3 * nblk + bd], abs_floor, c, sn)) nrot_sw[0] = 1;
+    rcs[0][t] = make_double2(c, sn);
+    jrot_record(rot, 0, t, M, P, c, sn);
+  }
+  __syncthreads();
+
+  const int total_rounds = max_sweeps * M;
   int round = 0;
   int sweep = 0;
+  int cur = 0;
   for (; sweep < max_sweeps; ++sweep) {
-    if (t == 0) nrot_s = 0;
-    __syncthreads();
     for (int r = 0; r < M; ++r, ++round) {
-      if (t < P) {
-        double c = 1.0, sn = 0.0;
-        const double aab = a[s_ab], aaa = a[s_aa], abb = a[s_bb];
-        if (fabs(aab) > abs_floor && aab * aab > rel_tol2 * fabs(aaa * abb)) {
-          // tan of the rotation angle (|angle| <= pi/4): tt = sign(d) e / (|d| + hypot(d, e)) with
-          // d = abb - aaa, e = 2 aab; then c = 1 / sqrt(1 + tt^2), s = tt c, formed without tt
-          const double d = abb - aaa, e = 2.0 * aab;
-          const double den = fabs(d) + sqrt(d * d + e * e);
-          const double inv = 1.0 / sqrt(den * den + e * e);
-          c = den * inv;
-          sn = (d >= 0.0 ? e : -e) * inv;
-          nrot_s = 1;   // every writer stores the same value
+      if (!NEXT) {
+        if (t < P) {
+          const int bd = jblk(t, t);
+          double c, sn;
+          if (jrot_cs(a[bd], a[nblk + bd], a[3 * nblk + bd], abs_floor, c, sn)) nrot_sw[sweep] = 1;
+          rcs[0][t] = make_double2(c, sn);
+          jrot_record(rot, round, t, M, P, c, sn);
         }
-        rcs[t] = make_double2(c, sn);
-        // the record is for the sorted index pair: (c, s) on (a, b) is (c, -s) on (b, a)
-        int ia, ib;
-        if (t == 0) { ia = r; ib = M; }
-        else {
-          ia = r + t;
-          if (ia >= M) ia -= M;
-          ib = r - t;
-          if (ib < 0) ib += M;
-        }
-        reinterpret_cast<double2*>(rot)[(size_t)round * P + t] = make_double2(c, ia < ib ? sn : -sn);
+        __syncthreads();
       }
-      __syncthreads();
+      if (ts >= 0 && round + 1 < total_rounds) {
+        const int4 p4 = pent_s[ts];
+        const int pent[3] = {p4.x, p4.y, p4.z};
+        double v[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+          const int pe = pent[k];
+          const int b = pe & 0x1fff, el = (pe >> 13) & 3, i = (pe >> 15) & 0x7f, j = (pe >> 22) & 0x7f;
+          const bool dgn = (pe >> 29) & 1;
+          double y[4];
+#pragma unroll
+          for (int e = 0; e < 4; ++e) y[e] = a[e * nblk + b];
+          const double2 csi = rcs[cur][i], csj = rcs[cur][j];
+          jblock_update(y, csi.x, csi.y, csj.x, csj.y, dgn);
+          v[k] = ((pe >> 30) & 1) ? 0.0 : (el == 0 ? y[0] : (el == 1 ? y[1] : (el == 2 ? y[2] : y[3])));
+        }
+        double c, sn;
+        if (jrot_cs(v[0], v[1], v[2], abs_floor, c, sn)) nrot_sw[(round + 1) / M] = 1;
+        rcs[cur ^ 1][ts] = make_double2(c, sn);
+        jrot_record(rot, round + 1, ts, M, P, c, sn);
+      }
       // 2x2 block updates A' = Ji^T A Jj over unordered pairs of rotation slots.  Every entry belongs
       // to exactly one block; all blocks are read, then (after a barrier, since the write-back goes
-      // to next round's positions) written.  A diagonal block (i == j) holds (a, b) twice: the same
-      // formula gives its rotated diagonal, and its off-diagonal entry is set to zero.
+      // to next round's positions) written.
       double x[JB_MAX][4];
 #pragma unroll
       for (int u = 0; u < JB_MAX; ++u) {
@@ -206,16 +300,15 @@ __global__ __launch_bounds__(JR_NT) void jacobi_rot_kernel(const double* __restr
 #pragma unroll
       for (int u = 0; u < JB_MAX; ++u) {
         const int bb = bij[u] < 0 ? 0 : bij[u];
-        const double2 csi = rcs[bb & 0xffff], csj = rcs[bb >> 16];
-        const double ci = csi.x, si = csi.y, cj = csj.x, sj = csj.y;
-        const bool dgn = (bb & 0xffff) == (bb >> 16);
-        const double x2 = dgn ? x[u][1] : x[u][2];   // a diagonal block's b a word is not kept
-        const double ypr = ci * x[u][0] - si * x2, yps = ci * x[u][1] - si * x[u][3];
-        const double yqr = si * x[u][0] + ci * x2, yqs = si * x[u][1] + ci * x[u][3];
-        x[u][0] = cj * ypr - sj * yps;
-        x[u][1] = dgn ? 0.0 : sj * ypr + cj * yps;
-        x[u][2] = dgn ? 0.0 : cj * yqr - sj * yqs;
-        x[u][3] = sj * yqr + cj * yqs;
+        const double2 csi = rcs[cur][bb & 0xffff], csj = rcs[cur][bb >> 16];
+        jblock_update(x[u], csi.x, csi.y, csj.x, csj.y, (bb & 0xffff) == (bb >> 16));
+      }
+      // the rotated entries are formed before the barrier: only they (not the entries and both
+      // rotations of every block) stay live across it, which keeps the kernel within its registers
+#pragma unroll
+      for (int u = 0; u < JB_MAX; ++u) {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) asm volatile("" : "+v"(x[u][e]));
       }
       __syncthreads();
 #pragma unroll
@@ -229,9 +322,10 @@ __global__ __launch_bounds__(JR_NT) void jacobi_rot_kernel(const double* __restr
         a[(unsigned)wadr[u][1] >> 16] = x[u][3];
       }
       __syncthreads();
+      if (NEXT) cur ^= 1;
     }
-    if (nrot_s == 0) { ++sweep; break; }
-    __syncthreads();
+    // every flag of this sweep was set before the last round's first barrier
+    if (nrot_sw[sweep] == 0) { ++sweep; break; }
   }
   // whole sweeps ran: every index is back at its round-0 position
   for (int i = t; i < m; i += JR_NT) {
@@ -353,12 +447,16 @@ __global__ __launch_bounds__(JS_NT) void jacobi_sort_kernel(const double* __rest
 // Cholesky (with diagonal shift) + triangular inverse of an m x m SPD matrix, m <= 192,
 // in one workgroup of 32 x 32 threads.  Thread (ty, tx) holds the 6 x 6 elements
 // (ty + 32a, tx + 32b) in registers (cyclic distribution, so every step keeps all
-// threads busy); each elimination step broadcasts one column through LDS and costs
-// one barrier (double-buffered).  L is then parked in LDS (packed lower) and
-// B = L^-1 is formed by right-looking forward substitution with the same register
-// layout.  The steps run in six 32-column blocks whose block index is a template
-// parameter, so every register access is static and blocks that a step cannot
-// touch (finished columns, the upper triangle) are not even emitted.
+// threads busy).  One sweep of m elimination steps forms both L and B = L^-1: step jj
+// broadcasts column jj of the Schur complement and row jj of B (both through LDS, one
+// barrier per step, double-buffered); every thread then updates its part of the trailing
+// Schur complement (A -= l l^T) and of the rows of B below jj (B[i,:] -= L[i,jj] B[jj,:],
+// forward substitution for L B = I), and the owners finish column jj of L and row jj of B.
+// Only L's current column is ever needed, so finished columns of L are dropped, and the
+// rows of B in a 32-row block are final when the block's steps are done: they go straight
+// to the output.  The steps run in six 32-column blocks whose block index is a template
+// parameter, so every register access is static and blocks that a step cannot touch
+// (finished columns, the upper triangle) are not even emitted.
 // Output Rinv = L^-T (upper triangular, column-major, zeros below the diagonal) so
 // that X = Y Rinv; piv (nullable) receives diag(L).
 constexpr int CH_G = 32;                  // thread grid edge
@@ -366,16 +464,12 @@ constexpr int CH_R = JACOBI_MAX / CH_G;   // elements per thread per dimension
 static_assert(CH_R * CH_G == JACOBI_MAX, "Cholesky tiling");
 
 struct CholShared {
-  double* Lp;     // packed lower L, column-major
-  double* cb;     // 2 x JACOBI_MAX broadcast buffers
-  double* invd;   // 1 / diag(L) (JACOBI_MAX)
+  double* cb;     // 2 x JACOBI_MAX broadcast buffers: column jj of the Schur complement
+  double* rb;     // 2 x JACOBI_MAX broadcast buffers: row jj of B before its scaling
   int* bad;
   double* pmin;   // smallest pivot so far (thread 0 only)
 };
 
-__device__ __forceinline__ int chol_pl(int m, int i, int j) { return j * m - j * (j - 1) / 2 + (i - j); }
-
-// elimination steps jj = 32 BJ .. min(32 BJ + 31, m - 1)
 // 1 / sqrt(d) for a positive normal d: the hardware estimate refined by two Newton steps
 __device__ __forceinline__ double rsqrt_refined(double d) {
   double y = __builtin_amdgcn_rsq(d);
@@ -387,13 +481,17 @@ __device__ __forceinline__ double rsqrt_refined(double d) {
 // elimination steps jj = 32 BJ .. min(32 BJ + 31, m - 1).  The owner column broadcasts the whole
 // column jj (zeros above the diagonal and in the padding), so the update needs no bounds tests: its
 // only stray products land in row jj above the diagonal (the unused upper triangle) and in column
-// jj, which the owner rewrites.
+// jj, which the owner rewrites.  Likewise row jj of B is zero right of its diagonal, its stray
+// product lands in row jj itself (rewritten by its owner), and rows above jj meet a zero L[i,jj].
+// After the block's steps its rows of B are final and are written to Rinv.
 template <int BJ>
-__device__ __forceinline__ void chol_block(double (&A)[CH_R][CH_R], int m, const CholShared& sh, int tx, int ty,
-                                           int t, double* __restrict__ piv) {
+__device__ __forceinline__ void chol_block(double (&A)[CH_R][CH_R], double (&B)[CH_R][CH_R], int m,
+                                           const CholShared& sh, int tx, int ty, int t, double* __restrict__ piv,
+                                           double* __restrict__ Rinv) {
   const int j0 = BJ * CH_G, j1 = min(m, j0 + CH_G);
   for (int jj = j0; jj < j1; ++jj) {
     double* col = sh.cb + (jj & 1) * JACOBI_MAX;
+    double* row = sh.rb + (jj & 1) * JACOBI_MAX;
     const int tj = jj - j0;
     if (tx == tj) {
 #pragma unroll
@@ -401,6 +499,10 @@ __device__ __forceinline__ void chol_block(double (&A)[CH_R][CH_R], int m, const
         const int i = ty + CH_G * a;
         col[i] = (i >= jj && i < m) ? A[a][BJ] : 0.0;
       }
+    }
+    if (ty == tj) {
+#pragma unroll
+      for (int b = 0; b <= BJ; ++b) row[tx + CH_G * b] = B[BJ][b];
     }
     __syncthreads();
     double d = col[jj];
@@ -414,17 +516,24 @@ __device__ __forceinline__ void chol_block(double (&A)[CH_R][CH_R], int m, const
       if (piv) piv[jj] = sd;
       *sh.pmin = fmin(*sh.pmin, sd);
     }
-    double li[CH_R], lj[CH_R];
+    double li[CH_R], lj[CH_R], xr[CH_R];
 #pragma unroll
     for (int a = BJ; a < CH_R; ++a) {
       li[a] = col[ty + CH_G * a] * isd;
       lj[a] = col[tx + CH_G * a] * isd;
     }
-    // trailing lower blocks BJ <= b <= a
+#pragma unroll
+    for (int b = 0; b <= BJ; ++b) xr[b] = row[tx + CH_G * b] * isd;   // row jj of B, final
+    // trailing lower blocks BJ <= b <= a of the Schur complement
 #pragma unroll
     for (int a = BJ; a < CH_R; ++a)
 #pragma unroll
       for (int b = BJ; b <= a; ++b) A[a][b] -= li[a] * lj[b];
+    // rows of B below jj (blocks a >= BJ, columns b <= BJ)
+#pragma unroll
+    for (int a = BJ; a < CH_R; ++a)
+#pragma unroll
+      for (int b = 0; b <= BJ; ++b) B[a][b] -= li[a] * xr[b];
     if (tx == tj) {
 #pragma unroll
       for (int a = BJ; a < CH_R; ++a) {
@@ -432,39 +541,18 @@ __device__ __forceinline__ void chol_block(double (&A)[CH_R][CH_R], int m, const
         if (i >= jj) A[a][BJ] = (i == jj) ? sd : li[a];
       }
     }
-  }
-}
-
-// forward-substitution steps jj = 32 AJ .. min(32 AJ + 31, m - 1) of B = L^-1:
-// row jj = B[jj,:] / L[jj,jj] ; rows i > jj: B[i,:] -= L[i,jj] B[jj,:].  Row jj of B is zero right of
-// its diagonal, so its broadcast needs no mask (padding columns j >= m may pick up stale values; they
-// are never written out).
-template <int AJ>
-__device__ __forceinline__ void inv_block(double (&A)[CH_R][CH_R], int m, const CholShared& sh, int tx, int ty) {
-  const int j0 = AJ * CH_G, j1 = min(m, j0 + CH_G);
-  for (int jj = j0; jj < j1; ++jj) {
-    double* row = sh.cb + (jj & 1) * JACOBI_MAX;
-    const double ild = sh.invd[jj];
-    if (ty == jj - j0) {
+    if (ty == tj) {
 #pragma unroll
-      for (int b = 0; b <= AJ; ++b) {
-        const int j = tx + CH_G * b;
-        const double v = A[AJ][b] * ild;
-        A[AJ][b] = v;
-        if (j < m) row[j] = v;
-      }
+      for (int b = 0; b <= BJ; ++b) B[BJ][b] = xr[b];
     }
-    __syncthreads();
-    double xr[CH_R];
+  }
+  // rows i = ty + 32 BJ of B are final: Rinv(j, i) = B(i, j), zero above B's diagonal
+  const int i = ty + CH_G * BJ;
+  if (i < m) {
 #pragma unroll
-    for (int b = 0; b <= AJ; ++b) xr[b] = row[tx + CH_G * b];
-    const int colbase = jj * (m - 1) - jj * (jj - 1) / 2;   // chol_pl(m, i, jj) = colbase + i
-#pragma unroll
-    for (int a = AJ; a < CH_R; ++a) {
-      const int i = ty + CH_G * a;
-      const double lij = (i > jj && i < m) ? sh.Lp[colbase + i] : 0.0;
-#pragma unroll
-      for (int b = 0; b <= AJ; ++b) A[a][b] -= lij * xr[b];
+    for (int b = 0; b < CH_R; ++b) {
+      const int j = tx + CH_G * b;
+      if (j < m) Rinv[j + (size_t)m * i] = (b <= BJ && j <= i) ? B[BJ][b <= BJ ? b : 0] : 0.0;
     }
   }
 }
@@ -473,13 +561,12 @@ __global__ __launch_bounds__(CH_G * CH_G) void chol_inv_kernel(const double* __r
                                                                double shift, double* __restrict__ Rinv,
                                                                int* info, double* __restrict__ piv,
                                                                double* __restrict__ pmin_out) {
-  extern __shared__ double smem[];
+  __shared__ double bufs[4 * JACOBI_MAX];
   __shared__ int bad;
   __shared__ double pmin_s;
-  CholShared sh{smem, smem + (size_t)m * (m + 1) / 2, smem + (size_t)m * (m + 1) / 2 + 2 * JACOBI_MAX, &bad,
-                &pmin_s};
+  CholShared sh{bufs, bufs + 2 * JACOBI_MAX, &bad, &pmin_s};
   const int t = threadIdx.x, tx = t % CH_G, ty = t / CH_G;
-  double A[CH_R][CH_R];
+  double A[CH_R][CH_R], B[CH_R][CH_R];
 #pragma unroll
   for (int a = 0; a < CH_R; ++a) {
 #pragma unroll
@@ -488,47 +575,17 @@ __global__ __launch_bounds__(CH_G * CH_G) void chol_inv_kernel(const double* __r
       double v = (i == j) ? 1.0 : 0.0;   // padding beyond m: identity, never coupled to the rest
       if (i < m && j < m) v = G[j + (size_t)m * i] + (i == j ? shift : 0.0);
       A[a][b] = v;
+      B[a][b] = (i == j) ? 1.0 : 0.0;
     }
   }
   if (t == 0) { bad = 0; pmin_s = HUGE_VAL; }
   __syncthreads();
-  // ---- right-looking Cholesky, six 32-column blocks ----
-  chol_block<0>(A, m, sh, tx, ty, t, piv);
-  chol_block<1>(A, m, sh, tx, ty, t, piv);
-  chol_block<2>(A, m, sh, tx, ty, t, piv);
-  chol_block<3>(A, m, sh, tx, ty, t, piv);
-  chol_block<4>(A, m, sh, tx, ty, t, piv);
-  chol_block<5>(A, m, sh, tx, ty, t, piv);
-  // park L (lower part of A) in LDS
-#pragma unroll
-  for (int a = 0; a < CH_R; ++a)
-#pragma unroll
-    for (int b = 0; b <= a; ++b) {
-      const int i = ty + CH_G * a, j = tx + CH_G * b;
-      if (i < m && j <= i) sh.Lp[chol_pl(m, i, j)] = A[a][b];
-    }
-  __syncthreads();
-  for (int i = t; i < m; i += CH_G * CH_G) sh.invd[i] = 1.0 / sh.Lp[chol_pl(m, i, i)];
-  __syncthreads();
-  // ---- B = L^-1 ----
-#pragma unroll
-  for (int a = 0; a < CH_R; ++a)
-#pragma unroll
-    for (int b = 0; b < CH_R; ++b) A[a][b] = (ty + CH_G * a == tx + CH_G * b) ? 1.0 : 0.0;
-  inv_block<0>(A, m, sh, tx, ty);
-  inv_block<1>(A, m, sh, tx, ty);
-  inv_block<2>(A, m, sh, tx, ty);
-  inv_block<3>(A, m, sh, tx, ty);
-  inv_block<4>(A, m, sh, tx, ty);
-  inv_block<5>(A, m, sh, tx, ty);
-  // Rinv(r, c) = B(c, r)
-#pragma unroll
-  for (int a = 0; a < CH_R; ++a)
-#pragma unroll
-    for (int b = 0; b < CH_R; ++b) {
-      const int i = ty + CH_G * a, j = tx + CH_G * b;
-      if (i < m && j < m) Rinv[j + (size_t)m * i] = (j <= i) ? A[a][b] : 0.0;
-    }
+  chol_block<0>(A, B, m, sh, tx, ty, t, piv, Rinv);
+  chol_block<1>(A, B, m, sh, tx, ty, t, piv, Rinv);
+  chol_block<2>(A, B, m, sh, tx, ty, t, piv, Rinv);
+  chol_block<3>(A, B, m, sh, tx, ty, t, piv, Rinv);
+  chol_block<4>(A, B, m, sh, tx, ty, t, piv, Rinv);
+  chol_block<5>(A, B, m, sh, tx, ty, t, piv, Rinv);
   if (t == 0) {
     if (info) *info = bad;
     if (pmin_out) *pmin_out = pmin_s;
@@ -792,6 +849,7 @@ size_t jacobi_work_doubles(int m, int max_sweeps) {
 void jacobi_eig_ws(const double* H, int ldh, int m, double* evals, double* V, double* Vwork, int max_sweeps,
                    int* sweeps_out, hipStream_t s, const int* gate) {
   if (m < 1 || m > JACOBI_MAX) throw Error(TNAD_ERR_ARG, "jacobi_eig: size out of range");
+  if (max_sweeps < 1 || max_sweeps > JR_MAX_SWEEPS) throw Error(TNAD_ERR_ARG, "jacobi_eig: sweep limit out of range");
   const int mm = m + (m & 1);
   const size_t P = mm / 2;
   double* rot = Vwork;
@@ -799,10 +857,15 @@ void jacobi_eig_ws(const double* H, int ldh, int m, double* evals, double* V, do
   double* diag = Vu + (size_t)m * m;
   int* rounds = reinterpret_cast<int*>(diag + m);
   size_t lds = jacobi_lds(m);
-  TNAD_HIP_TRY(hipFuncSetAttribute((const void*)jacobi_rot_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                   (int)lds));
-  hipLaunchKernelGGL(jacobi_rot_kernel, dim3(1), dim3(JR_NT), lds, s, H, ldh, m, rot, max_sweeps, diag, rounds,
-                     sweeps_out, gate);
+  const bool next = (long long)P * (P + 1) / 2 > (long long)(JB_MAX - 1) * JR_NT;
+  const void* kern = next ? (const void*)jacobi_rot_kernel<true> : (const void*)jacobi_rot_kernel<false>;
+  TNAD_HIP_TRY(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  if (next)
+    hipLaunchKernelGGL(jacobi_rot_kernel<true>, dim3(1), dim3(JR_NT), lds, s, H, ldh, m, rot, max_sweeps, diag,
+                       rounds, sweeps_out, gate);
+  else
+    hipLaunchKernelGGL(jacobi_rot_kernel<false>, dim3(1), dim3(JR_NT), lds, s, H, ldh, m, rot, max_sweeps, diag,
+                       rounds, sweeps_out, gate);
   TNAD_HIP_TRY(hipGetLastError());
   hipLaunchKernelGGL(jacobi_apply_kernel, dim3((m + JA_WAVES - 1) / JA_WAVES), dim3(64 * JA_WAVES), 0, s, rot,
                      rounds, m, Vu, gate);
@@ -824,10 +887,7 @@ void ritz_select(const double* H, int ldh, int m, int nh, double thresh, double*
 void chol_inv(const double* G, int m, double shift, double* Rinv, int* info, double* piv, hipStream_t s,
               double* pmin) {
   if (m < 1 || m > JACOBI_MAX) throw Error(TNAD_ERR_ARG, "chol_inv: size out of range");
-  size_t lds = ((size_t)m * (m + 1) / 2 + 3 * JACOBI_MAX) * sizeof(double);
-  TNAD_HIP_TRY(hipFuncSetAttribute((const void*)chol_inv_kernel,
-                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-  hipLaunchKernelGGL(chol_inv_kernel, dim3(1), dim3(CH_G * CH_G), lds, s, G, m, shift, Rinv, info, piv, pmin);
+  hipLaunchKernelGGL(chol_inv_kernel, dim3(1), dim3(CH_G * CH_G), 0, s, G, m, shift, Rinv, info, piv, pmin);
   TNAD_HIP_TRY(hipGetLastError());
 }
 

@@ -22,11 +22,15 @@ namespace tnad {
 // below tol^2 times b.M^-1 b, or below 1e-30 |b_j|^2 / sig_j (the complement part of b
 // is then at the rounding level of the whole).  applyG(V, GV) must write G V for a
 // rows x r block V.
+// sweep_ref (nullable, host): the largest b.M^-1 b of any column in the solves so far (a reverse
+// sweep's running reference); this solve raises it to its own largest.  With abs_tol > 0 a column
+// also stops once r.M^-1 r <= abs_tol^2 * sweep_ref: the same absolute accuracy as the sweep's
+// largest right-hand side, for a solve whose right-hand side is much smaller.
 // All pointers are device FP64 arrays, column-major; sig (r) and tz (k - r) on device.
 // Returns the number of iterations.
 int resolvent_pcg(Ctx* c, int rows, int r, int k, const double* W, const double* sig, const double* tz,
                   const std::function<void(const double*, double*)>& applyG, const double* b, double* x,
-                  double tol, int maxit, double eta = 1e-40);
+                  double tol, int maxit, double eta = 1e-40, double* sweep_ref = nullptr, double abs_tol = 0.0);
 
 // out[i] = in[i]^2
 void k_square(const double* in, double* out, int n, hipStream_t s);
