@@ -463,8 +463,9 @@ int topk_eig(Ctx* c, const double* A, int n, int want, int k, const double* X0, 
     double mx = 0.0;
     for (int j = 0; j < want; ++j) mx = std::max(mx, hres[j]);
     rel = mx / std::max(std::fabs(hth[0]), 1e-300);
-    if (dbg) std::fprintf(stderr, "[tnad eig] n=%d k=%d want=%d it=%d rel_resid=%.3e theta0=%.6e rr=%s sweeps=%d\n",
-                          n, k, want, it, rel, hth[0], jac ? "jacobi" : "select", jac ? sweeps_done : 0);
+    if (dbg)
+      std::fprintf(stderr, "[tnad eig] n=%d k=%d want=%d it=%d rel_resid=%.3e theta0=%.6e rr=%s(%d) sweeps=%d\n", n,
+                   k, want, it, rel, hth[0], jac ? "jacobi" : "select", jac_forced ? -1 : flag, jac ? sweeps_done : 0);
     force_jacobi = !(rel < 0.9 * prev_rel);
     prev_rel = rel;
     if (!(rel == rel)) throw Error(TNAD_ERR_NAN, "eigensolver: NaN in Ritz residual");

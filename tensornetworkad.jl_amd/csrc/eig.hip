@@ -612,7 +612,8 @@ __global__ __launch_bounds__(CH_G * CH_G) void chol_inv_kernel(const double* __r
 //     a direction that belongs among the kept ones, which its Gershgorin bound (below the smallest
 //     kept |d_i| by 0.1%) rules out.
 // The pairs are written sorted by |theta| descending (stable) and *flag = 0.  When a limit of step 1
-// is hit or the Gershgorin test fails, *flag = 1 and nothing else is written (the caller falls back
+// is hit or the Gershgorin test fails, *flag > 0 names the reason (2: more strong pairs than the list
+// holds, 3: passes, 4: rotation budget, 5: Gershgorin) and nothing else is written (the caller falls back
 // to Jacobi).  The vectors are orthonormal to O((E/gap)^2); the subspace iteration's residual test
 // sees any remainder.
 constexpr int RS_LIST = 256;
@@ -672,7 +673,7 @@ __global__ __launch_bounds__(EIG_NT) void ritz_select_kernel(const double* __res
     const int cnt = cnt_s;
     if (cnt == 0) break;
     if (cnt > RS_LIST || pass == RS_PASSES) {
-      if (t == 0) bad_s = 1;
+      if (t == 0) bad_s = cnt > RS_LIST ? 2 : 3;
       break;
     }
     // the list fills in arrival order: sort it (keys are distinct) so that the selection, and with it
@@ -697,7 +698,7 @@ __global__ __launch_bounds__(EIG_NT) void ritz_select_kernel(const double* __res
         }
       }
       nsel_s = ns;
-      if (ns == 0) bad_s = 1;
+      if (ns == 0) bad_s = 4;
     }
     __syncthreads();
     if (bad_s) break;
@@ -755,7 +756,7 @@ __global__ __launch_bounds__(EIG_NT) void ritz_select_kernel(const double* __res
   }
   __syncthreads();
   if (bad_s) {
-    if (t == 0) *flag = 1;
+    if (t == 0) *flag = bad_s;
     return;
   }
   const int nrot = nrot_s;
@@ -777,8 +778,8 @@ __global__ __launch_bounds__(EIG_NT) void ritz_select_kernel(const double* __res
   if (t == 0) {
     double g = 0.0, hm = HUGE_VAL;
     for (int w = 0; w < EIG_NT / 64; ++w) { g = fmax(g, red[w]); hm = fmin(hm, red2[w]); }
-    bad_s = !(g < (1.0 - 1e-3) * hm);
-    if (bad_s) *flag = 1;
+    bad_s = !(g < (1.0 - 1e-3) * hm) ? 5 : 0;
+    if (bad_s) *flag = bad_s;
   }
   __syncthreads();
   if (bad_s) return;
