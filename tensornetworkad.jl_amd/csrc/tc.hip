@@ -101,8 +101,11 @@ template <> struct MfmaOp<float> {
 // workgroup of 4 waves in 2 x 2; each wave owns a (TILE/2)^2 block of (TILE/32)^2 MFMA 16x16 blocks,
 // so TILE = 128 does twice the MFMA work per LDS fragment read of TILE = 64.
 // ---------------------------------------------------------------------------
-template <typename T, int BK, int TILE>
-__global__ __launch_bounds__(NT) void tc_kernel(TcDesc d, const T* __restrict__ A,
+// PF: K-tiles in flight in registers (1: the loads of tile t + 2 are issued while tile t is multiplied and
+// stored when tile t + 1 is; 2: those of tile t + 3, two register sets alternating, for operands whose
+// loads come from beyond the XCD's L2 and need more than one K-tile of cover).
+template <typename T, int BK, int TILE, int PF = 1>
+__global__ __launch_bounds__(NT, PF == 2 ? 4 : 1) void tc_kernel(TcDesc d, const T* __restrict__ A,
                                                 const T* __restrict__ B, T* __restrict__ C,
                                                 T* __restrict__ Wsp) {
   constexpr int PAD = 16;
@@ -153,7 +156,7 @@ __global__ __launch_bounds__(NT) void tc_kernel(TcDesc d, const T* __restrict__ 
     b_noff[r] = b_nok[r] ? off2(n, d.N0, d.bn0, d.bn1) : 0;
   }
 
-  T ra[NR], rb[NR];
+  T ra0[NR], rb0[NR], ra1[NR], rb1[NR];
   KPos a_kp[NR], b_kp[NR];
   const long long a_dk = d.ak1 - (long long)d.K0 * d.ak0, b_dk = d.bk1 - (long long)d.K0 * d.bk0;
 #pragma unroll
@@ -161,18 +164,19 @@ __global__ __launch_bounds__(NT) void tc_kernel(TcDesc d, const T* __restrict__ 
     a_kp[r].init(kbeg + a_ki[r], d.K0, d.ak0, d.ak1);
     b_kp[r].init(kbeg + b_ki[r], d.K0, d.bk0, d.bk1);
   }
-  // loads the K-tile starting at k0, then moves the loaders one tile on
-  auto gload = [&](int k0) {
+  // loads the K-tile starting at k0 into (xa, xb), then moves the loaders one tile on (tiles are
+  // loaded in order)
+  auto gload = [&](T (&xa)[NR], T (&xb)[NR], int k0) {
 #pragma unroll
     for (int r = 0; r < NR; ++r) {
       int k = k0 + a_ki[r];
       T v = T(0);
       if (a_mok[r] && k < K) v = A[a_moff[r] + a_kp[r].off];
-      ra[r] = v;
+      xa[r] = v;
       int kb = k0 + b_ki[r];
       T w = T(0);
       if (b_nok[r] && kb < K) w = B[b_noff[r] + b_kp[r].off];
-      rb[r] = w;
+      xb[r] = w;
       a_kp[r].advance(BK, d.K0, d.ak0, a_dk);
       b_kp[r].advance(BK, d.K0, d.bk0, b_dk);
     }
@@ -186,22 +190,17 @@ __global__ __launch_bounds__(NT) void tc_kernel(TcDesc d, const T* __restrict__ 
     for (int j = 0; j < FB; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
 
   // Software pipeline over K-tiles with two LDS buffers: while tile t is multiplied out of buffer t % 2,
-  // tile t + 1 (loaded into registers during the previous step) is stored into the other buffer and the
-  // loads of tile t + 2 are issued.  One barrier per K-tile: it publishes tile t + 1 and retires the reads
-  // of tile t, whose buffer the next step refills.
-  auto stage = [&](int buf) {
+  // tile t + 1 (loaded into registers earlier) is stored into the other buffer and the loads of tile
+  // t + 1 + PF are issued.  One barrier per K-tile: it publishes tile t + 1 and retires the reads of
+  // tile t, whose buffer the next step refills.
+  auto stage = [&](const T (&xa)[NR], const T (&xb)[NR], int buf) {
 #pragma unroll
     for (int r = 0; r < NR; ++r) {
-      As[buf][a_ki[r]][a_mi[r]] = ra[r];
-      Bs[buf][b_ki[r]][b_ni[r]] = rb[r];
+      As[buf][a_ki[r]][a_mi[r]] = xa[r];
+      Bs[buf][b_ki[r]][b_ni[r]] = xb[r];
     }
   };
-  gload(kbeg);
-  stage(0);
-  if (kbeg + BK < K) gload(kbeg + BK);
-  __syncthreads();
-  int buf = 0;
-  for (int k0 = kbeg; k0 < K; k0 += BK, buf ^= 1) {
+  auto mfma_tile = [&](int buf) {
 #pragma unroll
     for (int kk = 0; kk < BK; kk += 4) {
       T af[FB], bf[FB];
@@ -214,11 +213,43 @@ __global__ __launch_bounds__(NT) void tc_kernel(TcDesc d, const T* __restrict__ 
 #pragma unroll
         for (int j = 0; j < FB; ++j) acc[i][j] = MfmaOp<T>::run(af[i], bf[j], acc[i][j]);
     }
-    if (k0 + BK < K) {
-      stage(buf ^ 1);
-      if (k0 + 2 * BK < K) gload(k0 + 2 * BK);
+  };
+  gload(ra0, rb0, kbeg);
+  stage(ra0, rb0, 0);
+  if (kbeg + BK < K) gload(ra1, rb1, kbeg + BK);
+  if (PF == 2 && kbeg + 2 * BK < K) gload(ra0, rb0, kbeg + 2 * BK);
+  __syncthreads();
+  if constexpr (PF == 1) {
+    int buf = 0;
+    for (int k0 = kbeg; k0 < K; k0 += BK, buf ^= 1) {
+      mfma_tile(buf);
+      if (k0 + BK < K) {
+        stage(ra1, rb1, buf ^ 1);
+        if (k0 + 2 * BK < K) gload(ra1, rb1, k0 + 2 * BK);
+      }
+      __syncthreads();
     }
-    __syncthreads();
+  } else {
+    // unrolled by two, so the LDS buffer and the register set of every step are fixed: an even tile is
+    // multiplied out of buffer 0 while tile t + 1 comes from set 1, an odd one out of buffer 1 while
+    // tile t + 1 comes from set 0
+    for (int k0 = kbeg; k0 < K;) {
+      mfma_tile(0);
+      if (k0 + BK < K) {
+        stage(ra1, rb1, 1);
+        if (k0 + 3 * BK < K) gload(ra1, rb1, k0 + 3 * BK);
+      }
+      __syncthreads();
+      k0 += BK;
+      if (k0 >= K) break;
+      mfma_tile(1);
+      if (k0 + BK < K) {
+        stage(ra0, rb0, 0);
+        if (k0 + 3 * BK < K) gload(ra0, rb0, k0 + 3 * BK);
+      }
+      __syncthreads();
+      k0 += BK;
+    }
   }
 
   // epilogue: column lane%16 of each 16x16 block, rows per MfmaOp<T>::row
@@ -491,7 +522,7 @@ void tc_launch(Ctx* c, Prec p, const TcDesc& d, const void* A, const void* B, vo
         hipLaunchKernelGGL((tc_kernel<float, TC_BK_DEEP, 64>), grid, dim3(NT), 0, s, dd, (const float*)A,
                            (const float*)B, (float*)C, (float*)W);
       else
-        hipLaunchKernelGGL((tc_kernel<float, TC_BK, 64>), grid, dim3(NT), 0, s, dd, (const float*)A,
+        hipLaunchKernelGGL((tc_kernel<float, TC_BK, 64, 2>), grid, dim3(NT), 0, s, dd, (const float*)A,
                            (const float*)B, (float*)C, (float*)W);
       break;
     case Prec::BF16:
