@@ -242,21 +242,6 @@ void tc(Ctx* c, Prec p, const TView& A, const void* a, const TView& B, const voi
   tc_launch(c, p, contract_desc(A, B, C, alpha, beta), a, b, o, c->stream);
 }
 
-// Opt-in diagnostics: with TNAD_DEBUG_DUMP=<dir> set, named FP64 device arrays are
-// written as raw little-endian files <dir>/<name>.bin (synchronises the stream).
-static void debug_dump(Ctx* c, const char* name, const double* dev, size_t n) {
-  const char* dir = std::getenv("TNAD_DEBUG_DUMP");
-  if (!dir || !*dir) return;
-  std::vector<double> h(n);
-  c->read_async(h.data(), dev, n * 8);
-  c->sync();
-  std::string path = std::string(dir) + "/" + name + ".bin";
-  if (FILE* f = std::fopen(path.c_str(), "wb")) {
-    std::fwrite(h.data(), 8, n, f);
-    std::fclose(f);
-  }
-}
-
 // Opt-in diagnostics (TNAD_DEBUG_EIG=1): report whether a device array holds a non-finite value.
 static void dbg_finite(Ctx* c, const char* what, const double* dev, size_t n) {
   static const bool on = [] {

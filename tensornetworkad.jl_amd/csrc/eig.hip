@@ -50,10 +50,6 @@ __device__ __forceinline__ void rr_pair(int r_in, int t, int M, int& p, int& q) 
   if (p > q) { int tmp = p; p = q; q = tmp; }
 }
 
-__device__ __forceinline__ int pk24(int lo, int hi) {  // pk for 0 <= lo <= hi < 2^12, 24-bit multiply
-  return (int)(__umul24((unsigned)hi, (unsigned)hi + 1u) >> 1) + lo;
-}
-
 // Jacobi sweeps on the matrix in LDS, stored in slot-position order.  Round r of the round-robin
 // ordering pairs, in slot t, the indices a = r + t and b = r - t (mod M = mm - 1; slot 0 pairs r with
 // the fixed index M).  Position 2t holds a and 2t + 1 holds b, so every round works on the same
