@@ -267,10 +267,7 @@ static void dbg_finite(Ctx* c, const char* what, const double* dev, size_t n) {
 static void orthonormalize(Ctx* c, double* Y, int n, int k, double* X, EigWork& w, unsigned seed) {
   hipStream_t s = c->stream;
   // column scaling, random refill of (numerically) null columns
-  k_col_norms(Y, n, k, w.nrm.as<double>(), s);
-  k_random_fill_cols(Y, n, k, w.nrm.as<double>(), 1e-13, seed, s);
-  k_col_norms(Y, n, k, w.nrm.as<double>(), s);
-  k_scale_cols(Y, n, k, w.nrm.as<double>(), 1, s);
+  k_normalize_cols(Y, n, k, w.nrm.as<double>(), 1e-13, seed, s);
   // A well-conditioned block takes plain CholeskyQR passes: one when the result is orthonormal to
   // working precision (|X^T X - I|_F <= 1e-12, the usual case once the iteration converges), else a
   // second (CholeskyQR2).  That holds when the Cholesky pivots of the Gram matrix stay above 1e-4 (unit

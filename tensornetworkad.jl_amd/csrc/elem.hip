@@ -126,16 +126,44 @@ __device__ __forceinline__ double hash_uniform(unsigned seed, unsigned long long
   return ((bits & ((1ull << 53) - 1)) + 0.5) * (1.0 / 9007199254740992.0);   // (0,1)
 }
 
-__global__ void random_fill_cols_kernel(double* X, int n, int k, const double* nrm, double rel,
-                                        unsigned seed) {
+// One block per column j, after col_norms_kernel: a column whose norm is at most rel times the largest
+// is refilled with uniform random values in (-1, 1) (hash of the seed and the element index n j + i) and its
+// norm recomputed in col_norms_kernel's order; then the column is scaled to unit norm as scale_cols_kernel
+// (invert) does.  nrm keeps the first-pass norms (every block reads them for the threshold).
+__global__ void normalize_fill_cols_kernel(double* X, int n, int k, const double* __restrict__ nrm, double rel,
+                                           unsigned seed) {
+  __shared__ double red[16];
+  __shared__ double bc;
+  const int j = blockIdx.x;
   double mx = 0.0;
-  for (int j = 0; j < k; ++j) mx = fmax(mx, nrm[j]);
-  const double thresh = rel * mx;
-  size_t tot = (size_t)n * k;
-  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < tot; e += (size_t)gridDim.x * blockDim.x) {
-    int j = e / n;
-    if (nrm[j] <= thresh) X[e] = 2.0 * hash_uniform(seed, e) - 1.0;
+  for (int q = threadIdx.x; q < k; q += blockDim.x) mx = fmax(mx, nrm[q]);
+  for (int o = 32; o > 0; o >>= 1) mx = fmax(mx, __shfl_down(mx, o));
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = mx;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = 0.0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) m = fmax(m, red[w]);
+    bc = m;
   }
+  __syncthreads();
+  const double thresh = rel * bc;
+  double v = nrm[j];
+  double* col = X + (size_t)j * n;
+  if (v <= thresh) {   // the same decision in every thread of the block
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+      const double r = 2.0 * hash_uniform(seed, (size_t)j * n + i) - 1.0;
+      col[i] = r;
+      acc += r * r;
+    }
+    __syncthreads();   // red is reused
+    const double ssum = block_sum(acc, red);
+    if (threadIdx.x == 0) bc = sqrt(ssum);
+    __syncthreads();
+    v = bc;
+  }
+  const double f = v > 0.0 ? 1.0 / v : 0.0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) col[i] *= f;
 }
 
 __global__ void random_fill_flagged_kernel(double* X, int n, int k, const int* flag, unsigned seed) {
@@ -359,10 +387,9 @@ void k_scale_cols(double* X, int n, int k, const double* scale, int invert, hipS
   hipLaunchKernelGGL(scale_cols_kernel, dim3(nblk((size_t)n * k)), dim3(NT), 0, s, X, n, k, scale, invert);
   TNAD_HIP_TRY(hipGetLastError());
 }
-void k_random_fill_cols(double* X, int n, int k, const double* nrm, double rel, unsigned seed,
-                        hipStream_t s) {
-  hipLaunchKernelGGL(random_fill_cols_kernel, dim3(nblk((size_t)n * k)), dim3(NT), 0, s, X, n, k, nrm,
-                     rel, seed);
+void k_normalize_cols(double* X, int n, int k, double* nrm, double rel, unsigned seed, hipStream_t s) {
+  hipLaunchKernelGGL(col_norms_kernel, dim3(k), dim3(NT), 0, s, X, n, nrm);
+  hipLaunchKernelGGL(normalize_fill_cols_kernel, dim3(k), dim3(NT), 0, s, X, n, k, nrm, rel, seed);
   TNAD_HIP_TRY(hipGetLastError());
 }
 void k_random_fill_flagged(double* X, int n, int k, const int* flag, unsigned seed, hipStream_t s) {

@@ -120,9 +120,9 @@ void k_dot(const double* x, const double* y, size_t n, double* out, hipStream_t 
 void k_edge_sym(const double* T1, int chi, int D, double* T2, hipStream_t s);  // T2 = T1 + T1^{kji}
 void k_col_norms(const double* X, int n, int k, double* nrm, hipStream_t s);
 void k_scale_cols(double* X, int n, int k, const double* scale, int invert, hipStream_t s);
-// refill (with hash-random entries) every column whose norm is <= rel * max norm
-void k_random_fill_cols(double* X, int n, int k, const double* nrm, double rel, unsigned seed,
-                        hipStream_t s);
+// Column norms into nrm; every column whose norm is <= rel * max norm refilled with hash-random entries;
+// every column scaled to unit norm.  Two launches (norms, then one block per column).
+void k_normalize_cols(double* X, int n, int k, double* nrm, double rel, unsigned seed, hipStream_t s);
 // overwrite the columns j with flag[j] != 0 by hash-random entries
 void k_random_fill_flagged(double* X, int n, int k, const int* flag, unsigned seed, hipStream_t s);
 void k_random_matrix(double* X, size_t n, unsigned seed, hipStream_t s);
