@@ -49,17 +49,34 @@ __global__ void resid_norms_kernel(const double* __restrict__ X, const double* _
   if (threadIdx.x == 0) out[j] = sqrt(red[0] + red[1] + red[2] + red[3]);
 }
 
-// next subspace block from the Rayleigh-Ritz pairs: Z[:, j] = Yr[:, j] / theta_j = Xr[:, j] + R[:, j] / theta_j
-// (R = Yr - Xr Theta is orthogonal to Xr, so Z^T Z = I + (R Theta^-1)^T (R Theta^-1) >= I stays well
-// conditioned however collinear the residuals are); numerically null Ritz values keep Z[:, j] = Xr[:, j].
-__global__ void ritz_scale_kernel(const double* __restrict__ Xr, const double* __restrict__ Yr,
-                                  const double* __restrict__ theta, int n, int k, double* __restrict__ Z) {
-  const double t0 = fabs(theta[0]);
-  size_t tot = (size_t)n * k;
-  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < tot; e += (size_t)gridDim.x * blockDim.x) {
-    int j = e / n;
-    double th = theta[j];
-    Z[e] = (fabs(th) > 1e-14 * t0) ? Yr[e] / th : Xr[e];
+// Next subspace block from the Rayleigh-Ritz pairs, and its column norms, in one pass:
+// Z[:, j] = Yr[:, j] / theta_j = Xr[:, j] + R[:, j] / theta_j (R = Yr - Xr Theta is orthogonal to Xr, so
+// Z^T Z = I + (R Theta^-1)^T (R Theta^-1) >= I stays well conditioned however collinear the residuals are);
+// numerically null Ritz values keep Z[:, j] = Xr[:, j].  One block of 256 threads per column j, the sum of
+// squares taken in the order of the column-norm kernel (thread-strided partial sums, a shuffle tree per
+// wave, the wave sums added in order).
+__global__ __launch_bounds__(256) void ritz_scale_norms_kernel(const double* __restrict__ Xr,
+                                                               const double* __restrict__ Yr,
+                                                               const double* __restrict__ theta, int n,
+                                                               double* __restrict__ Z, double* __restrict__ nrm) {
+  __shared__ double red[4];
+  const int j = blockIdx.x;
+  const double t0 = fabs(theta[0]), th = theta[j];
+  const bool use_y = fabs(th) > 1e-14 * t0;
+  const size_t o = (size_t)n * j;
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < n; i += 256) {
+    const double z = use_y ? Yr[o + i] / th : Xr[o + i];
+    Z[o + i] = z;
+    acc += z * z;
+  }
+  for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double sum = 0.0;
+    for (int w = 0; w < 4; ++w) sum += red[w];
+    nrm[j] = sqrt(sum);
   }
 }
 
@@ -264,10 +281,13 @@ static void dbg_finite(Ctx* c, const char* what, const double* dev, size_t n) {
 // ==========================================================================
 // Eigensolver: top-k eigenpairs of a symmetric n x n FP64 matrix by |lambda|
 // ==========================================================================
-static void orthonormalize(Ctx* c, double* Y, int n, int k, double* X, EigWork& w, unsigned seed) {
+// norms_ready: w.nrm already holds the column norms of Y (ritz_scale_norms_kernel)
+static void orthonormalize(Ctx* c, double* Y, int n, int k, double* X, EigWork& w, unsigned seed,
+                           bool norms_ready = false) {
   hipStream_t s = c->stream;
   // column scaling, random refill of (numerically) null columns
-  k_normalize_cols(Y, n, k, w.nrm.as<double>(), 1e-13, seed, s);
+  if (norms_ready) k_fill_scale_cols(Y, n, k, w.nrm.as<double>(), 1e-13, seed, s);
+  else k_normalize_cols(Y, n, k, w.nrm.as<double>(), 1e-13, seed, s);
   // A well-conditioned block takes plain CholeskyQR passes: one when the result is orthonormal to
   // working precision (|X^T X - I|_F <= 1e-12, the usual case once the iteration converges), else a
   // second (CholeskyQR2).  That holds when the Cholesky pivots of the Gram matrix stay above 1e-4 (unit
@@ -455,10 +475,10 @@ int topk_eig(Ctx* c, const double* A, int n, int want, int k, const double* X0, 
     // stagnation at the rounding level: no halving of the best residual in 8 iterations
     if (rel < best) { if (rel < 0.5 * best_mark) { best_mark = rel; best_it = it; } best = rel; }
     if (best < 1e3 * tol && it - best_it >= 8) break;
-    hipLaunchKernelGGL(ritz_scale_kernel, dim3(nb((size_t)n * k)), dim3(NT), 0, s, w.Xr.as<double>(),
-                       w.Yr.as<double>(), theta, n, k, w.Y.as<double>());
+    hipLaunchKernelGGL(ritz_scale_norms_kernel, dim3(k), dim3(256), 0, s, w.Xr.as<double>(), w.Yr.as<double>(),
+                       theta, n, w.Y.as<double>(), w.nrm.as<double>());
     TNAD_HIP_TRY(hipGetLastError());
-    orthonormalize(c, w.Y.as<double>(), n, k, X, w, 0x51ed2701u + it);
+    orthonormalize(c, w.Y.as<double>(), n, k, X, w, 0x51ed2701u + it, true);
   }
   k_copy(w.Xr.as<double>(), X, (size_t)n * k, s);
   k_copy(w.Yr.as<double>(), AX, (size_t)n * k, s);

@@ -389,6 +389,9 @@ void k_scale_cols(double* X, int n, int k, const double* scale, int invert, hipS
 }
 void k_normalize_cols(double* X, int n, int k, double* nrm, double rel, unsigned seed, hipStream_t s) {
   hipLaunchKernelGGL(col_norms_kernel, dim3(k), dim3(NT), 0, s, X, n, nrm);
+  k_fill_scale_cols(X, n, k, nrm, rel, seed, s);
+}
+void k_fill_scale_cols(double* X, int n, int k, const double* nrm, double rel, unsigned seed, hipStream_t s) {
   hipLaunchKernelGGL(normalize_fill_cols_kernel, dim3(k), dim3(NT), 0, s, X, n, k, nrm, rel, seed);
   TNAD_HIP_TRY(hipGetLastError());
 }

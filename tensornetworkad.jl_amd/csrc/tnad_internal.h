@@ -123,6 +123,9 @@ void k_scale_cols(double* X, int n, int k, const double* scale, int invert, hipS
 // Column norms into nrm; every column whose norm is <= rel * max norm refilled with hash-random entries;
 // every column scaled to unit norm.  Two launches (norms, then one block per column).
 void k_normalize_cols(double* X, int n, int k, double* nrm, double rel, unsigned seed, hipStream_t s);
+// The second launch alone, for columns whose norms (256-thread column sums of squares, in col_norms' order)
+// are already in nrm.
+void k_fill_scale_cols(double* X, int n, int k, const double* nrm, double rel, unsigned seed, hipStream_t s);
 // overwrite the columns j with flag[j] != 0 by hash-random entries
 void k_random_fill_flagged(double* X, int n, int k, const int* flag, unsigned seed, hipStream_t s);
 void k_random_matrix(double* X, size_t n, unsigned seed, hipStream_t s);
