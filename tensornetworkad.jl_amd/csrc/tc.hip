@@ -105,7 +105,7 @@ template <> struct MfmaOp<float> {
 // stored when tile t + 1 is; 2: those of tile t + 3, two register sets alternating, for operands whose
 // loads come from beyond the XCD's L2 and need more than one K-tile of cover).
 template <typename T, int BK, int TILE, int PF = 1>
-__global__ __launch_bounds__(NT, PF == 2 ? 4 : 1) void tc_kernel(TcDesc d, const T* __restrict__ A,
+__global__ __launch_bounds__(NT, PF == 2 ? (sizeof(T) == 8 ? 3 : 4) : 1) void tc_kernel(TcDesc d, const T* __restrict__ A,
                                                 const T* __restrict__ B, T* __restrict__ C,
                                                 T* __restrict__ Wsp) {
   constexpr int PAD = 16;
@@ -508,7 +508,7 @@ void tc_launch(Ctx* c, Prec p, const TcDesc& d, const void* A, const void* B, vo
         hipLaunchKernelGGL((tc_kernel<double, TC_BK_DEEP, 64>), grid, dim3(NT), 0, s, dd, (const double*)A,
                            (const double*)B, (double*)C, (double*)W);
       else
-        hipLaunchKernelGGL((tc_kernel<double, TC_BK, 64>), grid, dim3(NT), 0, s, dd, (const double*)A,
+        hipLaunchKernelGGL((tc_kernel<double, TC_BK, 64, 2>), grid, dim3(NT), 0, s, dd, (const double*)A,
                            (const double*)B, (double*)C, (double*)W);
       break;
     case Prec::F32:
