@@ -41,8 +41,8 @@ PEAK = {"fp64": 78.6, "fp32": 157.3, "bf16": 2500.0}   # dense matrix TFLOP/s
 # rocprofv3 --pmc runs of `bench.py --kernel-only` (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch, mean of
 # 21 launches), each scaled by what the counter reads for a copy of known size in the same run: the 256 MiB
 # device copy reads FETCH_SIZE = 131089 KiB and WRITE_SIZE = 262144 KiB, so fetch x 2.0, write x 1.0
-# (the FP64->FP32 cast of 32 MiB gives the same fetch factor).  9344 KiB x 2 + 16384 KiB = 35072 KiB.
-TRAFFIC = 35072 * 1024
+# (the FP64->FP32 cast of 32 MiB gives the same fetch factor).  9349 KiB x 2 + 16384 KiB = 35081 KiB.
+TRAFFIC = 35081 * 1024
 
 
 def heisenberg_h():
