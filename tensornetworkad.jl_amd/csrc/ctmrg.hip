@@ -33,10 +33,18 @@ namespace {
 
 constexpr int NT = 256;
 
+// out[j] = |Y[:, j] - theta_j X[:, j]| for j < gridDim.x; block 0 also writes out[gridDim.x] = theta_0 and
+// out[gridDim.x + 1] = the Rayleigh-Ritz flag (*gate, or 1 without one), so that the host reads the
+// iteration's whole status with one transfer
 __global__ void resid_norms_kernel(const double* __restrict__ X, const double* __restrict__ Y,
-                                   const double* __restrict__ theta, int n, double* __restrict__ out) {
+                                   const double* __restrict__ theta, int n, double* __restrict__ out,
+                                   const int* __restrict__ gate) {
   __shared__ double red[4];
   const int j = blockIdx.x;
+  if (j == 0 && threadIdx.x == 0) {
+    out[gridDim.x] = theta[0];
+    out[gridDim.x + 1] = gate ? (double)*gate : 1.0;
+  }
   const double th = theta[j];
   double acc = 0.0;
   for (int i = threadIdx.x; i < n; i += blockDim.x) {
@@ -84,8 +92,10 @@ __global__ __launch_bounds__(256) void ritz_scale_norms_kernel(const double* __r
 // |G - I|_F^2 of a k x k matrix (k <= 256): one workgroup of GD_NT threads, thread (row i, column
 // phase jo) sums rows i over columns jo, jo + GD_NT / 256, ... (coalesced along rows)
 constexpr int GD_NT = 1024;
+// out[0] = |G - I|_F^2; out[1] = *info (the preceding Cholesky's breakdown column, as a double), so the host
+// reads the pass's status with the pivot in one transfer
 __global__ __launch_bounds__(GD_NT) void gram_dev_kernel(const double* __restrict__ G, int k,
-                                                         double* __restrict__ out) {
+                                                         double* __restrict__ out, const int* __restrict__ info) {
   __shared__ double red[GD_NT / 64];
   const int t = threadIdx.x, i = t & 255, jo = t >> 8;
   double acc = 0.0;
@@ -100,7 +110,8 @@ __global__ __launch_bounds__(GD_NT) void gram_dev_kernel(const double* __restric
   if (t == 0) {
     double s = 0.0;
     for (int w = 0; w < GD_NT / 64; ++w) s += red[w];
-    *out = s;
+    out[0] = s;
+    out[1] = (double)*info;
   }
 }
 
@@ -328,18 +339,18 @@ static void orthonormalize(Ctx* c, double* Y, int n, int k, double* X, EigWork& 
   // the dependent columns, they are refilled, and the plain passes are tried once more (a block whose
   // only defect was those columns is now well-conditioned).  What still fails takes sCholQR3 below.
   for (int round = 0; round < 2; ++round) {
-    double* scal = w.scal.as<double>();   // [0] smallest pivot, [1] |X^T X - I|_F^2
+    double* scal = w.scal.as<double>();   // [0] smallest pivot, [1] |X^T X - I|_F^2, [2] breakdown column
     tc_launch(c, Prec::F64, gemm_desc('T', 'N', k, k, n, n, n, k), Y, Y, w.G.as<double>(), s);
     chol_inv(w.G.as<double>(), k, 0.0, w.Rinv.as<double>(), w.info.as<int>(), nullptr, s, scal);
     tc_launch(c, Prec::F64, gemm_desc('N', 'N', n, k, k, n, k, n), Y, w.Rinv.as<double>(), X, s);
     tc_launch(c, Prec::F64, gemm_desc('T', 'N', k, k, n, n, n, k), X, X, w.G.as<double>(), s);
-    hipLaunchKernelGGL(gram_dev_kernel, dim3(1), dim3(GD_NT), 0, s, w.G.as<double>(), k, scal + 1);
+    hipLaunchKernelGGL(gram_dev_kernel, dim3(1), dim3(GD_NT), 0, s, w.G.as<double>(), k, scal + 1,
+                       w.info.as<int>());
     TNAD_HIP_TRY(hipGetLastError());
-    int info = 0;
-    double hs[2] = {0.0, 0.0};
-    c->read_async(&info, w.info.ptr, sizeof(int));
-    c->read_async(hs, scal, 2 * sizeof(double));
+    double hs[3] = {0.0, 0.0, 0.0};   // smallest pivot, |X^T X - I|_F^2, Cholesky breakdown column
+    c->read_async(hs, scal, 3 * sizeof(double));
     c->sync();
+    int info = (int)hs[2];
     if (std::getenv("TNAD_DEBUG_EIG"))
       std::fprintf(stderr, "[tnad eig] plain Cholesky info=%d min pivot=%.3e |X^T X - I|_F=%.3e\n", info, hs[0],
                    std::sqrt(hs[1]));
@@ -390,7 +401,7 @@ static void orthonormalize(Ctx* c, double* Y, int n, int k, double* X, EigWork& 
 EigWork::EigWork(Ctx* c, int n, int k)
     : Y(c, (size_t)n * k * 8), Xr(c, (size_t)n * k * 8), Yr(c, (size_t)n * k * 8), H(c, (size_t)k * k * 8),
       W(c, (size_t)k * k * 8), Vw(c, jacobi_work_doubles(k, 60) * 8), G(c, (size_t)k * k * 8), Rinv(c, (size_t)k * k * 8),
-      nrm(c, (size_t)k * 8), res(c, (size_t)k * 8), info(c, (size_t)k * sizeof(int) + 16), sweeps(c, 16),
+      nrm(c, (size_t)k * 8), res(c, (size_t)(k + 2) * 8), info(c, (size_t)k * sizeof(int) + 16), sweeps(c, 16),
       scal(c, 16 * 8) {}
 
 int topk_eig(Ctx* c, const double* A, int n, int want, int k, const double* X0, int maxit, double tol,
@@ -413,7 +424,7 @@ int topk_eig(Ctx* c, const double* A, int n, int want, int k, const double* X0, 
     k_random_matrix(w.Y.as<double>(), (size_t)n * k, 0x1234567u, s);
   }
   orthonormalize(c, w.Y.as<double>(), n, k, X, w, 0x9e3779b9u);
-  std::vector<double> hres(k), hth(k);
+  std::vector<double> hres(k + 2), hth(k);
   // opt-in trace of the iteration (TNAD_DEBUG_EIG=1), to stderr
   static const bool dbg = [] {
     const char* e = std::getenv("TNAD_DEBUG_EIG");
@@ -452,14 +463,14 @@ int topk_eig(Ctx* c, const double* A, int n, int want, int k, const double* X0, 
     tc_launch(c, Prec::F64, gemm_desc('N', 'N', n, k, k, n, k, n), w.Y.as<double>(), w.W.as<double>(),
               w.Yr.as<double>(), s);
     hipLaunchKernelGGL(resid_norms_kernel, dim3(want), dim3(NT), 0, s, w.Xr.as<double>(), w.Yr.as<double>(),
-                       theta, n, w.res.as<double>());
+                       theta, n, w.res.as<double>(), jac_forced ? nullptr : gate);
     TNAD_HIP_TRY(hipGetLastError());
-    c->read_async(hres.data(), w.res.ptr, want * 8);
-    c->read_async(hth.data(), theta, 8);
-    int flag = 1, sweeps_done = 0;
-    if (!jac_forced) c->read_async(&flag, gate, sizeof(int));
+    int sweeps_done = 0;
+    c->read_async(hres.data(), w.res.ptr, (size_t)(want + 2) * 8);   // residuals, theta_0, flag
     if (dbg) c->read_async(&sweeps_done, w.sweeps.ptr, sizeof(int));
     c->sync();
+    hth[0] = hres[want];
+    const int flag = (int)hres[want + 1];
     const bool jac = flag != 0;   // Jacobi ran (forced, or the selective Rayleigh-Ritz declined)
     n_jacobi += jac;
     double mx = 0.0;
