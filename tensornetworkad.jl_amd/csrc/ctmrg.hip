@@ -115,6 +115,15 @@ __global__ __launch_bounds__(GD_NT) void gram_dev_kernel(const double* __restric
   }
 }
 
+// F = (I + E)^{-1/2} to second order for G = I + E (binomial series: I - E/2 + 3/8 E^2), written from G and
+// G2 = G G as 15/8 I - 5/4 G + 3/8 G2; F may overwrite G2 (each entry is read before it is written)
+__global__ void inv_sqrt2_coef_kernel(const double* __restrict__ G, const double* G2, int k, double* F) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= k * k) return;
+  const int i = q % k, j = q / k;
+  F[q] = 0.375 * G2[q] - 1.25 * G[q] + (i == j ? 1.875 : 0.0);
+}
+
 // make the largest-|.| entry of every column positive (first occurrence wins);
 // the same sign is applied to the companion block Y (A*X) when given.
 __global__ void canon_sign_kernel(double* __restrict__ X, double* __restrict__ Y, int n) {
@@ -301,7 +310,7 @@ static void orthonormalize(Ctx* c, double* Y, int n, int k, double* X, EigWork& 
   else k_normalize_cols(Y, n, k, w.nrm.as<double>(), 1e-13, seed, s);
   // A well-conditioned block takes plain CholeskyQR passes: one when the result is orthonormal to
   // working precision (|X^T X - I|_F <= 1e-12, the usual case once the iteration converges), else a
-  // second (CholeskyQR2).  That holds when the Cholesky pivots of the Gram matrix stay above 1e-4 (unit
+  // second: X (X^T X)^{-1/2} to second order when |X^T X - I|_F <= 1e-5, CholeskyQR2 above.  That holds when the Cholesky pivots of the Gram matrix stay above 1e-4 (unit
   // columns: condition number around 1e4, far below u^-1/2).  The first pass and the test run before
   // the host looks at the pivots, so that one round trip serves both.  Otherwise: shifted CholeskyQR
   // followed by two plain passes (sCholQR3); columns that the shifted pass finds linearly dependent
@@ -355,7 +364,21 @@ static void orthonormalize(Ctx* c, double* Y, int n, int k, double* X, EigWork& 
       std::fprintf(stderr, "[tnad eig] plain Cholesky info=%d min pivot=%.3e |X^T X - I|_F=%.3e\n", info, hs[0],
                    std::sqrt(hs[1]));
     if (info == 0 && hs[0] > 1e-4) {
-      if (std::sqrt(hs[1]) <= 1e-12) return;
+      const double dev = std::sqrt(hs[1]);
+      if (dev <= 1e-12) return;
+      if (dev <= 1e-5) {
+        // X^T X = I + E with |E|_F <= 1e-5: X F with F = (I + E)^{-1/2} to second order is orthonormal to
+        // O(|E|^3) (below 1e-15) and spans the same columns, which is all the iteration keeps; it takes
+        // two small products instead of a second Cholesky and its host check
+        tc_launch(c, Prec::F64, gemm_desc('N', 'N', k, k, k, k, k, k), w.G.as<double>(), w.G.as<double>(),
+                  w.Rinv.as<double>(), s);
+        hipLaunchKernelGGL(inv_sqrt2_coef_kernel, dim3((k * k + 255) / 256), dim3(256), 0, s, w.G.as<double>(),
+                           w.Rinv.as<double>(), k, w.Rinv.as<double>());
+        TNAD_HIP_TRY(hipGetLastError());
+        tc_launch(c, Prec::F64, gemm_desc('N', 'N', n, k, k, n, k, n), X, w.Rinv.as<double>(), Y, s);
+        k_copy(Y, X, (size_t)n * k, s);
+        return;
+      }
       chol_inv(w.G.as<double>(), k, 0.0, w.Rinv.as<double>(), w.info.as<int>(), nullptr, s);
       c->read_async(&info, w.info.ptr, sizeof(int));
       c->sync();
