@@ -138,6 +138,12 @@ int tnad_trg(tnad_ctx* ctx, const double* a, int q, int chi, int niter, double t
 int tnad_trg_svd(tnad_ctx* ctx, const double* t, int d1, int d2, int d3, int d4, int dmax, double tol,
                  double* u, double* v, int* k_out);
 
+/* Pullback of trg_svd: the svd adjoint of src/trg.jl:55-61,72-106 (with the non-square projector terms of
+ * :98-104 for d1*d2 != d3*d4) applied to the cotangents ubar (d1,d2,k) and vbar (k,d3,d4); writes tbar
+ * (d1,d2,d3,d4).  k must be the kept count tnad_trg_svd returned for the same t, dmax and tol. */
+int tnad_trg_svd_back(tnad_ctx* ctx, const double* t, int d1, int d2, int d3, int d4, int dmax, double tol,
+                      const double* ubar, const double* vbar, int k, double* tbar);
+
 /* One strided contraction C = A * B given as an einsum over single-character index names
  * (e.g. "ibcl", "jkcb" -> "ijlk"; the contraction kernel behind every ein"..." call site of the
  * step).  Tensors are contiguous column-major FP64 with the listed extents; precision selects the

@@ -569,6 +569,11 @@ int tnad_trg_svd(tnad_ctx* ctx, const double* t, int d1, int d2, int d3, int d4,
   return guarded(ctx, [&] { trg_svd_run(&ctx->c, t, d1, d2, d3, d4, dmax, tol, u, v, k_out); });
 }
 
+int tnad_trg_svd_back(tnad_ctx* ctx, const double* t, int d1, int d2, int d3, int d4, int dmax, double tol,
+                      const double* ubar, const double* vbar, int k, double* tbar) {
+  return guarded(ctx, [&] { trg_svd_back_run(&ctx->c, t, d1, d2, d3, d4, dmax, tol, ubar, vbar, k, tbar); });
+}
+
 int tnad_contract(tnad_ctx* ctx, int precision, const char* ia, const long long* da, const double* A, const char* ib,
                   const long long* db, const double* B, const char* ic, const long long* dc, double alpha, double beta,
                   double* Cout) {

@@ -594,7 +594,30 @@ void trg_svd_run(Ctx* c, const double* t, int d1, int d2, int d3, int d4, int dm
     for (int cc = 0; cc < mp; ++cc) ov[j + (size_t)kmax * cc] = hv[j + (size_t)k * cc];
   copy_out(c, u, ou.data(), ou.size() * 8);
   copy_out(c, v, ov.data(), ov.size() * 8);
+  c->sync();   // ou and ov are the copies' sources
   if (k_out) *k_out = k;
+}
+
+// Pullback of trg_svd (src/trg.jl:33-44 through the svd adjoint of src/trg.jl:55-61,72-106): the split is
+// recomputed from t (the same singular triplets as the forward), then svd_split_back maps the cotangents
+// of u (d1,d2,k) and v (k,d3,d4) to that of t.  k must be the forward's kept count.
+void trg_svd_back_run(Ctx* c, const double* t, int d1, int d2, int d3, int d4, int dmax, double tol,
+                      const double* ubar, const double* vbar, int k, double* tbar) {
+  if (d1 < 1 || d2 < 1 || d3 < 1 || d4 < 1 || dmax < 1 || k < 1) throw Error(TNAD_ERR_ARG, "trg_svd_back: bad sizes");
+  const int m = d1 * d2, mp = d3 * d4;
+  if (m + mp > 4096) throw Error(TNAD_ERR_UNSUPPORTED, "trg_svd_back: matrix too large");
+  const int kmax = std::min(dmax, std::min(m, mp));
+  DevBuf bt(c, (size_t)m * mp * 8), bu(c, (size_t)m * kmax * 8 + 8), bv(c, (size_t)kmax * mp * 8 + 8);
+  copy_in(c, bt.ptr, t, (size_t)m * mp * 8);
+  SvdTape tp;
+  svd_split(c, bt.as<double>(), d1, d2, d3, d4, dmax, tol, bu.as<double>(), bv.as<double>(), tp);
+  if (tp.k != k) throw Error(TNAD_ERR_SHAPE, "trg_svd_back: k is not the forward's kept count");
+  DevBuf bub(c, (size_t)m * k * 8), bvb(c, (size_t)k * mp * 8), btb(c, (size_t)m * mp * 8);
+  copy_in(c, bub.ptr, ubar, (size_t)m * k * 8);
+  copy_in(c, bvb.ptr, vbar, (size_t)k * mp * 8);
+  svd_split_back(c, tp, bub.as<double>(), bvb.as<double>(), btb.as<double>());
+  copy_out(c, tbar, btb.ptr, (size_t)m * mp * 8);
+  c->sync();
 }
 
 }  // namespace tnad

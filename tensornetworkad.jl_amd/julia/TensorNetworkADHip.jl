@@ -102,6 +102,38 @@ function ChainRulesCore.rrule(::typeof(gpu_trg), a::Array{Float64,4}, χ, niter;
     return lnz, pullback
 end
 
+# trg_svd (src/trg.jl:33-44) and its pullback through svd_back (src/trg.jl:55-61,72-106)
+function gpu_trg_svd(t::Array{Float64,4}, dmax::Int, tol::Float64)
+    d1, d2, d3, d4 = size(t)
+    kmax = min(dmax, d1 * d2, d3 * d4)
+    u = zeros(d1, d2, kmax); v = zeros(kmax, d3, d4); k = Ref{Cint}(0)
+    GC.@preserve t u v begin
+        check(ccall((:tnad_trg_svd, libtnad), Cint,
+                    (Ptr{Cvoid}, Ptr{Cdouble}, Cint, Cint, Cint, Cint, Cint, Cdouble, Ptr{Cdouble}, Ptr{Cdouble}, Ptr{Cint}),
+                    context(), t, d1, d2, d3, d4, dmax, tol, u, v, k))
+    end
+    u[:, :, 1:k[]], v[1:k[], :, :]
+end
+
+function ChainRulesCore.rrule(::typeof(gpu_trg_svd), t::Array{Float64,4}, dmax::Int, tol::Float64)
+    u, v = gpu_trg_svd(t, dmax, tol)
+    function pullback(ȳ)
+        ū, v̄ = ȳ isa AbstractZero ? (ȳ, ȳ) : (ȳ[1], ȳ[2])
+        k = size(u, 3)
+        ūd = ū isa AbstractZero ? zero(u) : Array{Float64}(unthunk(ū))
+        v̄d = v̄ isa AbstractZero ? zero(v) : Array{Float64}(unthunk(v̄))
+        t̄ = similar(t)
+        GC.@preserve t ūd v̄d t̄ begin
+            check(ccall((:tnad_trg_svd_back, libtnad), Cint,
+                        (Ptr{Cvoid}, Ptr{Cdouble}, Cint, Cint, Cint, Cint, Cint, Cdouble, Ptr{Cdouble}, Ptr{Cdouble},
+                         Cint, Ptr{Cdouble}),
+                        context(), t, size(t)..., dmax, tol, ūd, v̄d, k, t̄))
+        end
+        (NoTangent(), t̄, NoTangent(), NoTangent())
+    end
+    return (u, v), pullback
+end
+
 # ---------------------------------------------------------------------------
 # ctmrg with a device tape: the pullback runs the reverse sweep of the same run
 # ---------------------------------------------------------------------------

@@ -44,5 +44,6 @@ from .api import (  # noqa: F401
     trg,
     trg_and_grad,
     trg_svd,
+    trg_svd_back,
 )
 from ._lib import LIB_PATH, DimensionMismatch, TnadError, load  # noqa: F401

@@ -528,6 +528,25 @@ def trg_svd(t, dmax: int, tol: float, ctx: Context | None = None):
     return np.asfortranarray(u[:, :, :kk]), np.asfortranarray(v[:kk, :, :])
 
 
+def trg_svd_back(t, dmax: int, tol: float, ubar, vbar, ctx: Context | None = None):
+    """Pullback of trg_svd (src/trg.jl:33-44 through svd_back, src/trg.jl:55-61,72-106): the cotangent of t
+    from those of u (d1, d2, k) and v (k, d3, d4); k is the kept count of trg_svd(t, dmax, tol)."""
+    ctx = ctx or default_context()
+    t = _host(t)
+    d1, d2, d3, d4 = t.shape
+    ubar = np.asfortranarray(_host(ubar), dtype=np.float64)
+    vbar = np.asfortranarray(_host(vbar), dtype=np.float64)
+    k = ubar.shape[2]
+    if ubar.shape != (d1, d2, k) or vbar.shape != (k, d3, d4):
+        raise ValueError("trg_svd_back: ubar must be (d1, d2, k) and vbar (k, d3, d4)")
+    pt, _t = _in(t)
+    pub, _ub = _in(ubar)
+    pvb, _vb = _in(vbar)
+    ptb, tbar = _out((d1, d2, d3, d4))
+    ctx.check(ctx.lib.tnad_trg_svd_back(ctx.h, pt, d1, d2, d3, d4, int(dmax), float(tol), pub, pvb, k, ptb))
+    return tbar
+
+
 # ---------------------------------------------------------------------------
 # finite differences (src/autodiff.jl:44,58-63) and the optimiser (src/variationalipeps.jl:67-75)
 # ---------------------------------------------------------------------------

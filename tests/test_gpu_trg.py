@@ -74,3 +74,45 @@ def test_trg_svd_keep_count_gpu(ctx):
     u, v = tnad.trg_svd(t, 10, 1e-8, ctx=ctx)
     assert u.shape[2] == 3 and v.shape[0] == 3       # src/trg.jl:37 keeps #{s > tol} + 1
     assert np.allclose(np.einsum("ija,akl->ijkl", u, v), t, atol=1e-12)
+
+
+# pullback of trg_svd on square and non-square splits (the projector terms of src/trg.jl:98-104), with and
+# without truncation: a loss invariant under the singular vectors' signs, so the GPU's gauge and the
+# oracle's give the same gradient; the oracle's is pinned by central differences in test_oracle.py
+@pytest.mark.parametrize("shape,dmax", [((2, 3, 3, 1), 10), ((3, 1, 2, 3), 10), ((2, 3, 3, 1), 2),
+                                        ((3, 1, 2, 3), 2), ((2, 3, 2, 3), 4), ((4, 5, 3, 3), 6),
+                                        ((6, 6, 6, 6), 12), ((4, 8, 2, 5), 7)])
+def test_trg_svd_back_matches_oracle(ctx, shape, dmax):
+    import tnad
+    print(f"case: trg_svd_back {shape} dmax={dmax}", flush=True)
+    rng = np.random.default_rng(5)
+    t = rng.standard_normal(shape)
+    d1, d2, d3, d4 = shape
+    m, mp = d1 * d2, d3 * d4
+    A = rng.standard_normal((m, m))
+    A = A + A.T
+    B = rng.standard_normal((mp, mp))
+    B = B + B.T
+
+    def cotangents(u, v):
+        k = u.shape[2]
+        um, vm = O.reshape_f(u, (m, k)), O.reshape_f(v, (k, mp))
+        return O.reshape_f(2 * A @ um, (d1, d2, k)), O.reshape_f(2 * vm @ B, (k, d3, d4))
+
+    uo, vo, cache = O.trg_svd(t, dmax, 0.0, keep=True)
+    g_o = O.trg_svd_back(cache, *cotangents(uo, vo))
+    ug, vg = tnad.trg_svd(t, dmax, 0.0, ctx=ctx)
+    assert ug.shape == uo.shape and vg.shape == vo.shape
+    g = tnad.trg_svd_back(t, dmax, 0.0, *cotangents(ug, vg), ctx=ctx)
+    err = np.max(np.abs(g - g_o)) / np.max(np.abs(g_o))
+    print(f"max |g - g_oracle| / max |g_oracle| = {err:.2e}", flush=True)
+    assert err < 1e-10
+
+
+def test_trg_svd_back_rejects_a_wrong_kept_count(ctx):
+    import tnad
+    print("case: trg_svd_back wrong k", flush=True)
+    t = np.random.default_rng(2).standard_normal((2, 3, 3, 1))
+    u, v = tnad.trg_svd(t, 10, 0.0, ctx=ctx)
+    with pytest.raises(tnad.TnadError):
+        tnad.trg_svd_back(t, 10, 0.0, u[:, :, :2], v[:2], ctx=ctx)

@@ -114,6 +114,46 @@ def test_svd_back_matches_fd(shape):
     assert np.allclose(g, gn, atol=1e-6)
 
 
+# trg_svd's pullback on non-square splits (the projector terms of src/trg.jl:98-104; test/svd.jl checks
+# (6,3) and (3,6) matrices): central FD of a loss that is invariant under the singular vectors' signs
+def trg_svd_quadratic_loss(shape, dmax, seed=5):
+    rng = np.random.default_rng(seed)
+    t = rng.standard_normal(shape)
+    d1, d2, d3, d4 = shape
+    m, mp = d1 * d2, d3 * d4
+    A = rng.standard_normal((m, m))
+    A = A + A.T
+    B = rng.standard_normal((mp, mp))
+    B = B + B.T
+
+    def loss(x):
+        u, v = O.trg_svd(x, dmax, 0.0)
+        k = u.shape[2]
+        um, vm = O.reshape_f(u, (m, k)), O.reshape_f(v, (k, mp))
+        return np.trace(um.T @ A @ um) + np.trace(vm @ B @ vm.T)
+
+    u, v = O.trg_svd(t, dmax, 0.0)
+    k = u.shape[2]
+    um, vm = O.reshape_f(u, (m, k)), O.reshape_f(v, (k, mp))
+    ubar = O.reshape_f(2 * A @ um, (d1, d2, k))
+    vbar = O.reshape_f(2 * vm @ B, (k, d3, d4))
+    return t, loss, ubar, vbar
+
+
+TRG_SVD_BACK_CASES = [((2, 3, 3, 1), 10), ((3, 1, 2, 3), 10), ((2, 3, 3, 1), 2), ((3, 1, 2, 3), 2),
+                      ((2, 3, 2, 3), 4), ((4, 5, 3, 3), 6)]
+
+
+@pytest.mark.parametrize("shape,dmax", TRG_SVD_BACK_CASES)
+def test_trg_svd_back_rectangular_matches_fd(shape, dmax):
+    t, loss, ubar, vbar = trg_svd_quadratic_loss(shape, dmax)
+    _, _, cache = O.trg_svd(t, dmax, 0.0, keep=True)
+    g = O.trg_svd_back(cache, ubar, vbar)
+    gn = O.num_grad(loss, t, 1e-6)
+    # without the projector terms the error is O(|g|) (checked by planting the omission)
+    assert np.max(np.abs(g - gn)) <= 1e-6 * np.max(np.abs(g))
+
+
 def test_svd_back_all_nothing_returns_none():
     # src/trg.jl:73
     U = np.eye(3)
