@@ -39,6 +39,32 @@ def test_trg_readme_config_a_gpu(ctx):
     assert abs(lnz - O.onsager_lnz(0.5)) < 1e-6
 
 
+def test_trg_chi32_matches_oracle(ctx):
+    # splits of 1024 x 1024 (augmented 2048): the truncated subspace eigensolver, not full Jacobi
+    import tnad
+    print("case: trg grad beta=0.4 chi=32 niter=8", flush=True)
+    lnz, g = _dlnz(tnad, ctx, 0.4, 32, 8)
+    lnz_o, g_o = O.trg_and_grad(0.4, 32, 8)
+    print(f"lnz {lnz:.15f} (oracle {lnz_o:.15f}), dlnz/dbeta {g:.12f} (oracle {g_o:.12f})", flush=True)
+    assert lnz == pytest.approx(lnz_o, rel=1e-12, abs=0)
+    assert g == pytest.approx(g_o, rel=1e-8)
+
+
+def test_trg_chi64_approaches_onsager(ctx):
+    # the largest supported bond (splits of 4096 x 4096, augmented 8192); the oracle would take minutes
+    # here, so the check is Onsager's exact free energy and its beta-derivative: the oracle's errors at
+    # beta = 0.4, 20 iterations are 7.6e-7 / 3.4e-5 at chi = 16 and 3.5e-8 / 1.9e-6 at chi = 32
+    import tnad
+    print("case: trg grad beta=0.4 chi=64 niter=20", flush=True)
+    beta, h = 0.4, 1e-4
+    exact = O.onsager_lnz(beta)
+    dexact = (O.onsager_lnz(beta + h) - O.onsager_lnz(beta - h)) / (2 * h)
+    lnz, g = _dlnz(tnad, ctx, beta, 64, 20)
+    print(f"lnz - exact {lnz - exact:.3e}, dlnz/dbeta - exact {g - dexact:.3e}", flush=True)
+    assert abs(lnz - exact) < 1.5e-8
+    assert abs(g - dexact) < 1e-6
+
+
 @pytest.mark.parametrize("chi,niter", [(6, 3), (8, 4)])
 def test_trg_full_gradient_matches_oracle_direction(ctx, chi, niter):
     # a generic full-rank tensor: every singular value is resolved, so the reference's
