@@ -470,7 +470,7 @@ int topk_eig(Ctx* c, const double* A, int n, int want, int k, const double* X0, 
     // (ritz_select's flag gates the Jacobi kernels on the device: no host round trip)
     const bool jac_forced = force_jacobi || (it == 1 && !X0);
     int* gate = w.info.as<int>();
-    if (!jac_forced) ritz_select(w.H.as<double>(), k, k, want, 1e-3, theta, w.W.as<double>(), gate, s);
+    if (!jac_forced) ritz_select(w.H.as<double>(), k, k, want, 1e-2, theta, w.W.as<double>(), gate, s);
     // far from convergence the Rayleigh-Ritz step only has to improve the basis (the span, which
     // is what converges, does not depend on it): two Jacobi sweeps do.  Closer in, three: Jacobi
     // converges quadratically on a nearly diagonal H, and what is left is taken up by the next
