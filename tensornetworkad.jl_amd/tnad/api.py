@@ -536,8 +536,8 @@ def trg_svd_back(t, dmax: int, tol: float, ubar, vbar, ctx: Context | None = Non
     d1, d2, d3, d4 = t.shape
     ubar = np.asfortranarray(_host(ubar), dtype=np.float64)
     vbar = np.asfortranarray(_host(vbar), dtype=np.float64)
-    k = ubar.shape[2]
-    if ubar.shape != (d1, d2, k) or vbar.shape != (k, d3, d4):
+    k = ubar.shape[2] if ubar.ndim == 3 else -1
+    if k < 1 or ubar.shape != (d1, d2, k) or vbar.shape != (k, d3, d4):
         raise ValueError("trg_svd_back: ubar must be (d1, d2, k) and vbar (k, d3, d4)")
     pt, _t = _in(t)
     pub, _ub = _in(ubar)

@@ -49,3 +49,14 @@ def test_no_cpu_fallback_in_product_path():
     for p in pkg.rglob("*.py"):
         src = p.read_text()
         assert "oracle" not in src.replace("oracle.", "").lower() or p.name == "build_lib.py", p
+
+
+def test_trg_svd_back_checks_cotangent_shapes_before_the_library():
+    # host-side validation: a malformed cotangent never reaches the C ABI (no context is needed)
+    import numpy as np
+    import tnad
+    t = np.zeros((2, 3, 3, 1))
+    with pytest.raises(ValueError):
+        tnad.trg_svd_back(t, 10, 0.0, np.zeros((6, 3)), np.zeros((3, 3, 1)), ctx=object())
+    with pytest.raises(ValueError):
+        tnad.trg_svd_back(t, 10, 0.0, np.zeros((2, 3, 3)), np.zeros((2, 3, 1)), ctx=object())
