@@ -204,6 +204,7 @@ void run_ctmrg(tnad_ctx* ctx, tnad_tape& t, const double* C0, const double* T0, 
   so.eig_max_iters = o.eig_max_iters > 0 ? o.eig_max_iters : 200;
   so.eig_warm_iters = o.eig_warm_iters > 0 ? o.eig_warm_iters : 200;
   so.eig_tol = o.eig_tol > 0 ? o.eig_tol : 1e-13;
+  const double base_eig_tol = so.eig_tol;
   // a low-precision step's contractions carry ~1e-7 relative error: its eigenvectors need not
   // be resolved beyond what the switch test can see
   so.eig_tol_low = std::max(so.eig_tol, 1e-3 * o.switch_tol);
@@ -231,6 +232,11 @@ void run_ctmrg(tnad_ctx* ctx, tnad_tape& t, const double* C0, const double* T0, 
     if (!switched && diff <= o.switch_tol) switched = true;
     oldvals = vals;
     so.prec = switched ? Prec::F64 : low;
+    // in a mixed run an FP64 step before convergence resolves its eigenvectors only as far as the
+    // step-to-step change can see: 1e-3 of |vals - oldvals|, between eig_tol and the low-precision steps'
+    // tolerance (the plain path keeps eig_tol on every step)
+    if (lowp) so.eig_tol = std::isfinite(diff) ? std::max(base_eig_tol, std::min(so.eig_tol_low, 1e-3 * diff))
+                                               : base_eig_tol;
     const PBuf& ap = ap_at(c, t, so.prec);
     StepTape* tp = nullptr;
     if (keep_steps) {
