@@ -233,10 +233,12 @@ void run_ctmrg(tnad_ctx* ctx, tnad_tape& t, const double* C0, const double* T0, 
     oldvals = vals;
     so.prec = switched ? Prec::F64 : low;
     // in a mixed run an FP64 step before convergence resolves its eigenvectors only as far as the
-    // step-to-step change can see: 1e-3 of |vals - oldvals|, between eig_tol and the low-precision steps'
-    // tolerance (the plain path keeps eig_tol on every step)
-    if (lowp) so.eig_tol = std::isfinite(diff) ? std::max(base_eig_tol, std::min(so.eig_tol_low, 1e-3 * diff))
-                                               : base_eig_tol;
+    // step-to-step change can see: 1e-3 of |vals - oldvals|, between eig_tol and 1e-10 (the FP64 steps'
+    // adjoint assumes accurate eigenpairs: with tolerances near the low-precision steps' ones, e.g. 1e-5 for
+    // a BF16 switch at 1e-2, the reverse sweep's resolvent solves lose the gradient); the plain path keeps
+    // eig_tol on every step
+    if (lowp)
+      so.eig_tol = std::isfinite(diff) ? std::max(base_eig_tol, std::min(1e-10, 1e-3 * diff)) : base_eig_tol;
     const PBuf& ap = ap_at(c, t, so.prec);
     StepTape* tp = nullptr;
     if (keep_steps) {
