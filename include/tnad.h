@@ -17,6 +17,13 @@
  *    reported through *iters == maxit, as the reference's fixedpoint returns
  *    silently at maxit (src/fixedpoint.jl:33).
  *
+ * Size limits (checked on the host before any kernel runs; TNAD_ERR_UNSUPPORTED otherwise):
+ *  - CTMRG / energy: chi*D <= 8192, and chi <= 192 whenever chi*D > 192 (the chi kept eigenvectors
+ *    must fit the eigensolver's block of at most 192 columns).
+ *  - tnad_symeig: n <= 8192, and want <= 192 whenever n > 192.
+ *  - tnad_trg_svd / tnad_trg_svd_back: d1 d2 + d3 d4 <= 8192, and min(dmax, d1 d2, d3 d4) <= 95 whenever
+ *    d1 d2 + d3 d4 > 192; tnad_trg: chi <= 64.
+ *
  * The reference (TensorNetworkAD.jl) has no FFI; the functions below replace
  * the Julia methods cited next to each declaration (paths relative to the
  * reference repository).  The Julia-side ccall bindings and ChainRulesCore
@@ -101,14 +108,21 @@ int tnad_symeig(tnad_ctx* ctx, const double* A, int n, int want, int k, double* 
  * cpmat += cpmat', src/ctmrg.jl:135). */
 int tnad_svd_back_sym(tnad_ctx* ctx, const double* A, int n, int chi, int k, const double* G, double* Abar);
 
-/* ctmrgstep (src/ctmrg.jl:126-153): one step; vals has chi*D entries: all of them when chi*D <= 192
- * (full spectrum), otherwise the kept chi values followed by zeros. */
+/* ctmrgstep (src/ctmrg.jl:126-153): one step; vals has chi*D entries, s ./ s[1] sorted descending:
+ * all of them when chi*D <= 192 (full spectrum, as the reference), otherwise the k largest followed by
+ * zeros, k = eig_block or 2.25 chi capped at 192 (the chi kept values and the eigensolver's guard
+ * block). */
 int tnad_ctmrgstep(tnad_ctx* ctx, const double* a, int D, int chi, const double* C, const double* T,
                    const tnad_opts* opts, double* Cout, double* Tout, double* vals);
 
 /* ctmrg (src/ctmrg.jl:110-117 with fixedpoint/StopFunction, src/fixedpoint.jl:11-41).
- * C0/T0: initial environment (e.g. from tnad_init_raw).  If tape != NULL a checkpoint of every step is
- * kept on the device for tnad_ctmrg_back; free it with tnad_tape_free. */
+ * C0/T0: initial environment (e.g. from tnad_init_raw).  The StopFunction counter starts at -1
+ * (src/ctmrg.jl:114), so at most maxit + 1 steps run (maxit = 0: one step); a run stops earlier once
+ * |vals - oldvals|_2 <= tol.  vals is as for tnad_ctmrgstep: for chi*D > 192 the stop test sees the
+ * change of the k leading values only, which can stop a run one step before the reference when the
+ * change of the full spectrum lands just above tol (the leading k carry ~99.8% of its norm).
+ * If tape != NULL a checkpoint of every step is kept on the device for tnad_ctmrg_back; free it with
+ * tnad_tape_free. */
 int tnad_ctmrg(tnad_ctx* ctx, const double* a, int D, int chi, const double* C0, const double* T0, double tol,
                int maxit, const tnad_opts* opts, double* C, double* T, double* vals, int* iters,
                tnad_tape** tape);

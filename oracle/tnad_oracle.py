@@ -40,7 +40,7 @@ from dataclasses import dataclass, field
 import numpy as np
 
 ETA_SVD = 1e-40          # src/trg.jl:72 (Lorentzian broadening in svd_back)
-EPS0F0 = float(np.finfo(np.float32).smallest_subnormal)  # eps(0f0), src/autodiff.jl:131
+EPS0F0 = float(np.finfo(np.float32).smallest_subnormal)  # eps(0f0), src/autodiff.jl:26
 
 
 def ein(spec, *ops):
@@ -366,14 +366,15 @@ def ctmrg(a, C0, T0, tol, maxit, keep=False):
     """src/ctmrg.jl:110-117 with fixedpoint/StopFunction (src/fixedpoint.jl:11-41).
 
     StopFunction is evaluated on every state including the initial one: the counter
-    is incremented, the run stops once counter > maxit, otherwise when
-    norm(vals - oldvals) <= tol (L2); oldvals starts at Inf, so the first two
-    evaluations (NaN, Inf) never stop."""
+    (constructed as -1, src/ctmrg.jl:114) is incremented, the run stops once
+    counter > maxit, otherwise when norm(vals - oldvals) <= tol (L2); oldvals starts
+    at Inf, so the first two evaluations (NaN, Inf) never stop.  Hence at most
+    maxit + 1 steps, and maxit = 0 still takes one step."""
     chi, D = C0.shape[0], a.shape[0]
     oldvals = np.full(chi * D, np.inf)
     vals = oldvals
     C, T = C0, T0
-    counter = 0
+    counter = -1
     res = CtmrgResult(C, T, 0)
     while True:
         counter += 1

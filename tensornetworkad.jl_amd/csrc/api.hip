@@ -168,10 +168,7 @@ Prec to_prec_enum(int p) {
   throw Error(TNAD_ERR_ARG, "unknown precision");
 }
 
-void check_dims(int D, int chi) {
-  if (D < 1 || chi < 1) throw Error(TNAD_ERR_SHAPE, "bond dimensions must be positive");
-  if ((long long)D * chi > 8192) throw Error(TNAD_ERR_UNSUPPORTED, "chi*D above 8192 is not supported");
-}
+void check_dims(int D, int chi) { enforce(ctmrg_dims_limit(D, chi)); }
 
 const PBuf& ap_at(Ctx* c, tnad_tape& t, Prec p) {
   int i = (int)p;
@@ -213,7 +210,9 @@ void run_ctmrg(tnad_ctx* ctx, tnad_tape& t, const double* C0, const double* T0, 
   const double* X0 = nullptr;
   std::vector<double> oldvals(n, std::numeric_limits<double>::infinity());
   std::vector<double> vals = oldvals;
-  int counter = 0;
+  // StopFunction(oldvals, -1, tol, maxit) (src/ctmrg.jl:114): the counter starts at -1 and the test runs on
+  // the initial state too, so at most maxit + 1 steps are taken and maxit = 0 still takes one
+  int counter = -1;
   iters = 0;
   tnad_stats& st = ctx->stats;
   st = tnad_stats{};
@@ -345,7 +344,7 @@ int tnad_double_layer(tnad_ctx* ctx, const double* bulk, int d, int s, double* a
 
 int tnad_init_raw(tnad_ctx* ctx, const double* a, int D, int chi, double* C, double* T) {
   return guarded(ctx, [&] {
-    check_dims(D, chi);
+    enforce(env_dims_limit(D, chi));
     Ctx* c = &ctx->c;
     size_t nA = (size_t)D * D * D * D, nC = (size_t)chi * chi, nT = (size_t)chi * D * chi;
     DevBuf ba(c, nA * 8), bC(c, nC * 8), bT(c, nT * 8);
@@ -358,9 +357,9 @@ int tnad_init_raw(tnad_ctx* ctx, const double* a, int D, int chi, double* C, dou
 
 int tnad_symeig(tnad_ctx* ctx, const double* A, int n, int want, int k, double* evals, double* evecs, int* k_out) {
   return guarded(ctx, [&] {
-    if (n < 1 || want < 1 || want > n) throw Error(TNAD_ERR_ARG, "symeig: bad sizes");
+    enforce(symeig_limit(n, want));
     Ctx* c = &ctx->c;
-    int kk = (n <= JACOBI_MAX) ? n : std::min(std::max(k, want), std::min(n, JACOBI_MAX));
+    const int kk = eig_block_width(n, want, k);
     DevBuf bA(c, (size_t)n * n * 8), X(c, (size_t)n * kk * 8), AX(c, (size_t)n * kk * 8), th(c, (size_t)kk * 8);
     copy_in(c, bA.ptr, A, (size_t)n * n * 8);
     int kr = topk_eig(c, bA.as<double>(), n, want, kk, nullptr, 200, 1e-13, X.as<double>(), AX.as<double>(),
@@ -373,11 +372,11 @@ int tnad_symeig(tnad_ctx* ctx, const double* A, int n, int want, int k, double* 
 
 int tnad_svd_back_sym(tnad_ctx* ctx, const double* A, int n, int chi, int k, const double* G, double* Abar) {
   return guarded(ctx, [&] {
-    if (n < 1 || chi < 1 || chi > n) throw Error(TNAD_ERR_ARG, "svd_back: bad sizes");
+    enforce(svd_back_sym_limit(n, chi));
     Ctx* c = &ctx->c;
     StepDims d;
     d.chi = chi; d.D = n / chi; d.n = n;
-    int kk = (n <= JACOBI_MAX) ? n : std::min(std::max(k, chi), std::min(n, JACOBI_MAX));
+    const int kk = eig_block_width(n, chi, k);
     StepTape tp(c, d, Prec::F64, kk);
     copy_in(c, tp.M.ptr, A, (size_t)n * n * 8);
     tp.k = topk_eig(c, tp.M.as<double>(), n, chi, kk, nullptr, 200, 1e-13, tp.X.as<double>(), tp.AX.as<double>(),
@@ -504,7 +503,7 @@ int tnad_tape_free(tnad_tape* t) {
 int tnad_expectationvalue(tnad_ctx* ctx, const double* h, int s, const double* ap, int D, int chi, const double* C,
                           const double* T, double* e) {
   return guarded(ctx, [&] {
-    check_dims(D, chi);
+    enforce(env_dims_limit(D, chi));
     if (s < 1) throw Error(TNAD_ERR_SHAPE, "physical dimension must be positive");
     Ctx* c = &ctx->c;
     size_t nap = (size_t)D * D * D * D * s * s, nC = (size_t)chi * chi, nT = (size_t)chi * D * chi;

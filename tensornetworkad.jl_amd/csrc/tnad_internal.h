@@ -9,6 +9,7 @@
 #include <vector>
 #include <map>
 
+#include "tnad_limits.h"
 #include "tnad.h"
 
 #define TNAD_HIP_TRY(expr)                                                          \
@@ -77,7 +78,10 @@ void tc_launch(Ctx* c, Prec p, const TcDesc& d, const void* A, const void* B, vo
 // ---------------------------------------------------------------------------
 // Small dense kernels (eig.hip)
 // ---------------------------------------------------------------------------
-constexpr int JACOBI_MAX = 192;   // packed FP64 upper triangle of 192x192 fits in LDS
+// JACOBI_MAX (192): the packed FP64 upper triangle of 192x192 fits in LDS (tnad_limits.h)
+inline void enforce(const Limit& l) {
+  if (l) throw Error(l.code, l.msg);
+}
 
 // Full symmetric eigendecomposition of an m x m matrix (m <= JACOBI_MAX).
 // Output eigenpairs sorted by |lambda| descending; V column-major (ld = m).

@@ -576,7 +576,7 @@ void trg_svd_run(Ctx* c, const double* t, int d1, int d2, int d3, int d4, int dm
                  double* v, int* k_out) {
   if (d1 < 1 || d2 < 1 || d3 < 1 || d4 < 1 || dmax < 1) throw Error(TNAD_ERR_ARG, "trg_svd: bad sizes");
   const int m = d1 * d2, mp = d3 * d4;
-  if (m + mp > 8192) throw Error(TNAD_ERR_UNSUPPORTED, "trg_svd: d1 d2 + d3 d4 above 8192");
+  enforce(trg_split_limit(m, mp, dmax));
   const int kmax = std::min(dmax, std::min(m, mp));
   DevBuf bt(c, (size_t)m * mp * 8), bu(c, (size_t)m * kmax * 8 + 8), bv(c, (size_t)kmax * mp * 8 + 8);
   copy_in(c, bt.ptr, t, (size_t)m * mp * 8);
@@ -605,7 +605,7 @@ void trg_svd_back_run(Ctx* c, const double* t, int d1, int d2, int d3, int d4, i
                       const double* ubar, const double* vbar, int k, double* tbar) {
   if (d1 < 1 || d2 < 1 || d3 < 1 || d4 < 1 || dmax < 1 || k < 1) throw Error(TNAD_ERR_ARG, "trg_svd_back: bad sizes");
   const int m = d1 * d2, mp = d3 * d4;
-  if (m + mp > 8192) throw Error(TNAD_ERR_UNSUPPORTED, "trg_svd_back: d1 d2 + d3 d4 above 8192");
+  enforce(trg_split_limit(m, mp, dmax));
   const int kmax = std::min(dmax, std::min(m, mp));
   DevBuf bt(c, (size_t)m * mp * 8), bu(c, (size_t)m * kmax * 8 + 8), bv(c, (size_t)kmax * mp * 8 + 8);
   copy_in(c, bt.ptr, t, (size_t)m * mp * 8);

@@ -21,6 +21,7 @@ ERRORS = {
     -5: "TNAD_ERR_ALLOC",
     -6: "TNAD_ERR_UNSUPPORTED",
 }
+ERR_ARG, ERR_SHAPE, ERR_HIP, ERR_NAN, ERR_ALLOC, ERR_UNSUPPORTED = -1, -2, -3, -4, -5, -6
 PREC = {"fp64": 0, "fp32": 1, "bf16": 2}
 
 

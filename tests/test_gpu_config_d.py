@@ -1,8 +1,9 @@
 """Energy gradient at D=3, d=2, chi=64 (corner matrix 576 x 576: block subspace eigensolver in the
 forward, resolvent CG in the adjoint) against central finite differences, h = 1e-6, on 32 random entries
 of the bulk; plain FP64 and FP32 contractions (FP32 forward steps until the singular values settle, FP32
-reverse-sweep contractions).  The function differentiated is the energy after exactly N CTMRG steps
-(tol = 0), so the unrolled gradient and the finite differences see the same map; N is well past
+reverse-sweep contractions).  The function differentiated is the energy after a fixed number of CTMRG steps
+(tol = 0, maxit = N: N + 1 steps under the reference's StopFunction counter), so the unrolled gradient
+and the finite differences see the same map; N is well past
 convergence, so the result is also the fixed-point gradient."""
 import sys
 

@@ -1,6 +1,8 @@
 """GPU parity of the matrix-core contraction kernel (tnad_contract) against NumPy
 einsum: plain GEMM shapes (tile edges, K not a multiple of the K-tile) and every
 strided contraction pattern of the CTMRG step, its adjoint and the energy chain."""
+import zlib
+
 import numpy as np
 import pytest
 
@@ -47,7 +49,7 @@ def _dims(spec, chi, D, s):
 def test_step_contractions(ctx, prec, spec):
     import tnad
     print(f"case: contract {prec} {spec}", flush=True)
-    rng = np.random.default_rng(abs(hash(spec)) % 1000)
+    rng = np.random.default_rng(zlib.crc32(spec.encode()))   # reproducible across processes
     chi, D = 7, 3
     # extents: indices that carry the environment bond get chi, the others D
     chi_idx = {"iba,ad->ibd": "iad", "ibd,dcl->ibcl": "idl", "ibcl,jkcb->ijlk": "il",

@@ -36,6 +36,39 @@ def test_symmetrize_double_layer_init_raw(ctx):
     assert np.allclose(rt.corner, C0, atol=1e-14) and np.allclose(rt.edge, T0, atol=1e-14)
 
 
+@pytest.mark.parametrize("D,chi", [(2, 4), (5, 3), (4, 4)])
+def test_init_raw_non_symmetric_and_truncated(ctx, D, chi):
+    # the reference's doctest input (src/ctmrg.jl:52-60: randn(2,2,2,2), chi = 4) and D > chi, where
+    # src/ctmrg.jl:79-84 drops the entries outside the chi-sized arrays; non-symmetric inputs, so an
+    # index-order mistake on either side shows
+    import tnad
+    _say(f"case: init_raw non-symmetric D={D} chi={chi}")
+    a = np.random.default_rng(100 + D).standard_normal((D, D, D, D))
+    rt = tnad.SquareCTMRGRuntime.new(a, "raw", chi, ctx=ctx)
+    C0, T0 = O.init_raw(a, chi)
+    m = min(D, chi)
+    assert np.allclose(rt.corner[:m, :m], a.sum(axis=(2, 3))[:m, :m], atol=1e-13)
+    assert np.allclose(rt.edge[:m, :, :m], a.sum(axis=3)[:m, :, :m], atol=1e-13)
+    assert np.array_equal(rt.corner, C0) or np.allclose(rt.corner, C0, atol=1e-14)
+    assert np.allclose(rt.edge, T0, atol=1e-14)
+
+
+def test_unsupported_sizes_are_rejected_before_launch(ctx):
+    # host-side limits (csrc/tnad_limits.h): the kept block of the eigensolver is at most 192 wide above
+    # n = 192; these calls must fail with TNAD_ERR_UNSUPPORTED before anything is launched
+    import tnad
+    _say("case: size limits through the C ABI")
+    n = 200
+    A = np.random.default_rng(0).standard_normal((n, n))
+    A = A + A.T
+    with pytest.raises(tnad.TnadError) as ei:
+        _symeig(ctx, A, n, 193)
+    assert ei.value.code == tnad.ERR_UNSUPPORTED
+    with pytest.raises(tnad.TnadError) as ei:
+        tnad.trg_svd(np.ones((12, 12, 12, 12)), 96, 1e-16, ctx=ctx)
+    assert ei.value.code == tnad.ERR_UNSUPPORTED
+
+
 def test_dimension_mismatch_is_an_error(ctx):
     import tnad
     with pytest.raises(tnad.DimensionMismatch):
@@ -214,7 +247,7 @@ def test_ctmrg_tape_back_matches_energy_path(ctx):
         abar_o += ab
     rt, info, tape = tnad.ctmrg(tnad.SquareCTMRGRuntime(a, C0, T0), 0.0, 5, ctx=ctx, keep_tape=True)
     # the cotangent is gauge dependent: express it in the oracle's gauge by matching signs of C
-    assert info["iters"] == 5
+    assert info["iters"] == res.iters == 6   # maxit = 5: six steps (StopFunction counter from -1)
     abar = tape.back(*_gauge_match(rt, res, Cbar, Tbar))
     tape.free()
     assert np.max(np.abs(abar - abar_o)) < 1e-9 * np.max(np.abs(abar_o))
@@ -240,16 +273,21 @@ def _gauge_match(rt, res, Cbar, Tbar):
 
 
 # --- mixed precision -----------------------------------------------------------------------
-def test_mixed_fp32_energy_within_1e7(ctx):
+def test_mixed_fp32_energy_and_gradient_within_budget(ctx):
+    # FP32 contractions while the singular values change by more than switch_tol, then FP64 steps (their
+    # eigensolver tolerance tied to the step-to-step change); FP32 reverse sweep.  Energy within 1e-7 and
+    # gradient within 1e-5 (relative) of the plain FP64 path, the mixed-precision error budget.
     import tnad
     _say("case: mixed fp32 D=3 chi=24")
     h = O.hamiltonian_heisenberg()
     b = _bulk(13, d=3)
-    e64 = tnad.energy(h, b, 24, 1e-10, 100, ctx=ctx)
-    e32, _, info = tnad.energy_and_grad(h, b, 24, 1e-10, 100, ctx=ctx, want_grad=False, return_info=True,
-                                        **tnad.MIXED_FP32)
+    e64, g64 = tnad.energy_and_grad(h, b, 24, 1e-10, 100, ctx=ctx)
+    e32, g32, info = tnad.energy_and_grad(h, b, 24, 1e-10, 100, ctx=ctx, return_info=True, **tnad.MIXED_FP32)
     assert info["low_prec_iters"] > 0
+    rel = np.linalg.norm(g32 - g64) / np.linalg.norm(g64)
+    _say(f"fp32 mixed: |dE| {abs(e32 - e64):.2e}, gradient relative difference {rel:.2e}")
     assert abs(e32 - e64) < 1e-7
+    assert rel < 1e-5
     e_o = O.energy(h, b, 24, 1e-10, 100)
     assert abs(e64 - e_o) < 1e-9
 

@@ -170,6 +170,57 @@ def test_heisenberg_matches_readme_print():
     assert np.allclose(h[:, :, 1, 1], [[0.5, 0.0], [0.0, -0.5]])
 
 
+@pytest.mark.parametrize("hx", [0.0, 0.5, 1.0, 2.0])
+def test_tfising_two_site_spectrum(hx):
+    # hamiltonian(TFIsing(hx)) (src/hamiltonianmodels.jl:28-33) is the two-site operator
+    # -2 Z Z - hx/2 (X 1 + 1 X) with site one on indices (i, j) and site two on (k, l).  Its spectrum in
+    # closed form: the singlet (+2), (|uu> - |dd>)/sqrt2 (-2) and the pair +-sqrt(4 + hx^2) from
+    # {(|uu> + |dd>)/sqrt2, triplet 0} coupled by -hx.  A dropped /2, a wrong sign or a transposed site
+    # changes it.
+    h = O.hamiltonian_tfising(hx)
+    H = np.transpose(h, (0, 2, 1, 3)).reshape(4, 4)     # rows (i, k), columns (j, l)
+    assert np.allclose(H, H.T)
+    ev = np.sort(np.linalg.eigvalsh(H))
+    r = math.sqrt(4 + hx * hx)
+    assert np.allclose(ev, np.sort([2.0, -2.0, r, -r]), atol=1e-14)
+    # the ferromagnetic product state |uu> has <h> = -2 at any hx (the field has no diagonal part)
+    assert H[0, 0] == pytest.approx(-2.0)
+
+
+def test_init_raw_matches_reference_doctest():
+    # src/ctmrg.jl:52-60: SquareCTMRGRuntime(randn(2,2,2,2), Val(:raw), 4) has
+    # corner[1:2,1:2] = sum(bulk, dims=(3,4)) and edge[1:2,1:2,1:2] = sum(bulk, dims=4); the rest of the
+    # chi x chi / chi x D x chi arrays are the zeros they were created as (:77-78).  A non-symmetric
+    # bulk, so an index-order mistake shows.
+    rng = np.random.default_rng(52)
+    a = rng.standard_normal((2, 2, 2, 2))
+    C, T = O.init_raw(a, 4)
+    assert C.shape == (4, 4) and T.shape == (4, 2, 4)
+    Cref = np.zeros((4, 4))
+    Tref = np.zeros((4, 2, 4))
+    for i in range(2):
+        for j in range(2):
+            Cref[i, j] = sum(a[i, j, k, l] for k in range(2) for l in range(2))
+            for k in range(2):
+                Tref[i, j, k] = sum(a[i, j, k, l] for l in range(2))
+    assert np.allclose(C, Cref, atol=1e-15) and np.allclose(T, Tref, atol=1e-15)
+    assert not np.allclose(C[:2, :2], C[:2, :2].T)   # the input really is non-symmetric
+
+
+def test_init_raw_truncates_when_D_exceeds_chi():
+    # src/ctmrg.jl:79-84: entries of cinit / tinit outside CartesianIndices(corner / edge) are dropped
+    rng = np.random.default_rng(53)
+    a = rng.standard_normal((5, 5, 5, 5))
+    C, T = O.init_raw(a, 3)
+    assert C.shape == (3, 3) and T.shape == (3, 5, 3)
+    for i in range(3):
+        for j in range(3):
+            assert C[i, j] == pytest.approx(a[i, j].sum(), abs=1e-13)
+        for j in range(5):
+            for k in range(3):
+                assert T[i, j, k] == pytest.approx(a[i, j, k].sum(), abs=1e-13)
+
+
 def test_symmetrize_is_d4_invariant_and_normalised():
     rng = np.random.default_rng(4)
     x = O.indexperm_symmetrize(rng.random((2, 2, 2, 2, 2)))
@@ -203,13 +254,18 @@ def test_ctmrgstep_invariants():
 
 
 def test_stopfunction_semantics():
-    # src/fixedpoint.jl:31-41: maxit bounds the steps exactly; the first two states never stop
+    # src/ctmrg.jl:114 builds StopFunction(oldvals, -1, tol, maxit) and src/fixedpoint.jl:12-14 applies it
+    # to every state of iterated(f, guess), the initial one included: with the counter at -1, the state
+    # after step m sees counter = m, so tol = 0 runs maxit + 1 steps and maxit = 0 runs one; the first two
+    # states never stop (NaN, then Inf)
     b = O.indexperm_symmetrize(_rand_bulk(1))
     ap, a = O.double_layer(b)
     C, T = O.init_raw(a, 4)
     assert O.ctmrg(a, C, T, tol=1e9, maxit=100).iters == 2
-    assert O.ctmrg(a, C, T, tol=0.0, maxit=7).iters == 7
-    assert O.ctmrg(a, C, T, tol=0.0, maxit=0).iters == 0
+    assert O.ctmrg(a, C, T, tol=1e9, maxit=1).iters == 2
+    assert O.ctmrg(a, C, T, tol=0.0, maxit=7).iters == 8
+    assert O.ctmrg(a, C, T, tol=0.0, maxit=0).iters == 1
+    assert O.ctmrg(a, C, T, tol=0.0, maxit=3).iters == 4
 
 
 def test_product_state_energies():
