@@ -209,14 +209,17 @@ def double_layer(bulk):
 
 
 def double_layer_back(bulk, apbar, abar):
-    """Adjoint of double_layer for real bulk: cotangents of ap (6-index) and a."""
+    """Adjoint of double_layer: cotangents of ap (6-index) and a -> cotangent of the bulk.
+
+    Cotangent convention for complex arrays (the one of Zygote's gradients): xbar with
+    dL = Re sum conj(xbar) dx, i.e. xbar = dL/dRe x + i dL/dIm x.  For ap = ein(A, conj(A)) the
+    first operand receives ein(apbar, A) and the second, through the conj, conj(ein(apbar, conj(A)))."""
     d = bulk.shape[0]
     s = bulk.shape[4]
-    D = d * d
     g6 = apbar + ein("ijkl,xy->ijklxy", abar, np.eye(s))
     g10 = reshape_f(g6, (d, d, d, d, d, d, d, d, s, s))
-    gb = ein("aibjckdlxy,ijkly->abcdx", g10, bulk.conj())
-    gb = gb + ein("aibjckdlxy,abcdx->ijkly", g10, bulk).conj()
+    gb = ein("aibjckdlxy,ijkly->abcdx", g10, bulk)
+    gb = gb + ein("aibjckdlxy,abcdx->ijkly", g10.conj(), bulk)
     return gb
 
 
@@ -260,8 +263,47 @@ class StepCache:
     T2: np.ndarray
 
 
+def phase_weights(n):
+    """Weights of the complex phase rule: w_r = 1 + frac(0.6180339887 r), r = 0..n-1."""
+    r = np.arange(n, dtype=np.float64)
+    return 1.0 + np.mod(0.6180339887 * r, 1.0)
+
+
+def fix_phases(U):
+    """Phase gauge of complex singular / eigenvectors: column j is multiplied by conj(q)/|q| with
+    q = sum_r w_r U[r, j] (phase_weights), so that q becomes real positive; when |q| <= 1e-6 |U_j|
+    the largest-modulus entry (first on ties) is made real positive instead.  Real columns stay real
+    (the phase is a sign).  Returns (U', phases)."""
+    n, k = U.shape
+    w = phase_weights(n)
+    ph = np.ones(k, dtype=U.dtype)
+    for j in range(k):
+        q = np.sum(w * U[:, j])
+        if abs(q) <= 1e-6 * np.linalg.norm(U[:, j]):
+            q = U[int(np.argmax(np.abs(U[:, j]))), j]
+        if q != 0:
+            ph[j] = np.conj(q) / abs(q)
+    return U * ph[None, :], ph
+
+
+def herm_svd(M):
+    """SVD of a Hermitian matrix through its eigendecomposition: U = eigenvectors sorted by |lambda|
+    descending (stable), s = |lambda|, V = U sign(lambda), phases fixed by fix_phases.  For a complex
+    Hermitian matrix the reference's result depends on LAPACK's singular-vector phases (the energy
+    contracts l with l, not conj(l), src/variationalipeps.jl:53-54, so it is invariant under real
+    orthogonal but not under unitary changes of the environment's gauge): this rule is the oracle's
+    fixed choice, which the GPU path follows."""
+    lam, X = np.linalg.eigh(M)
+    o = np.argsort(-np.abs(lam), kind="stable")
+    lam, X = lam[o], X[:, o]
+    X, _ = fix_phases(X)
+    sg = np.where(lam < 0, -1.0, 1.0)
+    return X, np.abs(lam), (X * sg[None, :]).conj().T
+
+
 def ctmrgstep(a, C, T, keep=False):
-    """src/ctmrg.jl:126-153.  Returns (C', T', vals) [and the cache for the reverse sweep]."""
+    """src/ctmrg.jl:126-153.  Returns (C', T', vals) [and the cache for the reverse sweep].
+    Real tensors: LAPACK's SVD.  Complex tensors: herm_svd (fixed phase rule)."""
     chi = C.shape[0]
     D = a.shape[0]
     n = chi * D
@@ -269,7 +311,10 @@ def ctmrgstep(a, C, T, keep=False):
     tp = ein("iam,jkla->ijklm", T, a)                        # :131
     cpmat = reshape_f(cp, (n, n))                            # :134
     cpmat = cpmat + cpmat.conj().T                           # :135
-    U, s, Vh = np.linalg.svd(cpmat)                          # :136 (LAPACK gesdd)
+    if np.iscomplexobj(cpmat):
+        U, s, Vh = herm_svd(cpmat)                           # :136
+    else:
+        U, s, Vh = np.linalg.svd(cpmat)                      # :136 (LAPACK gesdd)
     z = reshape_f(U[:, :chi], (chi, D, chi))                 # :137
     C1 = ein("abcd,abi,cdj->ij", cp, z.conj(), z)            # :139
     T1 = ein("abjcd,abi,dck->ijk", tp, z.conj(), z)          # :140
@@ -320,36 +365,43 @@ def _norm_back(x, ybar):
 
 
 def ctmrgstep_back(a, cache: StepCache, C3bar, T3bar):
-    """Reverse sweep of one ctmrgstep (real arithmetic).  Returns (Cbar, Tbar, abar)."""
+    """Reverse sweep of one ctmrgstep.  Returns (Cbar, Tbar, abar).
+
+    Real or complex: every einsum adjoint contracts the cotangent with the complex conjugates of the
+    other operands (the conj of a holomorphic map's derivative); an operand entering as conj(z)
+    passes the conjugate of its cotangent on to z.  On real arrays every conj is the identity."""
     C, T, cp, tp, U, s, V, z = cache.C, cache.T, cache.cp, cache.tp, cache.U, cache.s, cache.V, cache.z
     chi = C.shape[0]
     D = a.shape[0]
     n = chi * D
+    cj = np.conj
     C2bar = _norm_back(cache.C2, C3bar)
     T2bar = _norm_back(cache.T2, T3bar)
-    C1bar = C2bar + C2bar.T
-    T1bar = T2bar + ein("ijk->kji", T2bar)
-    # T1 = ein("abjcd,abi,dck->ijk", tp, z, z)
-    tpbar = ein("ijk,abi,dck->abjcd", T1bar, z, z)
-    zbar = ein("abjcd,dck,ijk->abi", tp, z, T1bar)
-    zbar = zbar + ein("abjcd,abi,ijk->dck", tp, z, T1bar)
-    # C1 = ein("abcd,abi,cdj->ij", cp, z, z)
-    cpbar = ein("ij,abi,cdj->abcd", C1bar, z, z)
-    zbar = zbar + ein("abcd,cdj,ij->abi", cp, z, C1bar)
-    zbar = zbar + ein("abcd,abi,ij->cdj", cp, z, C1bar)
-    dU = np.zeros((n, n))
+    C1bar = C2bar + C2bar.conj().T                               # C2 = C1 + C1'
+    T1bar = T2bar + ein("ijk->kji", T2bar).conj()                # T2 = T1 + conj(T1^{kji})
+    zc = z.conj()
+    # T1 = ein("abjcd,abi,dck->ijk", tp, conj(z), z)
+    tpbar = ein("ijk,abi,dck->abjcd", T1bar, cj(zc), cj(z))
+    zcbar = ein("abjcd,dck,ijk->abi", cj(tp), cj(z), T1bar)
+    zbar = ein("abjcd,abi,ijk->dck", cj(tp), cj(zc), T1bar)
+    # C1 = ein("abcd,abi,cdj->ij", cp, conj(z), z)
+    cpbar = ein("ij,abi,cdj->abcd", C1bar, cj(zc), cj(z))
+    zcbar = zcbar + ein("abcd,cdj,ij->abi", cj(cp), cj(z), C1bar)
+    zbar = zbar + ein("abcd,abi,ij->cdj", cj(cp), cj(zc), C1bar)
+    zbar = zbar + zcbar.conj()
+    dU = np.zeros((n, n), dtype=np.result_type(U, zbar))
     dU[:, :chi] = reshape_f(zbar, (n, chi))
     Mbar = svd_back(U, s, V, dU, None, None)
-    cpmatbar = Mbar + Mbar.T
+    cpmatbar = Mbar + Mbar.conj().T                              # cpmat += cpmat'
     cpbar = cpbar + reshape_f(cpmatbar, (chi, D, chi, D))
     # tp = ein("iam,jkla->ijklm", T, a)
-    Tbar = ein("ijklm,jkla->iam", tpbar, a)
-    abar = ein("ijklm,iam->jkla", tpbar, T)
+    Tbar = ein("ijklm,jkla->iam", tpbar, cj(a))
+    abar = ein("ijklm,iam->jkla", tpbar, cj(T))
     # cp = ein("ad,iba,dcl,jkcb->ijlk", C, T, T, a)
-    Cbar = ein("ijlk,iba,dcl,jkcb->ad", cpbar, T, T, a)
-    Tbar = Tbar + ein("ijlk,ad,dcl,jkcb->iba", cpbar, C, T, a)
-    Tbar = Tbar + ein("ijlk,ad,iba,jkcb->dcl", cpbar, C, T, a)
-    abar = abar + ein("ijlk,ad,iba,dcl->jkcb", cpbar, C, T, T)
+    Cbar = ein("ijlk,iba,dcl,jkcb->ad", cpbar, cj(T), cj(T), cj(a))
+    Tbar = Tbar + ein("ijlk,ad,dcl,jkcb->iba", cpbar, cj(C), cj(T), cj(a))
+    Tbar = Tbar + ein("ijlk,ad,iba,jkcb->dcl", cpbar, cj(C), cj(T), cj(a))
+    abar = abar + ein("ijlk,ad,iba,dcl->jkcb", cpbar, cj(C), cj(T), cj(T))
     return Cbar, Tbar, abar
 
 
@@ -418,33 +470,35 @@ def expectationvalue(h, ap, C, T):
 
 
 def expectationvalue_back(h, ap, C, T, Ebar=1.0):
-    """Reverse sweep of expectationvalue.  Returns (apbar, Cbar, Tbar)."""
+    """Reverse sweep of expectationvalue for the loss Re(Ebar * e/n) (Re(e/n) is what optimiseipeps
+    differentiates, src/variationalipeps.jl:71; for real tensors it is e/n).  Returns (apbar, Cbar, Tbar)."""
+    cj = np.conj
     nrm = np.linalg.norm(ap)
     apn = ap / nrm
     l, (E1, E2, E3, E4) = _half_plaquette(C, T, apn)
     e = ein("abcij,abckl,ijkl->", l, l, h)
     tr = ein("ijkaa->ijk", l)
     nn = np.sum(tr * tr)
-    ebar = Ebar / nn
-    nnbar = -Ebar * e / nn ** 2
-    lbar = ebar * (ein("abckl,ijkl->abcij", l, h) + ein("abcij,ijkl->abckl", l, h))
+    ebar = cj(Ebar / nn)                 # q = e/n, qbar = Ebar: ebar = conj(dq/de) qbar
+    nnbar = cj(-Ebar * e / nn ** 2)
+    lbar = ebar * cj(ein("abckl,ijkl->abcij", l, h) + ein("abcij,ijkl->abckl", l, h))
     s = l.shape[3]
-    lbar = lbar + 2 * nnbar * ein("ijk,ab->ijkab", tr, np.eye(s))
+    lbar = lbar + 2 * nnbar * ein("ijk,ab->ijkab", cj(tr), np.eye(s))
     # l = E4 E3
-    E4bar = ein("ijklm,efk->iejflm", lbar, E3)
-    E3bar = ein("iejflm,ijklm->efk", E4, lbar)
+    E4bar = ein("ijklm,efk->iejflm", lbar, cj(E3))
+    E3bar = ein("iejflm,ijklm->efk", cj(E4), lbar)
     # E4 = E2 apn
-    E2bar = ein("iejflm,cjfdlm->icde", E4bar, apn)
-    apnbar = ein("icde,iejflm->cjfdlm", E2, E4bar)
+    E2bar = ein("iejflm,cjfdlm->icde", E4bar, cj(apn))
+    apnbar = ein("icde,iejflm->cjfdlm", cj(E2), E4bar)
     # E3 = C T
-    Cbar = ein("efk,gfk->eg", E3bar, T)
-    Tbar = ein("eg,efk->gfk", C, E3bar)
+    Cbar = ein("efk,gfk->eg", E3bar, cj(T))
+    Tbar = ein("eg,efk->gfk", cj(C), E3bar)
     # E2 = E1 T
-    E1bar = ein("icde,bde->icb", E2bar, T)
-    Tbar = Tbar + ein("icb,icde->bde", E1, E2bar)
+    E1bar = ein("icde,bde->icb", E2bar, cj(T))
+    Tbar = Tbar + ein("icb,icde->bde", cj(E1), E2bar)
     # E1 = T C
-    Tbar = Tbar + ein("icb,ab->ica", E1bar, C)
-    Cbar = Cbar + ein("ica,icb->ab", T, E1bar)
+    Tbar = Tbar + ein("icb,ab->ica", E1bar, cj(C))
+    Cbar = Cbar + ein("ica,icb->ab", cj(T), E1bar)
     apbar = _norm_back(ap, apnbar)
     return apbar, Cbar, Tbar
 
@@ -480,7 +534,9 @@ def raw_env_of(bulk, chi):
 def energy_and_grad(h, bulk, chi, tol, maxit):
     """Energy and d energy / d bulk of the unrolled program (what Zygote computes for
     src/variationalipeps.jl:73): the :raw init is @nograd (src/autodiff.jl:5), vals
-    only feed the @nograd StopFunction (src/autodiff.jl:4)."""
+    only feed the @nograd StopFunction (src/autodiff.jl:4).  For a complex bulk the energy is
+    complex (e/n) and the gradient is that of Re(energy) (src/variationalipeps.jl:71), in Zygote's
+    convention dRe(e)/dRe x + i dRe(e)/dIm x."""
     b = indexperm_symmetrize(bulk)
     ap, a = double_layer(b)
     C0, T0 = init_raw(a, chi)
@@ -530,7 +586,9 @@ def trg_svd(t, dmax, tol, keep=False):
 
 
 def trg_svd_back(cache, ubar, vbar):
-    """Reverse of trg_svd through svd_back (dU and dV both present)."""
+    """Reverse of trg_svd through svd_back (dU and dV both present).  Real or complex (cotangent
+    convention of double_layer_back): u = U sqrt(s) and v = sqrt(s) V' give dU = ubar sqrt(s),
+    dV' = vbar sqrt(s) and, for the real s, dS = Re(<U, ubar> + <V', vbar>) / (2 sqrt(s))."""
     U, s, V, k, shape = cache
     d1, d2, d3, d4 = shape
     sq = np.sqrt(s[:k])
@@ -540,7 +598,7 @@ def trg_svd_back(cache, ubar, vbar):
     dU = np.zeros_like(U)
     dU[:, :k] = ub * sq[None, :]
     dS = np.zeros_like(s)
-    sqbar = np.sum(ub * U[:, :k], axis=0) + np.sum(vb * Vh[:k, :], axis=1)
+    sqbar = np.real(np.sum(ub.conj() * U[:, :k], axis=0) + np.sum(vb.conj() * Vh[:k, :], axis=1))
     dS[:k] = sqbar / (2 * sq)
     dVh = np.zeros_like(Vh)
     dVh[:k, :] = vb * sq[:, None]
@@ -567,30 +625,33 @@ def trg(a, chi, niter, tol=1e-16, with_grad_seed=None):
         a = ein("npu,por,dom,lmn->urdl", dr, ld, ul, ru)
         tape.append((a_in, idx, maxval, c1, c2, dr, ul, ld, ru))
     trace = ein("ijij->", a)
-    lnz += math.log(trace) / 2.0 ** niter
+    cplx = np.iscomplexobj(a)
+    lnz += (np.log(trace) if cplx else math.log(trace)) / 2.0 ** niter
     if with_grad_seed is None:
         return lnz
-    # reverse sweep
+    # reverse sweep of Re(lnZ) (the loss of test/trg.jl:26-27 for complex tensors; lnZ itself for real)
+    cj = np.conj
     abar = np.zeros_like(a)
     for i in range(a.shape[0]):
         for j in range(a.shape[1]):
             abar[i, j, i, j] = 1.0
-    abar *= 1.0 / (2.0 ** niter * trace)
+    abar *= cj(1.0 / (2.0 ** niter * trace))
     for n in range(niter, 0, -1):
         a_in, idx, maxval, c1, c2, dr, ul, ld, ru = tape[n - 1]
-        drbar = ein("urdl,por,dom,lmn->npu", abar, ld, ul, ru)
-        ldbar = ein("urdl,npu,dom,lmn->por", abar, dr, ul, ru)
-        ulbar = ein("urdl,npu,por,lmn->dom", abar, dr, ld, ru)
-        rubar = ein("urdl,npu,por,dom->lmn", abar, dr, ld, ul)
+        drbar = ein("urdl,por,dom,lmn->npu", abar, cj(ld), cj(ul), cj(ru))
+        ldbar = ein("urdl,npu,dom,lmn->por", abar, cj(dr), cj(ul), cj(ru))
+        ulbar = ein("urdl,npu,por,lmn->dom", abar, cj(dr), cj(ld), cj(ru))
+        rubar = ein("urdl,npu,por,dom->lmn", abar, cj(dr), cj(ld), cj(ul))
         g1 = trg_svd_back(c1, drbar, ulbar)   # d/d dr_ul
         g2 = trg_svd_back(c2, ldbar, rubar)   # d/d ld_ru
         anbar = ein("drul->urdl", g1) + ein("ldru->urdl", g2)
-        # a_n = a_in / maxval ; lnz += 2^(1-n) log(maxval)
-        mbar = 2.0 ** (1 - n) / maxval - np.sum(anbar * a_in) / maxval ** 2
+        # a_n = a_in / maxval ; lnz += 2^(1-n) log(maxval); maxval = |a_in[idx]| (real)
+        mbar = 2.0 ** (1 - n) / maxval - np.real(np.sum(anbar.conj() * a_in)) / maxval ** 2
         abar = anbar / maxval
         flat = abar.reshape(-1)
-        flat[idx] += mbar * np.sign(a_in.flat[idx])
-    return lnz, float(np.sum(abar * with_grad_seed))
+        z = a_in.flat[idx]
+        flat[idx] += mbar * (z / abs(z))
+    return lnz, float(np.real(np.sum(abar.conj() * with_grad_seed)))
 
 
 def trg_and_grad(beta, chi, niter, tol=1e-16):

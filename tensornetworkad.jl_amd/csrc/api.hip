@@ -10,6 +10,7 @@
 #include "api_internal.h"
 #include "ctmrg.h"
 #include "trg.h"
+#include "zc.h"
 
 namespace tnad {
 
@@ -210,6 +211,11 @@ void run_ctmrg(tnad_ctx* ctx, tnad_tape& t, const double* C0, const double* T0, 
   const double* X0 = nullptr;
   std::vector<double> oldvals(n, std::numeric_limits<double>::infinity());
   std::vector<double> vals = oldvals;
+  // n > 192: the spectrum beyond the chi kept values enters through its norm (StepState::rest)
+  const bool partial = n > JACOBI_MAX;
+  const int nk = partial ? chi : n;
+  double rest = std::numeric_limits<double>::infinity(), rest_err = 0.0;
+  double oldrest = rest, oldrest_err = 0.0;
   // StopFunction(oldvals, -1, tol, maxit) (src/ctmrg.jl:114): the counter starts at -1 and the test runs on
   // the initial state too, so at most maxit + 1 steps are taken and maxit = 0 still takes one
   int counter = -1;
@@ -221,15 +227,25 @@ void run_ctmrg(tnad_ctx* ctx, tnad_tape& t, const double* C0, const double* T0, 
     // StopFunction (src/fixedpoint.jl:31-41), evaluated on every state including the first
     counter += 1;
     if (counter > maxit) break;
+    // n <= 192: |vals - oldvals|_2 over the whole spectrum, as the reference.  n > 192: the change of the
+    // chi kept values plus | |rest| - |oldrest| | (less both error bounds) for the discarded ones, a lower
+    // bound on the reference's norm (triangle inequality): a run never stops after the reference, and
+    // before it only when the discarded values move by more than their norm shows
     double diff2 = 0.0;
-    for (int i = 0; i < n; ++i) {
+    for (int i = 0; i < nk; ++i) {
       double dv = vals[i] - oldvals[i];
       diff2 += dv * dv;
+    }
+    if (partial) {
+      double lb = std::fabs(rest - oldrest) - rest_err - oldrest_err;   // Inf while oldrest is Inf
+      if (lb > 0.0) diff2 += lb * lb;
     }
     double diff = std::sqrt(diff2);   // NaN for the first state (Inf - Inf), Inf for the second
     if (diff <= tol) break;
     if (!switched && diff <= o.switch_tol) switched = true;
     oldvals = vals;
+    oldrest = rest;
+    oldrest_err = rest_err;
     so.prec = switched ? Prec::F64 : low;
     // in a mixed run an FP64 step before convergence resolves its eigenvectors only as far as the
     // step-to-step change can see: 1e-3 of |vals - oldvals|, between eig_tol and 1e-10 (the FP64 steps'
@@ -254,6 +270,8 @@ void run_ctmrg(tnad_ctx* ctx, tnad_tape& t, const double* C0, const double* T0, 
     ctmrg_step(c, d, so, t.a64.as<double>(), ap, Ca.as<double>(), Ta.as<double>(), warm ? X0 : nullptr, X_keep,
                Cb.as<double>(), Tb.as<double>(), vals, tp, ss);
     if (warm) X0 = tp ? tp->X.as<double>() : Xprev.as<double>();
+    rest = ss.rest;
+    rest_err = ss.rest_err;
     std::swap(Ca, Cb);
     std::swap(Ta, Tb);
     iters += 1;
@@ -367,6 +385,67 @@ int tnad_symeig(tnad_ctx* ctx, const double* A, int n, int want, int k, double* 
     if (evals) copy_out(c, evals, th.ptr, (size_t)kr * 8);
     if (evecs) copy_out(c, evecs, X.ptr, (size_t)n * kr * 8);
     if (k_out) *k_out = kr;
+  });
+}
+
+int tnad_energy_z(tnad_ctx* ctx, const double* h, const double* bulk, int d, int s, int chi, double tol, int maxit,
+                  double* e, double* grad, int* iters) {
+  return guarded(ctx, [&] {
+    if (d < 1 || s < 1) throw Error(TNAD_ERR_SHAPE, "size of tensor error, should be (d, d, d, d, s)");
+    enforce(zctmrg_dims_limit((long long)d * d, chi));
+    if (!h || !bulk) throw Error(TNAD_ERR_ARG, "null input");
+    Ctx* c = &ctx->c;
+    const size_t nb5 = (size_t)d * d * d * d * s, nh = (size_t)s * s * s * s;
+    DevBuf hb(c, 2 * nh * 8), bb(c, 2 * nb5 * 8), gb;
+    z_from_interleaved(c, h, nh, hb.as<double>());
+    z_from_interleaved(c, bulk, nb5, bb.as<double>());
+    if (grad) gb = DevBuf(c, 2 * nb5 * 8);
+    double eh[2] = {0.0, 0.0};
+    int it = 0;
+    zenergy_run(c, hb.as<double>(), bb.as<double>(), d, s, chi, tol, maxit, eh, grad ? gb.as<double>() : nullptr, it);
+    if (grad) z_to_interleaved(c, gb.as<double>(), nb5, grad);
+    if (e) { e[0] = eh[0]; e[1] = eh[1]; }
+    if (iters) *iters = it;
+    ctx->stats = tnad_stats{};
+    ctx->stats.iters = it;
+    ctx->stats.eig_k = chi * d * d;
+  });
+}
+
+int tnad_svd_z(tnad_ctx* ctx, const double* A, int m, int p, double* U, double* S, double* V) {
+  return guarded(ctx, [&] {
+    enforce(zsvd_limit(m, p));
+    if (!A) throw Error(TNAD_ERR_ARG, "null input");
+    Ctx* c = &ctx->c;
+    const int K = std::min(m, p);
+    DevBuf Ab(c, 2 * (size_t)m * p * 8), Ub(c, 2 * (size_t)m * K * 8), Vb(c, 2 * (size_t)p * K * 8), Sb(c, (size_t)K * 8);
+    z_from_interleaved(c, A, (size_t)m * p, Ab.as<double>());
+    zsvd_small(c, Ab.as<double>(), m, p, Ub.as<double>(), Sb.as<double>(), Vb.as<double>());
+    if (U) z_to_interleaved(c, Ub.as<double>(), (size_t)m * K, U);
+    if (V) z_to_interleaved(c, Vb.as<double>(), (size_t)p * K, V);
+    if (S) copy_out(c, S, Sb.ptr, (size_t)K * 8);
+  });
+}
+
+int tnad_svd_back_z(tnad_ctx* ctx, const double* A, int m, int p, const double* dU, const double* dS, const double* dV,
+                    double* dA) {
+  return guarded(ctx, [&] {
+    enforce(zsvd_limit(m, p));
+    if (!A || !dA) throw Error(TNAD_ERR_ARG, "null input");
+    Ctx* c = &ctx->c;
+    const int K = std::min(m, p);
+    DevBuf Ab(c, 2 * (size_t)m * p * 8), dAb(c, 2 * (size_t)m * p * 8), dUb, dVb, dSb;
+    z_from_interleaved(c, A, (size_t)m * p, Ab.as<double>());
+    if (dU) { dUb = DevBuf(c, 2 * (size_t)m * K * 8); z_from_interleaved(c, dU, (size_t)m * K, dUb.as<double>()); }
+    if (dV) { dVb = DevBuf(c, 2 * (size_t)p * K * 8); z_from_interleaved(c, dV, (size_t)p * K, dVb.as<double>()); }
+    if (dS) { dSb = DevBuf(c, (size_t)K * 8); copy_in(c, dSb.ptr, dS, (size_t)K * 8); }
+    if (!dU && !dS && !dV) {   // svd_back returns nothing (src/trg.jl:73): a zero cotangent
+      k_fill(dAb.as<double>(), 2 * (size_t)m * p, 0.0, c->stream);
+    } else {
+      zsvd_back_small(c, Ab.as<double>(), m, p, dU ? dUb.as<double>() : nullptr, dS ? dSb.as<double>() : nullptr,
+                      dV ? dVb.as<double>() : nullptr, dAb.as<double>());
+    }
+    z_to_interleaved(c, dAb.as<double>(), (size_t)m * p, dA);
   });
 }
 

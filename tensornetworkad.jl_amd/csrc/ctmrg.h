@@ -43,6 +43,9 @@ struct StepState {
   int k = 0;
   int eig_iters = 0;
   double eig_resid = 0.0;
+  // n > 192 only: norm of the singular values beyond the chi kept ones, relative to s[1]
+  // (|M - Z Theta Z^T|_F / s[1]), and a bound on its rounding and deflation error
+  double rest = 0.0, rest_err = 0.0;
 };
 
 struct EigStats {

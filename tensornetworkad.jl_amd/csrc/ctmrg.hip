@@ -633,13 +633,33 @@ void ctmrg_step(Ctx* c, const StepDims& d, const StepOpts& o, const double* a64,
     k_copy(T, tape->T.as<double>(), (size_t)chi * D * chi, s);
   }
   // vals = s ./ s[1]   (src/ctmrg.jl:142).  With the whole spectrum computed (n <= 192) all n
-  // values; otherwise the k largest (the chi kept values and the guard block's Ritz values), with the
-  // rest 0.  The stop test then measures the change of the leading k values: over the oracle's runs
-  // at D = 3, chi = 24 (k = 54 of n = 216) that is 99.8% of the norm of the change of the full vector
-  // at every step (the chi kept values alone: 98.7%), so a run can stop one step before the reference
-  // only when the full change lands within that last 0.2% above tol.
+  // values; otherwise the k largest (the chi kept values, resolved to the eigensolver tolerance, and the
+  // guard block's Ritz values, which are not), with the rest 0.  For n > 192 the step also measures the
+  // norm of the spectrum beyond the kept chi values, rest = |M - Z Theta Z^T|_F / s[1] (Z the kept
+  // eigenvectors): formed elementwise, it is accurate to ~n eps |M|_F even where rest is far below
+  // sqrt(eps) (|M|_F^2 - sum theta^2 would lose it to cancellation).  The stop test uses it as a lower
+  // bound on the change of the discarded values (run_ctmrg).
   const int nv = k;
   std::vector<double> th(nv);
+  double sq[2] = {0.0, 0.0};
+  DevBuf E, XT, red1, red2, sqd;
+  if (n > JACOBI_MAX) {
+    XT = DevBuf(c, (size_t)n * chi * 8);
+    E = DevBuf(c, (size_t)n * n * 8);
+    red1 = DevBuf(c, 300 * 8);
+    red2 = DevBuf(c, 300 * 8);
+    sqd = DevBuf(c, 2 * 8);
+    k_copy(X, XT.as<double>(), (size_t)n * chi, s);
+    k_scale_cols(XT.as<double>(), n, chi, theta, 0, s);
+    k_copy(M, E.as<double>(), (size_t)n * n, s);
+    tc_launch(c, Prec::F64, gemm_desc('N', 'T', n, n, chi, n, n, n, -1.0, 1.0), XT.as<double>(), X,
+              E.as<double>(), s);
+    k_sumsq(E.as<double>(), (size_t)n * n, red1.as<double>(), s);
+    k_sumsq(M, (size_t)n * n, red2.as<double>(), s);
+    k_copy(red1.as<double>(), sqd.as<double>() + 0, 1, s);
+    k_copy(red2.as<double>(), sqd.as<double>() + 1, 1, s);
+    c->read_async(sq, sqd.ptr, 2 * 8);
+  }
   c->read_async(th.data(), theta, (size_t)nv * 8);
   c->sync();
   vals.assign(n, 0.0);
@@ -647,6 +667,13 @@ void ctmrg_step(Ctx* c, const StepDims& d, const StepOpts& o, const double* a64,
   for (int i = 0; i < nv; ++i) vals[i] = std::fabs(th[i]) / s0;
   for (int i = 0; i < nv; ++i)
     if (!(vals[i] == vals[i])) throw Error(TNAD_ERR_NAN, "ctmrgstep: NaN in singular values");
+  if (n > JACOBI_MAX) {
+    state.rest = std::sqrt(sq[0]) / s0;
+    // rounding of the n^2 entries of E and of its sum (~n eps |M|_F), plus the kept residuals R = M Z - Z Theta:
+    // |E - E_exact|_F <= sqrt(2) |R|_F <= sqrt(2 chi) max_j |r_j| (Mirsky)
+    state.rest_err = (double)n * 2.3e-16 * std::sqrt(sq[1]) / s0 + std::sqrt(2.0 * chi) * est.resid;
+    if (!(state.rest == state.rest)) throw Error(TNAD_ERR_NAN, "ctmrgstep: NaN in the discarded spectrum");
+  }
   (void)es;
 }
 

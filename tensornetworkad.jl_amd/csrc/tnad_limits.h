@@ -35,6 +35,21 @@ inline Limit ctmrg_dims_limit(long long D, long long chi) {
   return {};
 }
 
+// ComplexF64 CTMRG: the full spectrum of the complex corner matrix through its real embedding
+// (2 chi D <= JACOBI_MAX)
+inline Limit zctmrg_dims_limit(long long D, long long chi) {
+  if (D < 1 || chi < 1) return {kErrShape, "bond dimensions must be positive"};
+  if (2 * D * chi > JACOBI_MAX) return {kErrUnsupported, "complex CTMRG: chi*D above 96 is not supported"};
+  return {};
+}
+
+// ComplexF64 SVD of an m x p matrix through the embedding of [[0, A], [A^H, 0]] (2 (m + p) <= JACOBI_MAX)
+inline Limit zsvd_limit(long long m, long long p) {
+  if (m < 1 || p < 1) return {kErrShape, "complex svd: sizes must be positive"};
+  if (2 * (m + p) > JACOBI_MAX) return {kErrUnsupported, "complex svd: m + p above 96 is not supported"};
+  return {};
+}
+
 // environment-only work (the :raw initialisation, the expectation value): no eigensolver involved
 inline Limit env_dims_limit(long long D, long long chi) {
   if (D < 1 || chi < 1) return {kErrShape, "bond dimensions must be positive"};

@@ -78,6 +78,9 @@ SIGNATURES = {
     "tnad_trg": (I, [P, P, I, I, I, D, PD, P]),
     "tnad_trg_svd": (I, [P, P, I, I, I, I, I, D, P, P, PI]),
     "tnad_trg_svd_back": (I, [P, P, I, I, I, I, I, D, P, P, I, P]),
+    "tnad_energy_z": (I, [P, P, P, I, I, I, D, I, P, P, PI]),
+    "tnad_svd_z": (I, [P, P, I, I, P, P, P]),
+    "tnad_svd_back_z": (I, [P, P, I, I, P, P, P, P]),
     "tnad_contract": (I, [P, I, C.c_char_p, P, P, C.c_char_p, P, P, C.c_char_p, P, D, D, P]),
     "tnad_time_contract": (I, [P, I, C.c_char_p, P, C.c_char_p, P, C.c_char_p, P, I, PD]),
     "tnad_mlp_create": (I, [P, I, I, I, I, C.c_ulonglong, C.POINTER(P)]),

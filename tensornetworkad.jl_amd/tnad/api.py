@@ -87,6 +87,10 @@ class DeviceArray:
 
 
 def _host(x):
+    """column-major host copy: ComplexF64 for complex input (interleaved re, im as Julia stores it),
+    Float64 otherwise"""
+    if np.iscomplexobj(x):
+        return np.asfortranarray(np.asarray(x, dtype=np.complex128))
     return np.asfortranarray(np.asarray(x, dtype=np.float64))
 
 
@@ -97,8 +101,14 @@ def _in(x):
     return C.c_void_p(a.ctypes.data), a
 
 
-def _out(shape):
-    a = np.zeros(shape, dtype=np.float64, order="F")
+def _real_only(x, what):
+    if np.iscomplexobj(x):
+        raise TypeError(f"{what}: this entry point is Float64 only; complex bulks go through energy / "
+                        "energy_and_grad (tnad_energy_z)")
+
+
+def _out(shape, dtype=np.float64):
+    a = np.zeros(shape, dtype=dtype, order="F")
     return C.c_void_p(a.ctypes.data), a
 
 
@@ -257,8 +267,9 @@ def gets(ipeps: SquareIPEPS) -> int:
 
 
 def indexperm_symmetrize(ipeps: SquareIPEPS, ctx: Context | None = None) -> SquareIPEPS:
-    """src/ipeps.jl:32-39 on the GPU."""
+    """src/ipeps.jl:32-39 on the GPU (Float64 bulks; the complex path symmetrises inside energy)."""
     ctx = ctx or default_context()
+    _real_only(ipeps.bulk, "indexperm_symmetrize")
     pin, _keep = _in(ipeps.bulk)
     pout, out = _out(ipeps.bulk.shape)
     ctx.check(ctx.lib.tnad_indexperm_symmetrize(ctx.h, pin, ipeps.d, ipeps.s, pout))
@@ -268,6 +279,7 @@ def indexperm_symmetrize(ipeps: SquareIPEPS, ctx: Context | None = None) -> Squa
 def double_layer(ipeps: SquareIPEPS, ctx: Context | None = None, want_ap=True):
     """src/variationalipeps.jl:30-34: returns (ap, a) with D = d^2."""
     ctx = ctx or default_context()
+    _real_only(ipeps.bulk, "double_layer")
     d, s = ipeps.d, ipeps.s
     D = d * d
     pin, _keep = _in(ipeps.bulk)
@@ -405,6 +417,10 @@ def energy_and_grad(h, ipeps: SquareIPEPS, chi: int, tol: float, maxit: int, ctx
     ctx = ctx or default_context()
     if not isinstance(ipeps, SquareIPEPS):
         ipeps = SquareIPEPS(ipeps)
+    if np.iscomplexobj(ipeps.bulk) or np.iscomplexobj(h):
+        if opts:
+            raise ValueError("the complex path runs in FP64 with the reference's semantics: no run options")
+        return _energy_and_grad_z(h, ipeps, chi, tol, maxit, ctx, want_grad, return_info)
     o = make_opts(**opts)
     ph, _h = _in(h)
     pb, _b = _in(ipeps.bulk)
@@ -416,6 +432,64 @@ def energy_and_grad(h, ipeps: SquareIPEPS, chi: int, tol: float, maxit: int, ctx
     if return_info:
         return e.value, g, {"iters": it.value, **ctx.stats()}
     return e.value, g
+
+
+def _energy_and_grad_z(h, ipeps, chi, tol, maxit, ctx, want_grad, return_info):
+    """ComplexF64 energy (complex e/n) and the gradient of Re(energy) (tnad_energy_z)."""
+    hz = np.asfortranarray(np.asarray(h, dtype=np.complex128))
+    bz = np.asfortranarray(np.asarray(ipeps.bulk, dtype=np.complex128))
+    pg, g = _out(bz.shape, np.complex128) if want_grad else (None, None)
+    e = np.zeros(2)
+    it = C.c_int(0)
+    ctx.check(ctx.lib.tnad_energy_z(ctx.h, C.c_void_p(hz.ctypes.data), C.c_void_p(bz.ctypes.data), ipeps.d,
+                                    ipeps.s, int(chi), float(tol), int(maxit), C.c_void_p(e.ctypes.data), pg,
+                                    C.byref(it)))
+    ez = complex(e[0], e[1])
+    if return_info:
+        return ez, g, {"iters": it.value, **ctx.stats()}
+    return ez, g
+
+
+def svd(A, ctx: Context | None = None):
+    """LinearAlgebra.svd (thin) of a complex m x p matrix on the GPU (m + p <= 96): (U, S, V) with
+    A = U diag(S) V^H."""
+    ctx = ctx or default_context()
+    A = np.asfortranarray(np.asarray(A, dtype=np.complex128))
+    m, p = A.shape
+    K = min(m, p)
+    pu, U = _out((m, K), np.complex128)
+    pv, V = _out((p, K), np.complex128)
+    ps, S = _out((K,))
+    ctx.check(ctx.lib.tnad_svd_z(ctx.h, C.c_void_p(A.ctypes.data), m, p, pu, ps, pv))
+    return U, S, V
+
+
+def svd_back(A, dU=None, dS=None, dV=None, ctx: Context | None = None):
+    """svd_back (src/trg.jl:72-106) through the GPU's complex SVD of A: cotangents of U, S, V (each
+    optional) -> cotangent of A.  None for all three gives None, as the reference (:73)."""
+    if dU is None and dS is None and dV is None:
+        return None
+    ctx = ctx or default_context()
+    A = np.asfortranarray(np.asarray(A, dtype=np.complex128))
+    m, p = A.shape
+    K = min(m, p)
+    keep = []
+
+    def arg(x, shape, dtype):
+        if x is None:
+            return None
+        x = np.asfortranarray(np.asarray(x, dtype=dtype))
+        if x.shape != shape:
+            raise ValueError(f"svd_back: cotangent of shape {x.shape}, expected {shape}")
+        keep.append(x)
+        return C.c_void_p(x.ctypes.data)
+
+    pdu = arg(dU, (m, K), np.complex128)
+    pds = arg(dS, (K,), np.float64)
+    pdv = arg(dV, (p, K), np.complex128)
+    pa, dA = _out((m, p), np.complex128)
+    ctx.check(ctx.lib.tnad_svd_back_z(ctx.h, C.c_void_p(A.ctypes.data), m, p, pdu, pds, pdv, pa))
+    return dA
 
 
 def magnetisation(model: HamiltonianModel, beta: float, chi: int, rng=None, tol: float = 1e-6, maxit: int = 100,
@@ -586,15 +660,27 @@ def optimiseipeps(ipeps: SquareIPEPS, h, chi: int, tol: float, maxit: int, f_tol
     ctx = ctx or default_context()
     shape = ipeps.bulk.shape
     trace = []
+    # a complex bulk is optimised over (Re, Im) with the objective real(energy) (src/variationalipeps.jl:71);
+    # its gradient dRe(e)/dRe x + i dRe(e)/dIm x splits the same way
+    cplx = np.iscomplexobj(ipeps.bulk)
+    nel = int(np.prod(shape))
+
+    def unpack(x):
+        if cplx:
+            x = x[:nel] + 1j * x[nel:]
+        return np.reshape(x, shape, order="F")
 
     def fg(x):
-        b = np.reshape(x, shape, order="F")
-        e, g = energy_and_grad(h, SquareIPEPS(b), chi, tol, maxit, ctx=ctx, **opts)
+        e, g = energy_and_grad(h, SquareIPEPS(unpack(x)), chi, tol, maxit, ctx=ctx, **opts)
+        e = float(np.real(e))
         if store_trace:
             trace.append(e)
-        return e, np.reshape(g, -1, order="F")
+        g = np.reshape(g, -1, order="F")
+        return e, (np.concatenate([g.real, g.imag]) if cplx else g)
 
     x0 = np.reshape(ipeps.bulk, -1, order="F")
+    if cplx:
+        x0 = np.concatenate([x0.real, x0.imag])
     res = minimize(fg, x0, jac=True, method="L-BFGS-B",
                    options=dict(maxcor=m, ftol=f_tol, gtol=1e-12, maxiter=iterations))
-    return OptimResult(np.reshape(res.x, shape, order="F"), float(res.fun), int(res.nit), int(res.nfev), trace)
+    return OptimResult(unpack(res.x), float(res.fun), int(res.nit), int(res.nfev), trace)

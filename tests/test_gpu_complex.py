@@ -1,0 +1,179 @@
+"""ComplexF64 path on the GPU: the SVD and its adjoint (test/svd.jl), the complex CTMRG energy and gradient
+(test/variationalipeps.jl:136-175), against LAPACK and the oracle.
+
+The reference's complex CTMRG energy depends on the phases LAPACK gives the corner matrix's singular
+vectors (src/variationalipeps.jl:53-54 contracts l with l, not conj(l)).  The oracle and the GPU fix them by
+the same rule, so genuinely complex bulks are compared with the oracle under that rule; the reference's own
+complex tests (a real bulk as complex, a global phase, the complex optimum against the real one) hold under
+any rule."""
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _crandn(rng, *shape):
+    return rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+
+
+@pytest.mark.parametrize("shape", [(6, 3), (3, 6), (3, 3), (2, 2), (40, 17)])
+def test_complex_svd_matches_lapack(ctx, shape):
+    import tnad
+    _say(f"case: complex svd {shape}")
+    rng = np.random.default_rng(11)
+    A = _crandn(rng, *shape)
+    U, S, V = tnad.svd(A, ctx=ctx)
+    ref = np.linalg.svd(A, compute_uv=False)
+    K = min(shape)
+    assert np.max(np.abs(S - ref)) < 1e-13 * ref[0]
+    assert np.max(np.abs((U * S[None, :]) @ V.conj().T - A)) < 1e-13 * ref[0]
+    assert np.max(np.abs(U.conj().T @ U - np.eye(K))) < 1e-13
+    assert np.max(np.abs(V.conj().T @ V - np.eye(K))) < 1e-13
+
+
+def _losses(U, S, V, H1, H2):
+    """cotangents of the four losses of test/svd.jl on the factors (U, S, V)"""
+    out = {}
+    dU = np.zeros_like(U)
+    dU[:, 0] = 2 * H1 @ U[:, 0]
+    out["U"] = (dU, None, None)
+    dV = np.zeros_like(V)
+    dV[:, 0] = 2 * H2 @ V[:, 0]
+    out["V"] = (None, None, dV)
+    dU = np.zeros_like(U)
+    dV = np.zeros_like(V)
+    dU[0, 0] = V[0, 0]
+    dV[0, 0] = U[0, 0]
+    out["UV"] = (dU, None, dV)
+    out["S"] = (None, np.ones_like(S), None)
+    return out
+
+
+@pytest.mark.parametrize("shape", [(6, 3), (3, 6), (3, 3)])
+def test_complex_svd_back_matches_oracle(ctx, shape):
+    # test/svd.jl:21-98: each loss is invariant under the singular vectors' phases, so the pullback computed
+    # on the GPU's factors equals the oracle's on LAPACK's (and the oracle's is pinned by central differences)
+    import tnad
+    from oracle import tnad_oracle as O
+    _say(f"case: complex svd_back {shape}")
+    rng = np.random.default_rng(2)
+    m, p = shape
+    A = _crandn(rng, m, p)
+    H1 = _crandn(rng, m, m)
+    H1 = H1 + H1.conj().T
+    H2 = _crandn(rng, p, p)
+    H2 = H2 + H2.conj().T
+    Ug, Sg, Vg = tnad.svd(A, ctx=ctx)
+    Ul, Sl, Vhl = np.linalg.svd(A, full_matrices=False)
+    Vl = Vhl.conj().T
+    lg, ll = _losses(Ug, Sg, Vg, H1, H2), _losses(Ul, Sl, Vl, H1, H2)
+    for which in ("U", "V", "UV", "S"):
+        g_gpu = tnad.svd_back(A, *lg[which], ctx=ctx)
+        g_ref = O.svd_back(Ul, Sl, Vl, *ll[which])
+        err = np.max(np.abs(g_gpu - g_ref)) / max(1.0, np.max(np.abs(g_ref)))
+        _say(f"  {which}: {err:.2e}")
+        assert err < 1e-10, which
+
+
+def test_complex_svd_back_imaginary_diagonal_term(ctx):
+    # test/svd.jl:69-86: A = [-1+1im 2+1im; 1-2im 3+0.8im], loss real(conj(U[1,1]) V[1,1]); entry (2,1) of the
+    # gradient against central differences in the real and imaginary directions (the i Im diag(U'dU) Sinv
+    # term of src/trg.jl:85 carries it)
+    import tnad
+    _say("case: complex svd_back imaginary diagonal term")
+    A = np.array([[-1 + 1j, 2 + 1j], [1 - 2j, 3 + 0.8j]])
+
+    def loss(X):
+        U, S, Vh = np.linalg.svd(X)
+        return np.real(np.conj(U[0, 0]) * Vh.conj().T[0, 0])
+
+    U, S, V = tnad.svd(A, ctx=ctx)
+    dU = np.zeros_like(U)
+    dV = np.zeros_like(V)
+    dU[0, 0] = V[0, 0]
+    dV[0, 0] = U[0, 0]
+    g = tnad.svd_back(A, dU, None, dV, ctx=ctx)
+    da = np.zeros((2, 2), complex)
+    da[1, 0] = 1.0
+    h = 1e-4
+    nd = (loss(A + h * da) - loss(A - h * da)) / (2 * h) + 1j * (loss(A + 1j * h * da) - loss(A - 1j * h * da)) / (2 * h)
+    _say(f"  grad[2,1] {g[1, 0]:.10f}  central differences {nd:.10f}")
+    assert abs(g[1, 0] - nd) < 1e-6
+
+
+def test_complex_energy_real_copy_and_global_phase(ctx):
+    # test/variationalipeps.jl:136-155: energy and gradient of a real bulk, of the same bulk as ComplexF64,
+    # and of the bulk times a global phase
+    import tnad
+    from oracle import tnad_oracle as O
+    _say("case: complex energy, real copy and global phase")
+    h = O.hamiltonian_heisenberg()
+    b = O.symmetrized_random_bulk(2, 2, 2, "normal")
+    e_r, g_r, i_r = tnad.energy_and_grad(h, b, 4, 1e-12, 100, ctx=ctx, return_info=True)
+    e_c, g_c, i_c = tnad.energy_and_grad(h, b + 0j, 4, 1e-12, 100, ctx=ctx, return_info=True)
+    ph = np.exp(1j * 2.1)
+    e_p, g_p, i_p = tnad.energy_and_grad(h, b * ph, 4, 1e-12, 100, ctx=ctx, return_info=True)
+    e_o, g_o, it_o = O.energy_and_grad(h, b, 4, 1e-12, 100)
+    _say(f"  real {e_r:.15f} complex {e_c:.15f} phase {e_p:.15f} oracle {e_o:.15f}; iters "
+         f"{i_r['iters']} {i_c['iters']} {i_p['iters']} (oracle {it_o})")
+    assert i_r["iters"] == i_c["iters"] == i_p["iters"] == it_o
+    assert abs(e_c.imag) < 1e-14 and abs(e_p.imag) < 1e-14
+    assert abs(e_c.real - e_r) < 1e-12 and abs(e_p.real - e_r) < 1e-12 and abs(e_r - e_o) < 1e-12
+    assert np.max(np.abs(g_c - g_r)) < 1e-10 * np.max(np.abs(g_r))
+    assert np.max(np.abs(g_p - ph * g_r)) < 1e-10 * np.max(np.abs(g_r))
+
+
+@pytest.mark.parametrize("chi,maxit", [(4, 3), (6, 8)])
+def test_complex_energy_matches_oracle(ctx, chi, maxit):
+    # a genuinely complex bulk: the GPU against the oracle with the same phase rule (energy e/n and the
+    # gradient of Re(e/n), tol = 0: maxit + 1 steps)
+    import tnad
+    from oracle import tnad_oracle as O
+    _say(f"case: complex energy vs oracle chi={chi} maxit={maxit}")
+    rng = np.random.default_rng(7)
+    h = O.hamiltonian_heisenberg()
+    b = O.reshape_f(rng.standard_normal(32) + 1j * rng.standard_normal(32), (2, 2, 2, 2, 2))
+    e, g, info = tnad.energy_and_grad(h, b, chi, 0.0, maxit, ctx=ctx, return_info=True)
+    e_o, g_o, it_o = O.energy_and_grad(h, b, chi, 0.0, maxit)
+    rel = np.linalg.norm(g - g_o) / np.linalg.norm(g_o)
+    _say(f"  e {e:.15f} oracle {e_o:.15f} |dE| {abs(e - e_o):.2e} grad rel {rel:.2e} iters {info['iters']}")
+    assert info["iters"] == it_o == maxit + 1
+    assert abs(e - e_o) < 1e-11
+    assert rel < 1e-8
+
+
+def test_complex_optimiseipeps_reaches_the_real_minimum(ctx):
+    # test/variationalipeps.jl:157-175: L-BFGS from a real and from a complex random bulk reach the same
+    # minimum within 1e-3 (here chi = 5, as the real optimisation test, instead of 20)
+    import tnad
+    from oracle import tnad_oracle as O
+    _say("case: complex optimiseipeps chi=5")
+    h = O.hamiltonian_heisenberg()
+    rng = np.random.default_rng(2)
+    b_r = tnad.indexperm_symmetrize(tnad.SquareIPEPS(O.reshape_f(rng.standard_normal(32), (2, 2, 2, 2, 2))),
+                                    ctx=ctx)
+    bz = O.reshape_f(rng.standard_normal(32) + 1j * rng.standard_normal(32), (2, 2, 2, 2, 2))
+    b_c = tnad.SquareIPEPS(O.indexperm_symmetrize(bz))
+    r1 = tnad.optimiseipeps(b_r, h, 5, 1e-12, 100, ctx=ctx)
+    r2 = tnad.optimiseipeps(b_c, h, 5, 1e-12, 100, ctx=ctx)
+    _say(f"  real minimum {r1.minimum:.8f} ({r1.iterations} it), complex {r2.minimum:.8f} ({r2.iterations} it)")
+    assert abs(r1.minimum - r2.minimum) < 1e-3
+
+
+def test_complex_limits_checked_before_launch(ctx):
+    import tnad
+    from tnad import TnadError
+    _say("case: complex limits")
+    with pytest.raises(TnadError) as ei:
+        tnad.svd(np.ones((60, 40), complex), ctx=ctx)
+    assert ei.value.code == -6
+    with pytest.raises(TnadError) as ei:
+        tnad.energy_and_grad(np.zeros((2, 2, 2, 2)), np.ones((2, 2, 2, 2, 2), complex), 25, 0.0, 1, ctx=ctx)
+    assert ei.value.code == -6

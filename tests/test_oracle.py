@@ -337,3 +337,164 @@ def test_num_grad_convention():
     # src/autodiff.jl:39-42 doctest
     assert O.num_grad(lambda x: x * x, 3.0) == pytest.approx(6.0)
     assert np.allclose(O.num_grad(np.trace, np.random.default_rng(0).random((2, 2))), np.eye(2))
+
+
+# --- complex arithmetic (src/trg.jl:85 diagonal term; test/svd.jl, test/trg.jl:22-28,
+# test/variationalipeps.jl:136-155) -----------------------------------------------------------------
+def _cfd(f, x, idx, h=1e-6):
+    """central differences of the real function f along Re and Im of x[idx]: df/dRe + i df/dIm"""
+    out = []
+    for i in idx:
+        parts = []
+        for dz in (h, 1j * h):
+            xp = x.copy()
+            xm = x.copy()
+            xp[i] += dz / 2
+            xm[i] -= dz / 2
+            parts.append((f(xp) - f(xm)) / h)
+        out.append(parts[0] + 1j * parts[1])
+    return np.array(out)
+
+
+def _svd_losses(M, H1, H2):
+    """the four losses of test/svd.jl (U[:,1], V[:,1], U[1,1] with V[1,1], sum S) and their cotangents"""
+    U, s, Vh = np.linalg.svd(M, full_matrices=False)
+    V = Vh.conj().T
+    out = {}
+    psi = U[:, 0]
+    dU = np.zeros_like(U)
+    dU[:, 0] = 2 * H1 @ psi
+    out["U"] = (np.real(psi.conj() @ H1 @ psi), (dU, None, None))
+    psi = V[:, 0]
+    dV = np.zeros_like(V)
+    dV[:, 0] = 2 * H2 @ psi
+    out["V"] = (np.real(psi.conj() @ H2 @ psi), (None, None, dV))
+    # real(conj(U[1,1]) V[1,1])
+    dU = np.zeros_like(U)
+    dV = np.zeros_like(V)
+    dU[0, 0] = V[0, 0]
+    dV[0, 0] = U[0, 0]
+    out["UV"] = (np.real(np.conj(U[0, 0]) * V[0, 0]), (dU, None, dV))
+    out["S"] = (np.sum(s), (None, np.ones_like(s), None))
+    return out, (U, s, V)
+
+
+@pytest.mark.parametrize("shape", [(6, 3), (3, 6), (3, 3)])
+@pytest.mark.parametrize("which", ["U", "V", "UV", "S"])
+def test_complex_svd_back_matches_fd(shape, which):
+    # test/svd.jl:21-98: every loss is invariant under the singular vectors' phases, so its central
+    # differences through LAPACK's SVD pin svd_back, including the i Im diag(U'dU) Sinv term (src/trg.jl:85)
+    rng = np.random.default_rng(2)
+    m, p = shape
+    M = rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+    H1 = rng.standard_normal((m, m)) + 1j * rng.standard_normal((m, m))
+    H1 = H1 + H1.conj().T
+    H2 = rng.standard_normal((p, p)) + 1j * rng.standard_normal((p, p))
+    H2 = H2 + H2.conj().T
+    losses, (U, s, V) = _svd_losses(M, H1, H2)
+    g = O.svd_back(U, s, V, *losses[which][1])
+    f = lambda X: _svd_losses(X, H1, H2)[0][which][0]
+    idx = list(np.ndindex(shape))
+    fd = _cfd(f, M, idx)
+    gsel = np.array([g[i] for i in idx])
+    assert np.max(np.abs(gsel - fd)) < 1e-6 * max(1.0, np.max(np.abs(fd)))
+
+
+def test_complex_svd_back_imaginary_diagonal_case():
+    # test/svd.jl:69-86: A = [-1+1im 2+1im; 1-2im 3+0.8im], loss real(conj(U[1,1]) V[1,1]), entry (2,1)
+    A = np.array([[-1 + 1j, 2 + 1j], [1 - 2j, 3 + 0.8j]])
+    losses, (U, s, V) = _svd_losses(A, np.eye(2), np.eye(2))
+    g = O.svd_back(U, s, V, *losses["UV"][1])
+    f = lambda X: _svd_losses(X, np.eye(2), np.eye(2))[0]["UV"][0]
+    fd = _cfd(f, A, [(1, 0)], h=1e-4)[0]
+    assert abs(g[1, 0] - fd) < 1e-6
+    # without the diagonal term the pullback misses (the term the reference adds for complex input)
+    U_, s_, V_ = U, s, V
+    UdU = U_.conj().T @ losses["UV"][1][0]
+    assert np.max(np.abs(np.imag(np.diag(UdU)))) > 1e-3
+
+
+def test_complex_ctmrgstep_adjoint_matches_fd_on_closed_network():
+    # complex CTMRG steps are covariant under the environment's phase gauge, so a closed network of the
+    # outputs (every bond pairing a row with a column index) is gauge invariant and its central differences
+    # pin the complex reverse sweep (svd_back of the Hermitian corner matrix and the einsum adjoints)
+    rng = np.random.default_rng(4)
+    cr = lambda *s: rng.standard_normal(s) + 1j * rng.standard_normal(s)
+    chi, D = 4, 4
+    C = cr(chi, chi)
+    C = C + C.conj().T
+    T = cr(chi, D, chi)
+    T = T + O.ein("ijk->kji", T.conj())
+    a = cr(D, D, D, D)
+
+    def run(a_, C_, T_, keep=False):
+        caches = []
+        for _ in range(2):
+            out = O.ctmrgstep(a_, C_, T_, keep=keep)
+            C_, T_ = out[0], out[1]
+            if keep:
+                caches.append(out[3])
+        return C_, T_, caches
+
+    def L(a_, C_, T_):
+        C2, T2, _ = run(a_, C_, T_)
+        return np.real(O.ein("ij,ji->", C2, C2) + O.ein("ij,jak,kl,lai->", C2, T2, C2, T2))
+
+    C2, T2, caches = run(a, C, T, keep=True)
+    Cb = 2 * C2.T.conj() + (O.ein("jak,kl,lai->ij", T2, C2, T2) + O.ein("ij,jak,lai->kl", C2, T2, T2)).conj()
+    Tb = (O.ein("ij,kl,lai->jak", C2, C2, T2) + O.ein("ij,jak,kl->lai", C2, T2, C2)).conj()
+    ab = 0
+    for cache in reversed(caches):
+        Cb, Tb, x = O.ctmrgstep_back(a, cache, Cb, Tb)
+        ab = ab + x
+    for x, xb, f in ((C, Cb, lambda v: L(a, v, T)), (T, Tb, lambda v: L(a, C, v)), (a, ab, lambda v: L(v, C, T))):
+        idx = [tuple(int(rng.integers(0, n)) for n in x.shape) for _ in range(4)]
+        fd = _cfd(f, x, idx)
+        gsel = np.array([xb[i] for i in idx])
+        assert np.max(np.abs(gsel - fd)) < 2e-6 * np.max(np.abs(xb))
+
+
+def test_complex_energy_real_copy_and_global_phase():
+    # test/variationalipeps.jl:136-155: a real bulk and its complex copy give the same energy and
+    # gradient; a global phase leaves the double layer, hence the energy, unchanged
+    h = O.hamiltonian_heisenberg()
+    b = O.symmetrized_random_bulk(2, 2, 2, "normal")
+    e1, g1, it1 = O.energy_and_grad(h, b, 4, 1e-12, 100)
+    e2, g2, it2 = O.energy_and_grad(h, b + 0j, 4, 1e-12, 100)
+    ph = np.exp(1j * 2.1)
+    e3, g3, it3 = O.energy_and_grad(h, b * ph, 4, 1e-12, 100)
+    assert it1 == it2 == it3
+    assert abs(e1 - e2) < 1e-13 and abs(e1 - e3) < 1e-13
+    assert np.max(np.abs(g1 - g2)) < 1e-12
+    # d/d(phi x) = phi d/dx (the energy is phase invariant)
+    assert np.max(np.abs(g3 - ph * g1)) < 1e-12
+
+
+def test_complex_phase_rule():
+    # real columns stay real (a sign), complex ones end with sum_r w_r U[r, j] real positive
+    rng = np.random.default_rng(1)
+    X = rng.standard_normal((7, 3))
+    Y, ph = O.fix_phases(X + 0j)
+    assert np.all(np.abs(np.imag(Y)) == 0) and np.all(np.abs(np.abs(ph) - 1) < 1e-15)
+    Z = rng.standard_normal((7, 3)) + 1j * rng.standard_normal((7, 3))
+    Zf, _ = O.fix_phases(Z * np.exp(1j * 0.4))
+    q = O.phase_weights(7) @ Zf
+    assert np.all(np.abs(np.imag(q)) < 1e-14) and np.all(np.real(q) > 0)
+    Zg, _ = O.fix_phases(Z)
+    assert np.max(np.abs(Zf - Zg)) < 1e-14   # the rule removes any prior phase
+
+
+def test_complex_trg_matches_real_and_fd():
+    # test/trg.jl:22-28: the complex copy of the Ising tensor gives the real lnZ and d lnZ / d beta;
+    # a generic complex tensor's directional derivative of Re lnZ matches central differences
+    a, da = O.model_tensor_ising(0.4), O.dmodel_tensor_ising(0.4)
+    l1, d1 = O.trg(a, 12, 3, with_grad_seed=da)
+    l2, d2 = O.trg(a + 0j, 12, 3, with_grad_seed=da + 0j)
+    assert abs(l1 - l2) < 1e-14 and abs(d1 - d2) < 1e-12
+    rng = np.random.default_rng(1)
+    t = rng.standard_normal((3, 3, 3, 3)) + 1j * rng.standard_normal((3, 3, 3, 3)) + 3.0
+    dirn = rng.standard_normal(t.shape) + 1j * rng.standard_normal(t.shape)
+    _, d = O.trg(t, 6, 3, with_grad_seed=dirn)
+    h = 1e-6
+    fd = (np.real(O.trg(t + h / 2 * dirn, 6, 3)) - np.real(O.trg(t - h / 2 * dirn, 6, 3))) / h
+    assert abs(d - fd) < 1e-7 * abs(d)
