@@ -164,6 +164,11 @@ int tnad_energy(tnad_ctx* ctx, const double* h, const double* bulk, int d, int s
 int tnad_energy_z(tnad_ctx* ctx, const double* h, const double* bulk, int d, int s, int chi, double tol, int maxit,
                   double* e, double* grad, int* iters);
 
+/* trg (src/trg.jl:13-30) of a complex q x q x q x q tensor: lnz receives ln Z per site (2 doubles, re and im:
+ * the log of the final trace is complex) and, if abar != NULL, the gradient of Re(lnZ) with respect to a
+ * (q^4 ComplexF64, Zygote's convention), as test/trg.jl:22-28 differentiates real(trg(...)).  chi <= 32. */
+int tnad_trg_z(tnad_ctx* ctx, const double* a, int q, int chi, int niter, double tol, double* lnz, double* abar);
+
 /* svd (LinearAlgebra.svd, thin) of a complex m x p matrix A: U (m x K), S (K, real, descending),
  * V (p x K) with A = U diag(S) V^H, K = min(m, p). */
 int tnad_svd_z(tnad_ctx* ctx, const double* A, int m, int p, double* U, double* S, double* V);

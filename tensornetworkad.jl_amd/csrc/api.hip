@@ -412,6 +412,22 @@ int tnad_energy_z(tnad_ctx* ctx, const double* h, const double* bulk, int d, int
   });
 }
 
+int tnad_trg_z(tnad_ctx* ctx, const double* a, int q, int chi, int niter, double tol, double* lnz, double* abar) {
+  return guarded(ctx, [&] {
+    enforce(ztrg_limit(q, chi, niter));
+    if (!a) throw Error(TNAD_ERR_ARG, "null input");
+    Ctx* c = &ctx->c;
+    const size_t n = (size_t)q * q * q * q;
+    DevBuf ab(c, 2 * n * 8), gb;
+    z_from_interleaved(c, a, n, ab.as<double>());
+    if (abar) gb = DevBuf(c, 2 * n * 8);
+    double l2[2] = {0.0, 0.0};
+    ztrg_run(c, ab.as<double>(), q, chi, niter, tol, l2, abar ? gb.as<double>() : nullptr);
+    if (abar) z_to_interleaved(c, gb.as<double>(), n, abar);
+    if (lnz) { lnz[0] = l2[0]; lnz[1] = l2[1]; }
+  });
+}
+
 int tnad_svd_z(tnad_ctx* ctx, const double* A, int m, int p, double* U, double* S, double* V) {
   return guarded(ctx, [&] {
     enforce(zsvd_limit(m, p));

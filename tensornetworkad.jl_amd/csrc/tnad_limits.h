@@ -50,6 +50,15 @@ inline Limit zsvd_limit(long long m, long long p) {
   return {};
 }
 
+// ComplexF64 TRG: splits of up to chi^2 x chi^2 through the real embedding of the augmented matrix; the
+// eigensolver block (4 chi + 4 columns of the embedding, plus the guard) stays within JACOBI_MAX
+inline Limit ztrg_limit(long long q, long long chi, long long niter) {
+  if (q < 1 || chi < 1 || niter < 0) return {kErrArg, "complex trg: bad sizes"};
+  if (chi > 32) return {kErrUnsupported, "complex trg: chi above 32 is not supported"};
+  if (q * q > 1024) return {kErrUnsupported, "complex trg: input bond above 32 is not supported"};
+  return {};
+}
+
 // environment-only work (the :raw initialisation, the expectation value): no eigensolver involved
 inline Limit env_dims_limit(long long D, long long chi) {
   if (D < 1 || chi < 1) return {kErrShape, "bond dimensions must be positive"};

@@ -32,6 +32,8 @@ void z_edge_sym(Ctx* c, const double* T1, int chi, int D, double* T2);
 // all n eigenpairs of the Hermitian n x n H (n <= 96), |lambda| descending, phases by the rule of zc.hip:
 // U planar n x n, lam (real) n
 void z_heig(Ctx* c, const double* H, int n, double* U, double* lam);
+// res (K x K, planar) of svd_back (src/trg.jl:76-96): UdU = U^H dU, VdV = V^H dV (K x K, nullable), dS (K, nullable)
+void z_svdback_res(Ctx* c, const double* UdU, const double* VdV, const double* dS, const double* sv, int K, double* R);
 // X[:, j] *= f[j] (real f, device), planar n x k
 void z_scale_cols(Ctx* c, double* X, int n, int k, const double* f);
 
@@ -45,5 +47,10 @@ void zsvd_small(Ctx* c, const double* A, int m, int p, double* U, double* S, dou
 // its svd_back (src/trg.jl:72-106): dU, dS, dV nullable; dA m x p
 void zsvd_back_small(Ctx* c, const double* A, int m, int p, const double* dU, const double* dS, const double* dV,
                      double* dA);
+
+// trg (src/trg.jl:13-30) of a complex q^4 tensor (planar, device): lnz_out (2 doubles) = lnZ per site; abar_out
+// (nullable, planar q^4) the gradient of Re(lnZ)
+void ztrg_run(Ctx* c, const double* a_planar, int q, int chi, int niter, double tol, double* lnz_out,
+              double* abar_out);
 
 }  // namespace tnad

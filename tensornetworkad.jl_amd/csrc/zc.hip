@@ -652,6 +652,12 @@ void z_heig(Ctx* c, const double* H, int n, double* U, double* lam) {
   TNAD_HIP_TRY(hipGetLastError());
 }
 
+void z_svdback_res(Ctx* c, const double* UdU, const double* VdV, const double* dS, const double* sv, int K, double* R) {
+  hipLaunchKernelGGL(zsvdback_full_res_kernel, dim3(nb((size_t)K * K)), dim3(NT), 0, c->stream, UdU, VdV, dS, sv, K,
+                     1e-40, R);
+  TNAD_HIP_TRY(hipGetLastError());
+}
+
 void z_scale_cols(Ctx* c, double* X, int n, int k, const double* f) {
   hipLaunchKernelGGL(zscale_cols_kernel, dim3(nb((size_t)n * k)), dim3(NT), 0, c->stream, X, n, k, f);
   TNAD_HIP_TRY(hipGetLastError());

@@ -80,9 +80,38 @@ function ChainRulesCore.rrule(::typeof(gpu_energy), h, ipeps::IPEPS; χ::Int, to
     return e, pullback
 end
 
+# ComplexF64 bulk (tnad_energy_z): Julia's ComplexF64 memory is the interleaved (re, im) doubles the entry
+# takes; the value is complex (e/n) and the gradient is that of real(energy) (src/variationalipeps.jl:71)
+function energy_and_grad(h::AbstractArray{<:Number,4}, bulk::Array{ComplexF64,5}; χ::Int, tol::Real, maxit::Int,
+                         want_grad::Bool = true)
+    d, s = size(bulk, 1), size(bulk, 5)
+    hz = Array{ComplexF64}(h)
+    e = zeros(Cdouble, 2)
+    it = Ref{Cint}(0)
+    g = want_grad ? similar(bulk) : nothing
+    GC.@preserve hz bulk g e begin
+        check(ccall((:tnad_energy_z, libtnad), Cint,
+                    (Ptr{Cvoid}, Ptr{ComplexF64}, Ptr{ComplexF64}, Cint, Cint, Cint, Cdouble, Cint, Ptr{Cdouble},
+                     Ptr{ComplexF64}, Ptr{Cint}),
+                    context(), hz, bulk, d, s, χ, tol, maxit, e, want_grad ? g : C_NULL, it))
+    end
+    complex(e[1], e[2]), g, it[]
+end
+
 # ---------------------------------------------------------------------------
 # trg
 # ---------------------------------------------------------------------------
+function trg_and_grad(a::Array{ComplexF64,4}, χ::Int, niter::Int; tol::Float64 = 1e-16, want_grad::Bool = true)
+    lnz = zeros(Cdouble, 2)
+    g = want_grad ? similar(a) : nothing
+    GC.@preserve a g lnz begin
+        check(ccall((:tnad_trg_z, libtnad), Cint,
+                    (Ptr{Cvoid}, Ptr{ComplexF64}, Cint, Cint, Cint, Cdouble, Ptr{Cdouble}, Ptr{ComplexF64}),
+                    context(), a, size(a, 1), χ, niter, tol, lnz, want_grad ? g : C_NULL))
+    end
+    complex(lnz[1], lnz[2]), g
+end
+
 function trg_and_grad(a::Array{Float64,4}, χ::Int, niter::Int; tol::Float64 = 1e-16, want_grad::Bool = true)
     lnz = Ref{Cdouble}(0.0)
     g = want_grad ? similar(a) : nothing

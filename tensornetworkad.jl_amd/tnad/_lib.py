@@ -79,6 +79,7 @@ SIGNATURES = {
     "tnad_trg_svd": (I, [P, P, I, I, I, I, I, D, P, P, PI]),
     "tnad_trg_svd_back": (I, [P, P, I, I, I, I, I, D, P, P, I, P]),
     "tnad_energy_z": (I, [P, P, P, I, I, I, D, I, P, P, PI]),
+    "tnad_trg_z": (I, [P, P, I, I, I, D, P, P]),
     "tnad_svd_z": (I, [P, P, I, I, P, P, P]),
     "tnad_svd_back_z": (I, [P, P, I, I, P, P, P, P]),
     "tnad_contract": (I, [P, I, C.c_char_p, P, P, C.c_char_p, P, P, C.c_char_p, P, D, D, P]),

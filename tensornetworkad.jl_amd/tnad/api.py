@@ -566,9 +566,17 @@ def time_contract(spec: str, extents: dict, precision="fp64", reps: int = 20,
 # TRG (src/trg.jl)
 # ---------------------------------------------------------------------------
 def trg_and_grad(a, chi: int, niter: int, tol: float = 1e-16, ctx: Context | None = None, want_grad=True):
-    """trg (src/trg.jl:13-30) and d lnZ / d a in one library call."""
+    """trg (src/trg.jl:13-30) and d lnZ / d a in one library call.  A complex tensor gives a complex lnZ and
+    the gradient of Re(lnZ) (tnad_trg_z)."""
     ctx = ctx or default_context()
     a = _host(a)
+    if np.iscomplexobj(a):
+        q = a.shape[0]
+        pg, g = _out(a.shape, np.complex128) if want_grad else (None, None)
+        lz = np.zeros(2)
+        ctx.check(ctx.lib.tnad_trg_z(ctx.h, C.c_void_p(a.ctypes.data), q, int(chi), int(niter), float(tol),
+                                     C.c_void_p(lz.ctypes.data), pg))
+        return complex(lz[0], lz[1]), g
     q = a.shape[0]
     pa, _a = _in(a)
     pg, g = _out(a.shape) if want_grad else (None, None)
