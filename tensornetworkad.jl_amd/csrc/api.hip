@@ -6,6 +6,7 @@
 #include <cstring>
 #include <limits>
 #include <memory>
+#include <mutex>
 
 #include "api_internal.h"
 #include "ctmrg.h"
@@ -287,6 +288,56 @@ void run_ctmrg(tnad_ctx* ctx, tnad_tape& t, const double* C0, const double* T0, 
 
 }  // namespace
 
+// ---------------------------------------------------------------------------
+// Device scratch of the entry point that allocates nothing (tnad_symeig_noalloc): one file-scope
+// array per GPU, sized for the full-spectrum Jacobi at n = 192 with the sweep limit tnad_symeig
+// uses (rotation record 60 x 191 x 96 x 2 doubles, unsorted eigenvectors 192 x 192, diagonal,
+// round counter: 2,237,384 doubles, 17.9 MB).  Every call on a GPU shares that GPU's copy, and every
+// word a call reads there is written earlier in the same call.  Host code never passes `scratch`
+// itself to a kernel: its address on each device comes from hipGetSymbolAddress at tnad_create.
+// ---------------------------------------------------------------------------
+namespace {
+constexpr int kNoAllocSweeps = 60;   // the sweep limit of tnad_symeig's full-spectrum path (topk_eig)
+constexpr size_t kScratchDoubles = (size_t)kNoAllocSweeps * 191 * 96 * 2 + 192 * 192 + 192 + 8;
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch[kMaxDevices] = {};   // device address of `scratch` on each device
+}  // namespace
+
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
+namespace {
+// with `device` current: look up this device's copy of `scratch` once
+hipError_t scratch_setup(int device) {
+  if (device < 0 || device >= kMaxDevices) return hipSuccess;   // tnad_symeig_noalloc reports it
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch[device]) return hipSuccess;
+  double* p = nullptr;
+  hipError_t e = hipGetSymbolAddress((void**)&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess) return e;
+  g_scratch[device] = p;
+  return hipSuccess;
+}
+double* scratch_for(int device) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (device < 0 || device >= kMaxDevices || !g_scratch[device])
+    throw Error(TNAD_ERR_UNSUPPORTED, "tnad_symeig_noalloc: no scratch on device " + std::to_string(device));
+  return g_scratch[device];
+}
+// throw unless p is device memory of the context's GPU (no allocation: an attribute query)
+void require_device(const Ctx* c, const void* p, const char* what) {
+  if (!p) throw Error(TNAD_ERR_ARG, std::string("tnad_symeig_noalloc: null ") + what);
+  hipPointerAttribute_t at;
+  if (hipPointerGetAttributes(&at, p) != hipSuccess) {
+    (void)hipGetLastError();
+    throw Error(TNAD_ERR_ARG, std::string("tnad_symeig_noalloc: ") + what + " is not device memory");
+  }
+  if (at.type != hipMemoryTypeDevice || at.device != c->device)
+    throw Error(TNAD_ERR_ARG, std::string("tnad_symeig_noalloc: ") + what +
+                                  " must be device memory on the context's GPU");
+}
+}  // namespace
+
 extern "C" {
 
 int tnad_version(void) { return TNAD_ABI_VERSION; }
@@ -313,6 +364,11 @@ int tnad_create(int device, tnad_ctx** out) {
   tnad_ctx* ctx = new tnad_ctx();
   ctx->c.device = device;
   if (hipStreamCreateWithFlags(&ctx->c.stream, hipStreamNonBlocking) != hipSuccess) {
+    delete ctx;
+    return TNAD_ERR_HIP;
+  }
+  if (scratch_setup(device) != hipSuccess) {
+    (void)hipGetLastError();
     delete ctx;
     return TNAD_ERR_HIP;
   }
@@ -385,6 +441,22 @@ int tnad_symeig(tnad_ctx* ctx, const double* A, int n, int want, int k, double* 
     if (evals) copy_out(c, evals, th.ptr, (size_t)kr * 8);
     if (evecs) copy_out(c, evecs, X.ptr, (size_t)n * kr * 8);
     if (k_out) *k_out = kr;
+  });
+}
+
+int tnad_symeig_noalloc(tnad_ctx* ctx, const double* A, int n, double* evals, double* evecs) {
+  return guarded(ctx, [&] {
+    if (n < 1 || n > JACOBI_MAX)
+      throw Error(TNAD_ERR_UNSUPPORTED, "tnad_symeig_noalloc: n must be 1..192, got " + std::to_string(n));
+    Ctx* c = &ctx->c;
+    require_device(c, A, "A");
+    require_device(c, evals, "evals");
+    require_device(c, evecs, "evecs");
+    double* ws = scratch_for(c->device);
+    if (jacobi_work_doubles(n, kNoAllocSweeps) > kScratchDoubles)
+      throw Error(TNAD_ERR_UNSUPPORTED, "tnad_symeig_noalloc: work exceeds the scratch array");
+    // the kernels of tnad_symeig's full-spectrum path, on the scratch array instead of a pool block
+    jacobi_eig_ws(A, n, n, evals, evecs, ws, kNoAllocSweeps, nullptr, c->stream);
   });
 }
 

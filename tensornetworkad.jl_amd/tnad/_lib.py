@@ -68,6 +68,7 @@ SIGNATURES = {
     "tnad_double_layer": (I, [P, P, I, I, P, P]),
     "tnad_init_raw": (I, [P, P, I, I, P, P]),
     "tnad_symeig": (I, [P, P, I, I, I, P, P, PI]),
+    "tnad_symeig_noalloc": (I, [P, P, I, P, P]),
     "tnad_svd_back_sym": (I, [P, P, I, I, I, P, P]),
     "tnad_ctmrgstep": (I, [P, P, I, I, P, P, C.POINTER(Opts), P, P, P]),
     "tnad_ctmrg": (I, [P, P, I, I, P, P, D, I, C.POINTER(Opts), P, P, P, PI, C.POINTER(P)]),
