@@ -243,6 +243,14 @@ def main():
         "speedup_vs_plain_fp64_per_step": round(results["plain_fp64"]["ms_step"] / results["mixed"]["ms_step"], 3),
         "accuracy": acc,
         "stats": {k: v["stats"] for k, v in results.items()},
+        # device time of one evaluation by phase (HIP events on the library's stream): the CTMRG steps'
+        # contractions and epilogues, their eigensolver, the reverse sweep, and the rest (symmetrisation,
+        # double layer, expectation value, transfers)
+        "phases_ms_per_step": {k: {"forward_contractions": round(v["stats"]["fwd_ms"] - v["stats"]["eig_ms"], 3),
+                                   "eigensolver": round(v["stats"]["eig_ms"], 3),
+                                   "reverse_sweep": round(v["stats"]["bwd_ms"], 3),
+                                   "other": round(v["ms_step"] - v["stats"]["fwd_ms"] - v["stats"]["bwd_ms"], 3)}
+                               for k, v in results.items()},
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

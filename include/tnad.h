@@ -77,6 +77,9 @@ typedef struct tnad_stats {
   int eig_k;            /* eigensolver block size (n when the full spectrum is computed) */
   double eig_resid_max; /* largest relative Ritz residual over the steps */
   int bwd_steps;        /* steps covered by the last reverse sweep */
+  double fwd_ms;        /* device time of the CTMRG steps (HIP events on the context's stream) */
+  double eig_ms;        /* of which in the steps' eigensolver */
+  double bwd_ms;        /* device time of the reverse sweep (expectation value and steps), 0 without one */
 } tnad_stats;
 
 int tnad_version(void);

@@ -69,8 +69,23 @@ void Ctx::trim() {
   free_blocks.clear();
   cached_bytes = 0;
 }
+hipEvent_t Ctx::mark(int slot) {
+  if (!ev_[slot]) TNAD_HIP_TRY(hipEventCreate(&ev_[slot]));
+  TNAD_HIP_TRY(hipEventRecord(ev_[slot], stream));
+  return ev_[slot];
+}
+float Ctx::elapsed(int a, int b) {
+  float ms = 0.0f;
+  if (ev_[a] && ev_[b] && hipEventElapsedTime(&ms, ev_[a], ev_[b]) != hipSuccess) {
+    (void)hipGetLastError();
+    ms = 0.0f;
+  }
+  return ms;
+}
 Ctx::~Ctx() {
   if (stream) (void)hipStreamSynchronize(stream);
+  for (auto& e : ev_)
+    if (e) (void)hipEventDestroy(e);
   for (auto& kv : free_blocks) (void)hipFree(kv.second);
   if (pinned_) (void)hipHostFree(pinned_);
   if (stream) (void)hipStreamDestroy(stream);
@@ -224,6 +239,7 @@ void run_ctmrg(tnad_ctx* ctx, tnad_tape& t, const double* C0, const double* T0, 
   tnad_stats& st = ctx->stats;
   st = tnad_stats{};
   st.eig_k = k;
+  c->mark(0);
   while (true) {
     // StopFunction (src/fixedpoint.jl:31-41), evaluated on every state including the first
     counter += 1;
@@ -280,7 +296,11 @@ void run_ctmrg(tnad_ctx* ctx, tnad_tape& t, const double* C0, const double* T0, 
     if (so.prec != Prec::F64) st.low_prec_iters += 1;
     st.eig_iters_total += ss.eig_iters;
     st.eig_resid_max = std::max(st.eig_resid_max, ss.eig_resid);
+    st.eig_ms += ss.eig_ms;
   }
+  c->mark(1);
+  c->sync();
+  st.fwd_ms = c->elapsed(0, 1);
   k_copy(Ca.as<double>(), C_out, nC, c->stream);
   k_copy(Ta.as<double>(), T_out, nT, c->stream);
   vals_out = vals;
@@ -721,6 +741,7 @@ int tnad_energy(tnad_ctx* ctx, const double* h, const double* bulk, int d, int s
     if (e) *e = eh;
     if (iters) *iters = it;
     if (grad) {
+      c->mark(2);
       DevBuf apbar(c, nap * 8), Cbar(c, nC * 8), Tbar(c, nT * 8), abar(c, nA * 8), bsbar(c, N * 8), g(c, N * 8);
       k_fill(Cbar.as<double>(), nC, 0.0, st);
       k_fill(Tbar.as<double>(), nT, 0.0, st);
@@ -729,7 +750,10 @@ int tnad_energy(tnad_ctx* ctx, const double* h, const double* bulk, int d, int s
       ctmrg_back_impl(ctx, &t, Cbar.as<double>(), Tbar.as<double>(), abar.as<double>());
       k_double_layer_back(bs.as<double>(), d, s, apbar.as<double>(), abar.as<double>(), bsbar.as<double>(), st);
       k_symmetrize_bulk_back2(bsbar.as<double>(), bx.as<double>(), d, s, g.as<double>(), st);
+      c->mark(3);
       copy_out(c, grad, g.ptr, N * 8);
+      c->sync();
+      ctx->stats.bwd_ms = c->elapsed(2, 3);
     }
   });
 }

@@ -46,6 +46,7 @@ struct StepState {
   // n > 192 only: norm of the singular values beyond the chi kept ones, relative to s[1]
   // (|M - Z Theta Z^T|_F / s[1]), and a bound on its rounding and deflation error
   double rest = 0.0, rest_err = 0.0;
+  double eig_ms = 0.0;   // device time of this step's eigensolver
 };
 
 struct EigStats {

@@ -580,8 +580,10 @@ void ctmrg_step(Ctx* c, const StepDims& d, const StepOpts& o, const double* a64,
   }
   EigStats est;
   const bool warm = X0 != nullptr && n > JACOBI_MAX;
+  c->mark(4);
   int k = topk_eig(c, M, n, chi, kreq, warm ? X0 : nullptr, warm ? o.eig_warm_iters : o.eig_max_iters,
                    p == Prec::F64 ? o.eig_tol : o.eig_tol_low, X, AX, theta, &est);
+  c->mark(5);
   state.k = k;
   if (X_keep) k_copy(X, X_keep, (size_t)n * k, s);
   state.eig_iters = est.iters;
@@ -662,6 +664,7 @@ void ctmrg_step(Ctx* c, const StepDims& d, const StepOpts& o, const double* a64,
   }
   c->read_async(th.data(), theta, (size_t)nv * 8);
   c->sync();
+  state.eig_ms = c->elapsed(4, 5);
   vals.assign(n, 0.0);
   double s0 = std::fabs(th[0]);
   for (int i = 0; i < nv; ++i) vals[i] = std::fabs(th[i]) / s0;

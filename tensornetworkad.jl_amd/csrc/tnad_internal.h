@@ -165,8 +165,13 @@ struct Ctx {
   void read_async(void* dst_host, const void* src_dev, size_t bytes);
   void write_async(void* dst_dev, const void* src_host, size_t bytes);
   void sync();   // stream synchronise, then deliver the queued reads
+  // device-time phase clocks: record() marks the stream; elapsed(a, b) (ms) is valid once the stream
+  // has passed both marks (after a sync)
+  hipEvent_t mark(int slot);
+  float elapsed(int a, int b);
   ~Ctx();
  private:
+  hipEvent_t ev_[8] = {};
   char* pinned_ = nullptr;
   size_t pinned_used_ = 0;
   struct PendingRead { void* dst; size_t off, bytes; };

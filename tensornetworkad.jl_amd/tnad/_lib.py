@@ -47,6 +47,9 @@ class Stats(C.Structure):
         ("eig_k", C.c_int),
         ("eig_resid_max", C.c_double),
         ("bwd_steps", C.c_int),
+        ("fwd_ms", C.c_double),
+        ("eig_ms", C.c_double),
+        ("bwd_ms", C.c_double),
     ]
 
 
