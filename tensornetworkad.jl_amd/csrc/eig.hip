@@ -100,12 +100,17 @@ constexpr int JR_MAX_SWEEPS = 64;
 // Rotation of one slot from its 2x2 diagonal block; returns false (c = 1, s = 0) below the thresholds.
 // tan of the rotation angle (|angle| <= pi/4): tt = sign(d) e / (|d| + hypot(d, e)) with d = abb - aaa,
 // e = 2 aab; then c = 1 / sqrt(1 + tt^2), s = tt c, formed without tt.
+// whether the pair (a, b) with entries aaa, aab, abb rotates: |aab| above the matrix's rounding level and
+// above 1e-15 of the geometric mean of the two diagonal entries
+__device__ __forceinline__ bool jrot_needed(double aaa, double aab, double abb, double abs_floor) {
+  const double rel_tol2 = 1e-30;   // (1e-15)^2
+  return fabs(aab) > abs_floor && aab * aab > rel_tol2 * fabs(aaa * abb);
+}
 __device__ __forceinline__ bool jrot_cs(double aaa, double aab, double abb, double abs_floor, double& c,
                                         double& sn) {
-  const double rel_tol2 = 1e-30;   // (1e-15)^2
   c = 1.0;
   sn = 0.0;
-  if (!(fabs(aab) > abs_floor && aab * aab > rel_tol2 * fabs(aaa * abb))) return false;
+  if (!jrot_needed(aaa, aab, abb, abs_floor)) return false;
   const double d = abb - aaa, e = 2.0 * aab;
   const double den = fabs(d) + sqrt(d * d + e * e);
   const double inv = 1.0 / sqrt(den * den + e * e);
@@ -322,6 +327,27 @@ __global__ __launch_bounds__(JR_NT) void jacobi_rot_kernel(const double* __restr
     }
     // every flag of this sweep was set before the last round's first barrier
     if (nrot_sw[sweep] == 0) { ++sweep; break; }
+    // The next sweep would rotate no pair when every off-diagonal entry now fails the rotation test:
+    // without rotations no entry changes, so each pair meets the same test it fails here.  Stop now;
+    // the diagonal and the recorded rotations are those the rotation-free sweep would leave (it only
+    // clears the off-diagonal entries below the test, which nothing reads).
+    if (sweep + 1 < max_sweeps) {
+      int any = 0;
+#pragma unroll
+      for (int u = 0; u < JB_MAX; ++u) {
+        if (bij[u] < 0) continue;
+        const int i = bij[u] & 0xffff, j = bij[u] >> 16, b = t + u * JR_NT;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          if (i == j && e != 1) continue;   // a diagonal block holds one off-diagonal entry, in element 1
+          const int x = 2 * i + (e >> 1), y = 2 * j + (e & 1);
+          const double axx = a[((x & 1) ? 3 * nblk : 0) + jblk(x >> 1, x >> 1)];
+          const double ayy = a[((y & 1) ? 3 * nblk : 0) + jblk(y >> 1, y >> 1)];
+          any |= jrot_needed(axx, a[e * nblk + b], ayy, abs_floor) ? 1 : 0;
+        }
+      }
+      if (!__syncthreads_or(any)) { ++sweep; break; }
+    }
   }
   // whole sweeps ran: every index is back at its round-0 position
   for (int i = t; i < m; i += JR_NT) {

@@ -14,6 +14,9 @@ measure of the network is the number of CTMRG iterations to tolerance.
 from __future__ import annotations
 
 import ctypes as C
+import json
+import os
+import sys
 import time
 
 import numpy as np
@@ -145,9 +148,11 @@ def random_bulk(D: int, s: int, seed: int, dist: str = "normal") -> np.ndarray:
 
 
 def generate_dataset(seeds, D: int, s: int, chi: int, tol: float, maxit: int, dist: str = "normal",
-                     ctx: Context | None = None, log=None, **opts):
+                     ctx: Context | None = None, log=None, every: int = 0, **opts):
     """Converged environments of seeded random bulks: X (n_in x n) bulk features, Y (n_out x n)
-    canonical environment features, and the :raw iteration counts."""
+    canonical environment features, and the :raw iteration counts.  `log` hears of progress every
+    `every` environments (default: eight times in all)."""
+    every = every or len(list(seeds)) // 8
     ctx = ctx or default_context()
     Dd = D * D
     seeds = list(seeds)
@@ -163,9 +168,90 @@ def generate_dataset(seeds, D: int, s: int, chi: int, tol: float, maxit: int, di
         X[:, j] = bulk_features(b, ctx)
         Y[:, j] = env_features(Cm, Tm, ctx)
         iters[j] = it
-        if log and (j + 1) % max(1, len(seeds) // 8) == 0:
+        if log and ((j + 1) % max(1, every) == 0 or j + 1 == len(seeds)):
             log(f"[warmstart] generated {j + 1}/{len(seeds)} environments ({time.time() - t0:.1f} s)")
     return X, Y, iters
+
+
+def _generate_shard(args):
+    """worker of generate_dataset_parallel: its own context (its own HIP stream), one shard of seeds"""
+    seeds, D, s, chi, tol, maxit, dist, path, opts = args
+    ctx = Context(0)
+    tag = os.path.basename(path)
+
+    def log(msg):
+        print(f"{tag}: {msg}", file=sys.stderr, flush=True)
+
+    try:
+        X, Y, it = generate_dataset(seeds, D, s, chi, tol, maxit, dist=dist, ctx=ctx, log=log, every=4, **opts)
+        np.savez(path, X=X, Y=Y, iters=it, seeds=np.asarray(seeds, dtype=np.int64))
+    finally:
+        ctx.close()
+    return path
+
+
+def generate_dataset_parallel(seeds, D: int, s: int, chi: int, tol: float, maxit: int, workers: int,
+                              workdir: str, dist: str = "normal", log=None, **opts):
+    """generate_dataset over `workers` processes sharing the GPU, each with its own context and stream:
+    the CTMRG of one bulk is a chain of small, mostly one-workgroup kernels and host round trips, so
+    independent bulks run side by side on the other compute units.  Shard files go to `workdir`; the
+    result is in seed order."""
+    import multiprocessing as mp
+    seeds = list(seeds)
+    workers = max(1, min(workers, len(seeds)))
+    shards = [seeds[i::workers] for i in range(workers)]
+    jobs = [(sh, D, s, chi, tol, maxit, dist, os.path.join(workdir, f"shard{i}.npz"), opts)
+            for i, sh in enumerate(shards)]
+    t0 = time.time()
+    with mp.get_context("spawn").Pool(workers) as pool:
+        paths = pool.map(_generate_shard, jobs, chunksize=1)
+    if log:
+        log(f"[warmstart] generated {len(seeds)} environments on {workers} workers ({time.time() - t0:.1f} s)")
+    order = {sd: j for j, sd in enumerate(seeds)}
+    X = Y = None
+    iters = np.zeros(len(seeds), dtype=np.int64)
+    for pth in paths:
+        z = np.load(pth, allow_pickle=False)
+        if X is None:
+            X = np.zeros((z["X"].shape[0], len(seeds)), dtype=np.float32, order="F")
+            Y = np.zeros((z["Y"].shape[0], len(seeds)), dtype=np.float32, order="F")
+        cols = [order[int(sd)] for sd in z["seeds"]]
+        X[:, cols] = z["X"]
+        Y[:, cols] = z["Y"]
+        iters[cols] = z["iters"]
+    return X, Y, iters
+
+
+class Compressor:
+    """Target compression for the network: the training mean plus the leading r principal directions of
+    the canonical environment features, fitted once on the training set (host, FP64).  The network predicts
+    the r coefficients; decoding (mean + P c) runs as a library contraction on the GPU."""
+
+    def __init__(self, Y, r: int):
+        Y = np.asarray(Y, dtype=np.float64)
+        self.mean = Y.mean(axis=1)
+        Yc = Y - self.mean[:, None]
+        # principal directions from the n x n Gram matrix (n samples << n_out features)
+        G = Yc.T @ Yc
+        w, V = np.linalg.eigh(G)
+        o = np.argsort(w)[::-1][:r]
+        w, V = w[o], V[:, o]
+        keep = w > 1e-12 * max(w[0], 1e-300)
+        w, V = w[keep], V[:, keep]
+        self.P = np.asfortranarray((Yc @ V) / np.sqrt(w)[None, :])   # n_out x r, orthonormal columns
+        self.r = self.P.shape[1]
+        self.explained = float(np.sum(w) / max(np.sum(np.einsum("ij,ij->j", Yc, Yc)), 1e-300))
+
+    def encode(self, Y):
+        return np.asfortranarray((self.P.T @ (np.asarray(Y, dtype=np.float64) - self.mean[:, None])).astype(np.float32))
+
+    def decode(self, Cf, ctx: Context | None = None):
+        """mean + P c for coefficient columns Cf (r x B), on the GPU (FP64 contraction)"""
+        ctx = ctx or default_context()
+        Cf = np.asfortranarray(np.asarray(Cf, dtype=np.float64))
+        B = Cf.shape[1]
+        base = np.asfortranarray(np.repeat(self.mean[:, None], B, axis=1))
+        return api.contract("or,rb->ob", self.P, Cf, ctx=ctx, beta=1.0, Cin=base).astype(np.float32)
 
 
 def train(net: WarmStartNet, X, Y, epochs: int, batch: int = 64, lr: float = 1e-3, seed: int = 0,
@@ -194,6 +280,67 @@ def predict_env(net: WarmStartNet, bulk, chi: int, D: int, ctx: Context | None =
     return env_from_features(f, chi, D, ctx)
 
 
+def predict_envs(net: WarmStartNet, bulks, chi: int, D: int, comp: Compressor | None = None,
+                 ctx: Context | None = None):
+    """starting environments for a batch of bulks: one network launch for all of them (B columns)"""
+    ctx = ctx or default_context()
+    Xb = np.stack([bulk_features(b, ctx) for b in bulks], axis=1)
+    F = net.forward(Xb)
+    if comp is not None:
+        F = comp.decode(F, ctx)
+    return [env_from_features(F[:, j], chi, D, ctx) for j in range(F.shape[1])]
+
+
+def _compare_shard(args):
+    seeds, D, s, chi, tol, maxit, dist, theta, shape, comp_path, opts = args
+    ctx = Context(0)
+    try:
+        n_in, n_out, hidden, layers = shape
+        net = WarmStartNet(n_in, n_out, hidden=hidden, layers=layers, seed=0, ctx=ctx)
+        net.set_params(theta)
+        comp = None
+        if comp_path:
+            z = np.load(comp_path, allow_pickle=False)
+            comp = Compressor.__new__(Compressor)
+            comp.mean, comp.P, comp.r = z["mean"], np.asfortranarray(z["P"]), int(z["P"].shape[1])
+        bulks = [random_bulk(D, s, sd, dist) for sd in seeds]
+        envs = predict_envs(net, bulks, chi, D * D, comp, ctx)
+        raw, learned = [], []
+        for b, env0 in zip(bulks, envs):
+            a = double_layer_of(b, ctx)
+            raw.append(ctmrg_from(a, chi, tol, maxit, ctx=ctx, **opts)[2])
+            learned.append(ctmrg_from(a, chi, tol, maxit, env0=env0, ctx=ctx, **opts)[2])
+        net.free()
+        return list(seeds), raw, learned
+    finally:
+        ctx.close()
+
+
+def compare_iterations_parallel(net: WarmStartNet, seeds, D: int, s: int, chi: int, tol: float, maxit: int,
+                                workers: int, workdir: str, comp: Compressor | None = None, dist: str = "normal",
+                                **opts):
+    """compare_iterations over `workers` processes (each predicts its shard's starts in one batched launch)"""
+    import multiprocessing as mp
+    seeds = list(seeds)
+    workers = max(1, min(workers, len(seeds)))
+    comp_path = ""
+    if comp is not None:
+        comp_path = os.path.join(workdir, "compressor.npz")
+        np.savez(comp_path, mean=comp.mean, P=comp.P)
+    theta = net.get_params()
+    shape = (net.n_in, net.n_out, net.hidden, net.layers)
+    jobs = [(seeds[i::workers], D, s, chi, tol, maxit, dist, theta, shape, comp_path, opts) for i in range(workers)]
+    with mp.get_context("spawn").Pool(workers) as pool:
+        res = pool.map(_compare_shard, jobs, chunksize=1)
+    by = {}
+    for sds, r, l in res:
+        for sd, a, b in zip(sds, r, l):
+            by[sd] = (a, b)
+    raw = np.array([by[sd][0] for sd in seeds])
+    learned = np.array([by[sd][1] for sd in seeds])
+    return raw, learned
+
+
 def compare_iterations(net: WarmStartNet, seeds, D: int, s: int, chi: int, tol: float, maxit: int,
                        dist: str = "normal", ctx: Context | None = None, **opts):
     """CTMRG iterations to tol from :raw and from the network's prediction, per held-out bulk,
@@ -212,66 +359,132 @@ def compare_iterations(net: WarmStartNet, seeds, D: int, s: int, chi: int, tol: 
     return np.array(raw), np.array(learned)
 
 
-def main(argv=None):
-    """Train and evaluate a warm-start network end to end on the GPU and print a JSON summary:
-    python -m tnad.warmstart --D 3 --chi 64 --train 4096 --test 64"""
-    import argparse
-    import json
-    import sys
-    p = argparse.ArgumentParser(description=main.__doc__)
-    p.add_argument("--D", type=int, default=3, help="iPEPS virtual bond (double-layer bond D^2)")
-    p.add_argument("--s", type=int, default=2, help="physical dimension")
-    p.add_argument("--chi", type=int, default=64)
-    p.add_argument("--tol", type=float, default=1e-8)
-    p.add_argument("--maxit", type=int, default=300)
-    p.add_argument("--train", type=int, default=4096)
-    p.add_argument("--test", type=int, default=64)
-    p.add_argument("--hidden", type=int, default=512)
-    p.add_argument("--layers", type=int, default=2)
-    p.add_argument("--epochs", type=int, default=100)
-    p.add_argument("--batch", type=int, default=64)
-    p.add_argument("--lr", type=float, default=1e-3)
-    p.add_argument("--lr-decay", type=float, default=0.98)
-    p.add_argument("--save", default="", help="write the trained parameters to this .npy file")
-    a = p.parse_args(argv)
-    log = lambda m: print(m, file=sys.stderr, flush=True)
-    ctx = default_context()
-    Dd = a.D * a.D
-    # environments generated with the eigensolver warm-started across CTMRG steps (same fixed point)
+def _seed_range(txt):
+    lo, hi = (int(x) for x in txt.split(":"))
+    return range(lo, hi)
+
+
+def cmd_generate(a, log):
+    """generate converged environments for a seed range; keep only their coordinates in a basis of principal
+    directions (fitted here with --fit-rank, or read from --basis), so that the result stays small"""
+    import tempfile
+    os.makedirs(a.out, exist_ok=True)
+    seeds = _seed_range(a.seeds)
+    opts = dict(eig_warm=True, precision="fp32", switch_tol=1e-5)   # fp64 steps to tol: the same fixed point
+    workdir = tempfile.mkdtemp(prefix="shards_", dir=a.out)
     t0 = time.time()
-    X, Y, it_train = generate_dataset(range(a.train), a.D, a.s, a.chi, a.tol, a.maxit, ctx=ctx, log=log,
-                                      eig_warm=True)
+    X, Y, iters = generate_dataset_parallel(seeds, a.D, a.s, a.chi, a.tol, a.maxit, a.workers, workdir, log=log,
+                                            **opts)
     t_gen = time.time() - t0
-    net = WarmStartNet(X.shape[0], Y.shape[0], hidden=a.hidden, layers=a.layers, seed=0, ctx=ctx)
+    if a.fit_rank > 0:
+        comp = Compressor(Y, a.fit_rank)
+        np.savez(os.path.join(a.out, "basis.npz"), mean=comp.mean.astype(np.float32), P=comp.P.astype(np.float32),
+                 explained=comp.explained)
+    else:
+        z = np.load(a.basis, allow_pickle=False)
+        comp = Compressor.__new__(Compressor)
+        comp.mean, comp.P = z["mean"].astype(np.float64), np.asfortranarray(z["P"].astype(np.float64))
+        comp.r = comp.P.shape[1]
+    coef = comp.encode(Y)
+    resid = float(np.linalg.norm(comp.P.astype(np.float64) @ coef.astype(np.float64) + comp.mean[:, None] - Y) /
+                  np.linalg.norm(Y - comp.mean[:, None]))
+    np.savez(os.path.join(a.out, f"targets_{seeds.start}_{seeds.stop}.npz"), coef=coef, iters=iters,
+             seeds=np.asarray(list(seeds), dtype=np.int64))
+    for f in os.listdir(workdir):
+        os.remove(os.path.join(workdir, f))
+    os.rmdir(workdir)
+    print(json.dumps({"seeds": a.seeds, "generate_s": round(t_gen, 1), "mean_iters": float(iters.mean()),
+                      "rank": comp.r, "relative_residual_in_basis": resid}), flush=True)
+
+
+def cmd_train(a, log):
+    """train on the generated coordinates (network inputs recomputed from the seeds), then compare CTMRG
+    iterations to tol from :raw and from the prediction on held-out seeds"""
+    import tempfile
+    ctx = default_context()
+    z = np.load(a.basis, allow_pickle=False)
+    comp = Compressor.__new__(Compressor)
+    comp.mean, comp.P = z["mean"].astype(np.float64), np.asfortranarray(z["P"].astype(np.float64))
+    comp.r, comp.explained = comp.P.shape[1], float(z["explained"])
+    coefs, seeds, iters = [], [], []
+    for f in a.data.split(","):
+        t = np.load(f, allow_pickle=False)
+        coefs.append(t["coef"]); seeds.extend(int(x) for x in t["seeds"]); iters.append(t["iters"])
+    T = np.asfortranarray(np.concatenate(coefs, axis=1))
+    it_train = np.concatenate(iters)
+    X = np.stack([bulk_features(random_bulk(a.D, a.s, sd), ctx) for sd in seeds], axis=1)
+    net = WarmStartNet(X.shape[0], T.shape[0], hidden=a.hidden, layers=a.layers, seed=0, ctx=ctx)
     t0 = time.time()
-    hist = train(net, X, Y, a.epochs, batch=a.batch, lr=a.lr, lr_decay=a.lr_decay, log=log)
+    hist = train(net, X, T, a.epochs, batch=a.batch, lr=a.lr, lr_decay=a.lr_decay, log=log)
     t_train = time.time() - t0
     test_seeds = range(10 ** 6, 10 ** 6 + a.test)
-    Xte = np.stack([bulk_features(random_bulk(a.D, a.s, sd), ctx) for sd in test_seeds], axis=1)
-    _, Yte, _ = generate_dataset(test_seeds, a.D, a.s, a.chi, a.tol, a.maxit, ctx=ctx, eig_warm=True)
-    Yp = net.forward(Xte)
+    opts = dict(eig_warm=True, precision="fp32", switch_tol=1e-5)
+    workdir = tempfile.mkdtemp(prefix="test_")
+    Xte, Yte, _ = generate_dataset_parallel(test_seeds, a.D, a.s, a.chi, a.tol, a.maxit, a.workers, workdir, **opts)
+    Yp = comp.decode(net.forward(Xte), ctx)
+    mean_pred = comp.mean.astype(np.float32)[:, None]
     test_loss = 0.5 * float(np.sum((Yp - Yte) ** 2)) / a.test
-    base_loss = 0.5 * float(np.sum((Y.mean(axis=1, keepdims=True) - Yte) ** 2)) / a.test
+    base_loss = 0.5 * float(np.sum((mean_pred - Yte) ** 2)) / a.test
+    basis_loss = 0.5 * float(np.sum((comp.decode(comp.encode(Yte), ctx) - Yte) ** 2)) / a.test
     t0 = time.time()
-    raw, learned = compare_iterations(net, test_seeds, a.D, a.s, a.chi, a.tol, a.maxit, ctx=ctx, eig_warm=True)
+    raw, learned = compare_iterations_parallel(net, test_seeds, a.D, a.s, a.chi, a.tol, a.maxit, a.workers, workdir,
+                                               comp=comp, **opts)
     t_eval = time.time() - t0
     if a.save:
         np.save(a.save, net.get_params())
     out = {
-        "config": {"D": a.D, "s": a.s, "chi": a.chi, "tol": a.tol, "train": a.train, "test": a.test,
-                   "n_in": X.shape[0], "n_out": Y.shape[0], "hidden": a.hidden, "layers": a.layers,
-                   "params": net.num_params, "epochs": a.epochs, "batch": a.batch, "lr": a.lr},
-        "seconds": {"generate": round(t_gen, 1), "train": round(t_train, 1), "evaluate": round(t_eval, 1)},
+        "config": {"D": a.D, "s": a.s, "chi": a.chi, "tol": a.tol, "train": int(T.shape[1]), "test": a.test,
+                   "n_in": int(X.shape[0]), "n_out": int(comp.P.shape[0]), "rank": comp.r,
+                   "explained_on_train": comp.explained, "hidden": a.hidden, "layers": a.layers,
+                   "params": net.num_params, "epochs": a.epochs, "batch": a.batch, "lr": a.lr, "workers": a.workers},
+        "seconds": {"train": round(t_train, 1), "evaluate": round(t_eval, 1)},
         "train_loss_first_last": [hist[0], hist[-1]],
-        "test_loss": test_loss, "test_loss_of_mean_predictor": base_loss,
+        "test_loss": test_loss, "test_loss_of_mean_predictor": base_loss, "test_loss_of_basis_projection": basis_loss,
         "raw_iters_train_mean": float(np.mean(it_train)),
         "iters_to_tol_raw": raw.tolist(), "iters_to_tol_learned": learned.tolist(),
         "mean_raw": float(raw.mean()), "mean_learned": float(learned.mean()),
         "median_raw": float(np.median(raw)), "median_learned": float(np.median(learned)),
         "fewer": int(np.sum(learned < raw)), "more": int(np.sum(learned > raw)),
     }
-    print(json.dumps(out))
+    print(json.dumps(out), flush=True)
     net.free()
+
+
+def main(argv=None):
+    """The warm-start pipeline at the D=3, chi=64 workload, in stages whose files stay small:
+      python -m tnad.warmstart generate --seeds 0:2048 --fit-rank 128 --out DIR --workers 16
+      python -m tnad.warmstart generate --seeds 2048:4096 --basis DIR/basis.npz --out DIR --workers 16
+      python -m tnad.warmstart train --basis DIR/basis.npz --data DIR/targets_0_2048.npz,... --test 64"""
+    import argparse
+    p = argparse.ArgumentParser(description=main.__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    p.add_argument("cmd", choices=["generate", "train"])
+    p.add_argument("--D", type=int, default=3, help="iPEPS virtual bond (double-layer bond D^2)")
+    p.add_argument("--s", type=int, default=2, help="physical dimension")
+    p.add_argument("--chi", type=int, default=64)
+    p.add_argument("--tol", type=float, default=1e-8)
+    p.add_argument("--maxit", type=int, default=300)
+    p.add_argument("--workers", type=int, default=12, help="processes sharing the GPU")
+    p.add_argument("--seeds", default="0:4096")
+    p.add_argument("--out", default="warmstart_data")
+    p.add_argument("--fit-rank", type=int, default=0, help="fit a basis of this many principal directions")
+    p.add_argument("--basis", default="")
+    p.add_argument("--data", default="")
+    p.add_argument("--test", type=int, default=64)
+    p.add_argument("--hidden", type=int, default=512)
+    p.add_argument("--layers", type=int, default=2)
+    p.add_argument("--epochs", type=int, default=200)
+    p.add_argument("--batch", type=int, default=64)
+    p.add_argument("--lr", type=float, default=1e-3)
+    p.add_argument("--lr-decay", type=float, default=0.99)
+    p.add_argument("--save", default="", help="write the trained parameters to this .npy file")
+    a = p.parse_args(argv)
+    log = lambda m: print(m, file=sys.stderr, flush=True)
+    if a.cmd == "generate":
+        if a.fit_rank <= 0 and not a.basis:
+            p.error("generate needs --fit-rank or --basis")
+        cmd_generate(a, log)
+    else:
+        cmd_train(a, log)
 
 
 if __name__ == "__main__":
