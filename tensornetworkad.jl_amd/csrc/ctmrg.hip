@@ -282,7 +282,7 @@ void tc(Ctx* c, Prec p, const TView& A, const void* a, const TView& B, const voi
 // Opt-in diagnostics (TNAD_DEBUG_EIG=1): report whether a device array holds a non-finite value.
 static void dbg_finite(Ctx* c, const char* what, const double* dev, size_t n) {
   static const bool on = [] {
-    const char* e = std::getenv("TNAD_DEBUG_EIG");
+    const char* e = std::getenv("TNAD_DEBUG_FINITE");
     return e && *e && *e != '0';
   }();
   if (!on) return;
