@@ -88,6 +88,7 @@ inline int step_k(const StepDims& d, const StepOpts& o) {
 struct StepTape {
   Prec prec;
   int k;
+  double eig_tol = 1e-13;   // relative Ritz residual the step's eigensolver was run to
   DevBuf C, T;          // step inputs (FP64)
   DevBuf X, AX, theta;  // eigenpairs (FP64): X = [z | extra], AX = A X
   DevBuf M;             // cpmat + cpmat^T (FP64)

@@ -631,6 +631,7 @@ void ctmrg_step(Ctx* c, const StepDims& d, const StepOpts& o, const double* a64,
   k_copy(T2, Tout, (size_t)chi * D * chi, s);
   k_scale_by_dev(Tout, (size_t)chi * D * chi, nrm + 1, s);
   if (tape) {
+    tape->eig_tol = p == Prec::F64 ? o.eig_tol : o.eig_tol_low;
     k_copy(C, tape->C.as<double>(), (size_t)chi * chi, s);
     k_copy(T, tape->T.as<double>(), (size_t)chi * D * chi, s);
   }
@@ -721,12 +722,23 @@ void eig_adjoint(Ctx* c, const StepDims& d, const StepTape& tp, const double* G,
     };
     // A step computed with low-precision contractions resolved its eigenpairs only to the relative
     // residual its contractions can support (1e-8 and above), so this adjoint is no more accurate
-    // than that whatever the solve does: its complement part is solved to 1e-10 instead of 1e-13, and
+    // than that whatever the solve does: its complement part is solved to 1e-8 instead of 1e-13, and
     // no further than 1e-13 of the reverse sweep's largest right-hand side (the early steps of a
-    // converging CTMRG receive cotangents many orders smaller than the last ones).
+    // converging CTMRG receive cotangents many orders smaller than the last ones).  The same holds
+    // for an FP64 step whose eigensolver ran to a looser residual than 1e-13 (the mixed path ties it to
+    // the step-to-step change).
     const bool low = tp.prec != Prec::F64;
+    const bool loose = low || tp.eig_tol > 1e-13;
+    const double pcg_tol = loose ? 1e-8 : 1e-13;
     iters = resolvent_pcg(c, n, chi, k, X, sq.as<double>(), sq.as<double>() + chi, applyA2, G, Xc.as<double>(),
-                          low ? 1e-10 : 1e-13, 200, 1e-40, sweep_ref, low ? 1e-13 : 0.0);
+                          pcg_tol, 200, 1e-40, sweep_ref, loose ? 1e-13 : 0.0);
+    static const bool dbg = [] {
+      const char* e = std::getenv("TNAD_DEBUG_EIG");
+      return e && *e && *e != '0';
+    }();
+    if (dbg)
+      std::fprintf(stderr, "[tnad eig] adjoint n=%d chi=%d prec=%s eig_tol=%.1e: %d resolvent CG iterations (tol %.1e)\n",
+                   n, chi, low ? "low" : "fp64", tp.eig_tol, iters, pcg_tol);
     // W += X_c Lambda1 + A X_c
     tc_launch(c, Prec::F64, gemm_desc('N', 'N', n, chi, n, n, n, n), M, Xc.as<double>(), AXc.as<double>(), s);
     hipLaunchKernelGGL(add_lam_cols_kernel, dim3(nb((size_t)n * chi)), dim3(NT), 0, s, W.as<double>(),
