@@ -22,6 +22,8 @@
 //   BF16: v_mfma_f32_16x16x16bf16_1k  (BK = 32, fp32 in HBM, bf16 in LDS,
 //                                      fp32 accumulation)
 #include <algorithm>
+#include <cstdint>
+#include <cstdlib>
 
 #include "tnad_internal.h"
 
@@ -279,6 +281,166 @@ __global__ __launch_bounds__(NT, PF == 2 ? (sizeof(T) == 8 ? 3 : 4) : 1) void tc
   }
 }
 
+// ---------------------------------------------------------------------------
+// F64 / F32 kernel with 16-byte operand loads (tc_vec_ok decides when it applies): each operand is read
+// in vectors of V = 16 / sizeof(T) elements along its unit-stride index (m for A or n for B when that
+// index has unit stride, else k), so a thread issues V times fewer loads and no per-element address
+// arithmetic; the K-tile never straddles a K0 run (K0 % 16 == 0 or a single K sub-index) and K is a
+// multiple of the K-tile, so the K loop carries no bounds tests.  64 x 64 output tile, 4 waves in
+// 2 x 2, K-tile 16, double-buffered LDS (one barrier per K-tile), the next K-tile's vectors in flight in
+// registers while the current one is multiplied, and every LDS fragment of a K-tile read before its
+// MFMAs.
+// ---------------------------------------------------------------------------
+template <typename T> struct Vec16;
+template <> struct Vec16<double> { typedef double2 type; static constexpr int V = 2; };
+template <> struct Vec16<float> { typedef float4 type; static constexpr int V = 4; };
+
+template <typename T, bool A_MV, bool B_NV>
+__global__ __launch_bounds__(NT, sizeof(T) == 8 ? 3 : 4) void tc_kernel_vec(TcDesc d, const T* __restrict__ A,
+                                                                         const T* __restrict__ B,
+                                                                         T* __restrict__ C, T* __restrict__ Wsp) {
+  constexpr int TILE = 64, BK = 16, PAD = 16 / sizeof(T) * 2;
+  typedef typename Vec16<T>::type VT;
+  constexpr int V = Vec16<T>::V;
+  constexpr int NVEC = TILE * BK / (NT * V);   // vectors per thread and operand per K-tile
+  constexpr int FB = TILE / 32;
+  static_assert(NVEC >= 1 && NVEC * NT * V == TILE * BK, "loader coverage");
+  __shared__ __attribute__((aligned(16))) T As[2][BK][TILE + PAD];
+  __shared__ __attribute__((aligned(16))) T Bs[2][BK][TILE + PAD];
+
+  const int M = d.M0 * d.M1, N = d.N0 * d.N1;
+  const int tile_m = blockIdx.x * TILE, tile_n = blockIdx.y * TILE;
+  const int bz = Wsp ? 0 : blockIdx.z;
+  const int kbeg = Wsp ? blockIdx.z * d.kchunk : 0;
+  const int K = Wsp ? min(d.K0 * d.K1, kbeg + d.kchunk) : d.K0 * d.K1;
+  A += (long long)bz * d.abat;
+  B += (long long)bz * d.bbat;
+  C += (long long)bz * d.cbat;
+
+  const int t = threadIdx.x;
+  const int lane = t & 63, wave = t >> 6;
+  const int wm = wave & 1, wn = wave >> 1;
+
+  // loader slots: vectors along m (n) cover a K-tile row by TILE / V threads; vectors along k cover
+  // an m (n) column by BK / V threads
+  int a_m[NVEC], a_k[NVEC], b_n[NVEC], b_k[NVEC];
+  long long a_off[NVEC], b_off[NVEC];
+  bool a_ok[NVEC], b_ok[NVEC];
+  KPos a_kp[NVEC], b_kp[NVEC];
+  const long long a_dk = d.ak1 - (long long)d.K0 * d.ak0, b_dk = d.bk1 - (long long)d.K0 * d.bk0;
+#pragma unroll
+  for (int r = 0; r < NVEC; ++r) {
+    if (A_MV) { a_m[r] = (t % (TILE / V)) * V; a_k[r] = t / (TILE / V) + r * (NT * V / TILE); }
+    else      { a_k[r] = (t % (BK / V)) * V;   a_m[r] = t / (BK / V) + r * (NT * V / BK); }
+    if (B_NV) { b_n[r] = (t % (TILE / V)) * V; b_k[r] = t / (TILE / V) + r * (NT * V / TILE); }
+    else      { b_k[r] = (t % (BK / V)) * V;   b_n[r] = t / (BK / V) + r * (NT * V / BK); }
+    const int m = tile_m + a_m[r], n = tile_n + b_n[r];
+    a_ok[r] = m < M;
+    b_ok[r] = n < N;
+    a_off[r] = a_ok[r] ? off2(m, d.M0, d.am0, d.am1) : 0;
+    b_off[r] = b_ok[r] ? off2(n, d.N0, d.bn0, d.bn1) : 0;
+    a_kp[r].init(kbeg + a_k[r], d.K0, d.ak0, d.ak1);
+    b_kp[r].init(kbeg + b_k[r], d.K0, d.bk0, d.bk1);
+  }
+  auto gload = [&](VT (&xa)[NVEC], VT (&xb)[NVEC]) {
+#pragma unroll
+    for (int r = 0; r < NVEC; ++r) {
+      VT va = {}, vb = {};
+      if (a_ok[r]) va = *reinterpret_cast<const VT*>(A + a_off[r] + a_kp[r].off);
+      if (b_ok[r]) vb = *reinterpret_cast<const VT*>(B + b_off[r] + b_kp[r].off);
+      xa[r] = va;
+      xb[r] = vb;
+      // one K0 run holds whole K-tiles: at most one carry per tile
+      a_kp[r].k0 += BK;
+      a_kp[r].off += (long long)BK * d.ak0;
+      if (a_kp[r].k0 >= d.K0) { a_kp[r].k0 -= d.K0; a_kp[r].off += a_dk; }
+      b_kp[r].k0 += BK;
+      b_kp[r].off += (long long)BK * d.bk0;
+      if (b_kp[r].k0 >= d.K0) { b_kp[r].k0 -= d.K0; b_kp[r].off += b_dk; }
+    }
+  };
+  auto stage = [&](const VT (&xa)[NVEC], const VT (&xb)[NVEC], int buf) {
+#pragma unroll
+    for (int r = 0; r < NVEC; ++r) {
+      const T* ea = reinterpret_cast<const T*>(&xa[r]);
+      const T* eb = reinterpret_cast<const T*>(&xb[r]);
+      if (A_MV) *reinterpret_cast<VT*>(&As[buf][a_k[r]][a_m[r]]) = xa[r];
+      else {
+#pragma unroll
+        for (int e = 0; e < V; ++e) As[buf][a_k[r] + e][a_m[r]] = ea[e];
+      }
+      if (B_NV) *reinterpret_cast<VT*>(&Bs[buf][b_k[r]][b_n[r]]) = xb[r];
+      else {
+#pragma unroll
+        for (int e = 0; e < V; ++e) Bs[buf][b_k[r] + e][b_n[r]] = eb[e];
+      }
+    }
+  };
+
+  typedef typename MfmaOp<T>::acc_t acc_t;
+  acc_t acc[FB][FB];
+#pragma unroll
+  for (int i = 0; i < FB; ++i)
+#pragma unroll
+    for (int j = 0; j < FB; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
+  auto mfma_tile = [&](int buf) {
+    T af[BK / 4][FB], bf[BK / 4][FB];
+#pragma unroll
+    for (int kk = 0; kk < BK / 4; ++kk) {
+#pragma unroll
+      for (int i = 0; i < FB; ++i) af[kk][i] = As[buf][4 * kk + (lane >> 4)][wm * (TILE / 2) + i * 16 + (lane & 15)];
+#pragma unroll
+      for (int j = 0; j < FB; ++j) bf[kk][j] = Bs[buf][4 * kk + (lane >> 4)][wn * (TILE / 2) + j * 16 + (lane & 15)];
+    }
+#pragma unroll
+    for (int kk = 0; kk < BK / 4; ++kk)
+#pragma unroll
+      for (int i = 0; i < FB; ++i)
+#pragma unroll
+        for (int j = 0; j < FB; ++j) acc[i][j] = MfmaOp<T>::run(af[kk][i], bf[kk][j], acc[i][j]);
+  };
+
+  VT ra[NVEC], rb[NVEC];
+  gload(ra, rb);
+  stage(ra, rb, 0);
+  if (kbeg + BK < K) gload(ra, rb);
+  __syncthreads();
+  int buf = 0;
+  for (int k0 = kbeg; k0 < K; k0 += BK, buf ^= 1) {
+    mfma_tile(buf);
+    if (k0 + BK < K) {
+      stage(ra, rb, buf ^ 1);
+      if (k0 + 2 * BK < K) gload(ra, rb);
+    }
+    __syncthreads();
+  }
+
+  const T alpha = (T)d.alpha, beta = (T)d.beta;
+  T* Wz = Wsp ? Wsp + (size_t)blockIdx.z * M * N : nullptr;
+#pragma unroll
+  for (int j = 0; j < FB; ++j) {
+    int n = tile_n + wn * (TILE / 2) + j * 16 + (lane & 15);
+    if (n >= N) continue;
+    long long noff = off2(n, d.N0, d.cn0, d.cn1);
+#pragma unroll
+    for (int i = 0; i < FB; ++i) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        int m = tile_m + wm * (TILE / 2) + i * 16 + MfmaOp<T>::row(lane, r);
+        if (m >= M) continue;
+        if (Wz) {
+          Wz[m + (size_t)M * n] = (T)acc[i][j][r];
+          continue;
+        }
+        long long o = noff + off2(m, d.M0, d.cm0, d.cm1);
+        T v = alpha * (T)acc[i][j][r];
+        if (d.beta != 0.0) v += beta * C[o];
+        C[o] = v;
+      }
+    }
+  }
+}
+
 // split-K reduction: C = alpha * sum_s W[s] + beta * C over the strided output (slices summed in
 // order, so the result does not depend on scheduling)
 template <typename T>
@@ -447,6 +609,47 @@ static size_t extent_bytes(long long x0, long long s0, long long x1, long long s
   return (size_t)(last + 1) * es;
 }
 
+static bool tc_vec_disabled() {
+  static const bool off = [] {
+    const char* e = std::getenv("TNAD_TC_NOVEC");
+    return e && *e && *e != '0';
+  }();
+  return off;
+}
+
+// Vector layout of one operand for tc_kernel_vec (V elements of 16 bytes): 1 = along its M (N) index,
+// 2 = along K, 0 = none.  X0, X1 / x0, x1: extents and strides of the operand's M (N) sub-indices; K0, K1 /
+// k0, k1 those of K.  Every vector must be V consecutive elements of one sub-index run starting at a
+// multiple of V elements from a 16-byte aligned base.
+static int vec_mode(int X0, int X1, long long x0, long long x1, int K0, int K1, long long k0, long long k1,
+                    int batch, long long bat, const void* p, int V) {
+  auto mult = [&](long long stride, int ext) { return ext <= 1 || stride % V == 0; };
+  if (reinterpret_cast<uintptr_t>(p) % 16 != 0 || !mult(bat, batch)) return 0;
+  if (X0 > 1 && x0 == 1 && X0 % V == 0 && mult(x1, X1) && mult(k0, K0) && mult(k1, K1)) return 1;
+  if (K0 > 1 && k0 == 1 && K0 % V == 0 && mult(k1, K1) && mult(x0, X0) && mult(x1, X1)) return 2;
+  return 0;
+}
+
+template <typename T>
+static bool launch_vec(const TcDesc& dd, dim3 grid, const void* A, const void* B, void* C, void* W, hipStream_t s) {
+  constexpr int V = 16 / sizeof(T);
+  const long long K = (long long)dd.K0 * dd.K1;
+  // whole K-tiles, each inside one K0 run
+  if (K % 16 != 0 || (dd.K1 > 1 && dd.K0 % 16 != 0)) return false;
+  const int ma = vec_mode(dd.M0, dd.M1, dd.am0, dd.am1, dd.K0, dd.K1, dd.ak0, dd.ak1, dd.batch, dd.abat, A, V);
+  const int mb = vec_mode(dd.N0, dd.N1, dd.bn0, dd.bn1, dd.K0, dd.K1, dd.bk0, dd.bk1, dd.batch, dd.bbat, B, V);
+  if (ma == 0 || mb == 0) return false;
+  const T* a = (const T*)A;
+  const T* b = (const T*)B;
+  T* c = (T*)C;
+  T* w = (T*)W;
+  if (ma == 1 && mb == 1) hipLaunchKernelGGL((tc_kernel_vec<T, true, true>), grid, dim3(NT), 0, s, dd, a, b, c, w);
+  else if (ma == 1) hipLaunchKernelGGL((tc_kernel_vec<T, true, false>), grid, dim3(NT), 0, s, dd, a, b, c, w);
+  else if (mb == 1) hipLaunchKernelGGL((tc_kernel_vec<T, false, true>), grid, dim3(NT), 0, s, dd, a, b, c, w);
+  else hipLaunchKernelGGL((tc_kernel_vec<T, false, false>), grid, dim3(NT), 0, s, dd, a, b, c, w);
+  return true;
+}
+
 void tc_launch(Ctx* c, Prec p, const TcDesc& d, const void* A, const void* B, void* C, hipStream_t s) {
   const long long M = (long long)d.M0 * d.M1, N = (long long)d.N0 * d.N1;
   {
@@ -496,7 +699,11 @@ void tc_launch(Ctx* c, Prec p, const TcDesc& d, const void* A, const void* B, vo
   const dim3 grid_big((unsigned)((M + 127) / 128), (unsigned)((N + 127) / 128), grid.z);
   const bool big = !small && nsplit == 1 && (long long)grid_big.x * grid_big.y * grid_big.z >= TC_BIG_MIN_WG;
   const bool deep = !big && !small && (long long)grid.x * grid.y * grid.z < TC_DEEP_BELOW;
-  switch (p) {
+  // 64 x 64 tiles: the 16-byte-load kernel where the operand layouts allow it
+  const bool vec = !big && !small && !tc_vec_disabled() &&
+                   (p == Prec::F64 ? launch_vec<double>(dd, grid, A, B, C, W, s)
+                                   : p == Prec::F32 ? launch_vec<float>(dd, grid, A, B, C, W, s) : false);
+  if (!vec) switch (p) {
     case Prec::F64:
       if (small)
         hipLaunchKernelGGL((tc_kernel<double, TC_BK, 32>), grid, dim3(NT), 0, s, dd, (const double*)A,

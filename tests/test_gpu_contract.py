@@ -114,3 +114,45 @@ def test_split_k_strided_bulk_cotangent(ctx, prec):
     got = tnad.contract("ijlk,ibcl->jkcb", A, B, precision=prec, ctx=ctx)
     ref = np.einsum("ijlk,ibcl->jkcb", A, B)
     assert _rel(got, ref) < 10 * TOL[prec]
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+@pytest.mark.parametrize("tr", ["NN", "NT", "TN", "TT"])
+@pytest.mark.parametrize("m,n,k", [(256, 192, 512), (2048, 128, 2048), (136, 72, 48), (64, 64, 16)])
+def test_gemm_vector_loads(ctx, prec, tr, m, n, k):
+    # sizes whose unit-stride index and K are multiples of the 16-byte vector and of the K-tile take the
+    # vector-load kernel (also with split-K at 2048 x 128 x 2048, and with partial output tiles at
+    # 136 x 72); alpha / beta applied to a strided output
+    import tnad
+    print(f"case: vector-load gemm {prec} {tr} {m}x{n}x{k}", flush=True)
+    rng = np.random.default_rng(m + 3 * n + 7 * k + len(tr))
+    spec = {"NN": "ik,kj->ij", "NT": "ik,jk->ij", "TN": "ki,kj->ij", "TT": "ki,jk->ij"}[tr]
+    A = rng.standard_normal((m, k) if tr[0] == "N" else (k, m))
+    B = rng.standard_normal((k, n) if tr[1] == "N" else (n, k))
+    C0 = rng.standard_normal((m, n))
+    got = tnad.contract(spec, A, B, precision=prec, ctx=ctx, alpha=0.5, beta=-1.0, Cin=C0)
+    ref = 0.5 * np.einsum(spec, A, B) - C0
+    assert _rel(got, ref) < 10 * TOL[prec]
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+@pytest.mark.parametrize("spec", SPECS[:15])
+def test_step_contractions_vector_loads(ctx, prec, spec):
+    # the CTMRG step's contractions at chi = 32, D = 4 (double-layer bond 16): unit-stride runs and K
+    # sub-extents of 16 and 32 take the vector-load kernel where the layout allows
+    import tnad
+    print(f"case: vector-load contract {prec} {spec}", flush=True)
+    rng = np.random.default_rng(zlib.crc32(spec.encode()) + 1)
+    chi_idx = {"iba,ad->ibd": "iad", "ibd,dcl->ibcl": "idl", "ibcl,jkcb->ijlk": "il",
+               "aed,abx->edbx": "adx", "edbx,bjce->dcxj": "dx", "dcxj,dcy->xjy": "dxy",
+               "abx,xjy->abjy": "axy", "abjy,bjce->aecy": "ay", "aecy,dcy->aed": "ady",
+               "abjy,aecy->bjce": "ay", "ijlk,jkcb->ibcl": "il", "ijlk,ibcl->jkcb": "il",
+               "ibcl,dcl->ibd": "idl", "ibd,ibcl->dcl": "idl", "iba,ibd->ad": "iad"}[spec]
+    lhs, out = spec.split("->")
+    ia, ib = lhs.split(",")
+    ext = {ch: (32 if ch in chi_idx else 16) for ch in set(ia + ib + out)}
+    A = rng.standard_normal([ext[c] for c in ia])
+    B = rng.standard_normal([ext[c] for c in ib])
+    got = tnad.contract(spec, A, B, precision=prec, ctx=ctx)
+    ref = np.einsum(spec, A, B)
+    assert _rel(got, ref) < 10 * TOL[prec]
