@@ -282,24 +282,31 @@ __global__ __launch_bounds__(NT, PF == 2 ? (sizeof(T) == 8 ? 3 : 4) : 1) void tc
 }
 
 // ---------------------------------------------------------------------------
-// F64 / F32 kernel with 16-byte operand loads (tc_vec_ok decides when it applies): each operand is read
+// F64 / F32 kernel with 16-byte operand loads (launch_vec decides when it applies): each operand is read
 // in vectors of V = 16 / sizeof(T) elements along its unit-stride index (m for A or n for B when that
 // index has unit stride, else k), so a thread issues V times fewer loads and no per-element address
-// arithmetic; the K-tile never straddles a K0 run (K0 % 16 == 0 or a single K sub-index) and K is a
-// multiple of the K-tile, so the K loop carries no bounds tests.  64 x 64 output tile, 4 waves in
-// 2 x 2, K-tile 16, double-buffered LDS (one barrier per K-tile), the next K-tile's vectors in flight in
-// registers while the current one is multiplied, and every LDS fragment of a K-tile read before its
-// MFMAs.
+// arithmetic; the K-tile never straddles a K0 run (K0 a multiple of BK, or a single K sub-index) and K
+// is a multiple of BK, so the K loop carries no bounds tests.  TILE x TILE output tile (64, K-tile 16;
+// or 32, K-tile 32, for grids too small to fill the chip with 64 x 64 tiles), 4 waves in 2 x 2,
+// double-buffered LDS (one barrier per K-tile), the next K-tile's vectors in flight in registers while
+// the current one is multiplied.
 // ---------------------------------------------------------------------------
-template <typename T> struct Vec16;
-template <> struct Vec16<double> { typedef double2 type; static constexpr int V = 2; };
-template <> struct Vec16<float> { typedef float4 type; static constexpr int V = 4; };
+// 16-byte operand vectors as clang vector types (kept in registers; element e is a compile-time
+// constant after unrolling)
+typedef double d2 __attribute__((ext_vector_type(2)));
+template <typename VT> __device__ __forceinline__ auto vec_elem(const VT& v, int e) { return v[e]; }
 
-template <typename T, bool A_MV, bool B_NV>
+template <typename T> struct Vec16;
+template <> struct Vec16<double> { typedef d2 type; static constexpr int V = 2; };
+template <> struct Vec16<float> { typedef f4 type; static constexpr int V = 4; };
+
+template <typename T, int TILE, int BK, bool A_MV, bool B_NV>
 __global__ __launch_bounds__(NT, sizeof(T) == 8 ? 3 : 4) void tc_kernel_vec(TcDesc d, const T* __restrict__ A,
                                                                          const T* __restrict__ B,
                                                                          T* __restrict__ C, T* __restrict__ Wsp) {
-  constexpr int TILE = 64, BK = 16, PAD = 16 / sizeof(T) * 2;
+  // row pitch of the LDS tiles: 16-byte rows; FP32: the four k-rows a fragment read touches start 16
+  // banks apart, so its 64 lanes meet no bank twice
+  constexpr int PAD = sizeof(T) == 8 ? 4 : 16;
   typedef typename Vec16<T>::type VT;
   constexpr int V = Vec16<T>::V;
   constexpr int NVEC = TILE * BK / (NT * V);   // vectors per thread and operand per K-tile
@@ -342,14 +349,16 @@ __global__ __launch_bounds__(NT, sizeof(T) == 8 ? 3 : 4) void tc_kernel_vec(TcDe
     a_kp[r].init(kbeg + a_k[r], d.K0, d.ak0, d.ak1);
     b_kp[r].init(kbeg + b_k[r], d.K0, d.bk0, d.bk1);
   }
-  auto gload = [&](VT (&xa)[NVEC], VT (&xb)[NVEC]) {
+  // loads of one K-tile (live: a K-tile inside [kbeg, K)).  A slot outside the output tile, or any slot of a
+  // dead tile, reads the operand's first vector instead, so the loop body stays free of branches: such
+  // values reach only accumulator rows (columns) past M (N), which are never stored, or the LDS buffer
+  // the last iteration fills and nobody reads.
+  auto gload = [&](VT (&xa)[NVEC], VT (&xb)[NVEC], bool live) {
 #pragma unroll
     for (int r = 0; r < NVEC; ++r) {
-      VT va = {}, vb = {};
-      if (a_ok[r]) va = *reinterpret_cast<const VT*>(A + a_off[r] + a_kp[r].off);
-      if (b_ok[r]) vb = *reinterpret_cast<const VT*>(B + b_off[r] + b_kp[r].off);
-      xa[r] = va;
-      xb[r] = vb;
+      const bool oka = live && a_ok[r], okb = live && b_ok[r];
+      xa[r] = *reinterpret_cast<const VT*>(A + (oka ? a_off[r] + a_kp[r].off : 0));
+      xb[r] = *reinterpret_cast<const VT*>(B + (okb ? b_off[r] + b_kp[r].off : 0));
       // one K0 run holds whole K-tiles: at most one carry per tile
       a_kp[r].k0 += BK;
       a_kp[r].off += (long long)BK * d.ak0;
@@ -362,17 +371,15 @@ __global__ __launch_bounds__(NT, sizeof(T) == 8 ? 3 : 4) void tc_kernel_vec(TcDe
   auto stage = [&](const VT (&xa)[NVEC], const VT (&xb)[NVEC], int buf) {
 #pragma unroll
     for (int r = 0; r < NVEC; ++r) {
-      const T* ea = reinterpret_cast<const T*>(&xa[r]);
-      const T* eb = reinterpret_cast<const T*>(&xb[r]);
       if (A_MV) *reinterpret_cast<VT*>(&As[buf][a_k[r]][a_m[r]]) = xa[r];
       else {
 #pragma unroll
-        for (int e = 0; e < V; ++e) As[buf][a_k[r] + e][a_m[r]] = ea[e];
+        for (int e = 0; e < V; ++e) As[buf][a_k[r] + e][a_m[r]] = vec_elem(xa[r], e);
       }
       if (B_NV) *reinterpret_cast<VT*>(&Bs[buf][b_k[r]][b_n[r]]) = xb[r];
       else {
 #pragma unroll
-        for (int e = 0; e < V; ++e) Bs[buf][b_k[r] + e][b_n[r]] = eb[e];
+        for (int e = 0; e < V; ++e) Bs[buf][b_k[r] + e][b_n[r]] = vec_elem(xb[r], e);
       }
     }
   };
@@ -383,7 +390,17 @@ __global__ __launch_bounds__(NT, sizeof(T) == 8 ? 3 : 4) void tc_kernel_vec(TcDe
   for (int i = 0; i < FB; ++i)
 #pragma unroll
     for (int j = 0; j < FB; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
-  auto mfma_tile = [&](int buf) {
+  // One K-tile per iteration: its fragments from LDS buffer `buf`, the first half of its MFMAs, the next
+  // K-tile (in registers since the previous iteration) stored into the other buffer, the second half
+  // of the MFMAs, the loads of the K-tile after it, one barrier.  The store of the last iteration goes
+  // to the buffer nobody reads again, and its loads are dead (they read the operands' first vector).
+  VT ra[NVEC], rb[NVEC];
+  gload(ra, rb, true);
+  stage(ra, rb, 0);
+  gload(ra, rb, kbeg + BK < K);
+  __syncthreads();
+  int buf = 0;
+  for (int k0 = kbeg; k0 < K; k0 += BK, buf ^= 1) {
     T af[BK / 4][FB], bf[BK / 4][FB];
 #pragma unroll
     for (int kk = 0; kk < BK / 4; ++kk) {
@@ -393,25 +410,22 @@ __global__ __launch_bounds__(NT, sizeof(T) == 8 ? 3 : 4) void tc_kernel_vec(TcDe
       for (int j = 0; j < FB; ++j) bf[kk][j] = Bs[buf][4 * kk + (lane >> 4)][wn * (TILE / 2) + j * 16 + (lane & 15)];
     }
 #pragma unroll
-    for (int kk = 0; kk < BK / 4; ++kk)
+    for (int kk = 0; kk < BK / 8; ++kk)
 #pragma unroll
       for (int i = 0; i < FB; ++i)
 #pragma unroll
         for (int j = 0; j < FB; ++j) acc[i][j] = MfmaOp<T>::run(af[kk][i], bf[kk][j], acc[i][j]);
-  };
-
-  VT ra[NVEC], rb[NVEC];
-  gload(ra, rb);
-  stage(ra, rb, 0);
-  if (kbeg + BK < K) gload(ra, rb);
-  __syncthreads();
-  int buf = 0;
-  for (int k0 = kbeg; k0 < K; k0 += BK, buf ^= 1) {
-    mfma_tile(buf);
-    if (k0 + BK < K) {
-      stage(ra, rb, buf ^ 1);
-      if (k0 + 2 * BK < K) gload(ra, rb);
-    }
+    stage(ra, rb, buf ^ 1);
+    // the next loads reuse the staged registers: keep them after the stores (hoisted above, they would
+    // need a second register set and a copy that waits for them within the same K-tile)
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int kk = BK / 8; kk < BK / 4; ++kk)
+#pragma unroll
+      for (int i = 0; i < FB; ++i)
+#pragma unroll
+        for (int j = 0; j < FB; ++j) acc[i][j] = MfmaOp<T>::run(af[kk][i], bf[kk][j], acc[i][j]);
+    gload(ra, rb, k0 + 2 * BK < K);
     __syncthreads();
   }
 
@@ -630,12 +644,12 @@ static int vec_mode(int X0, int X1, long long x0, long long x1, int K0, int K1, 
   return 0;
 }
 
-template <typename T>
+template <typename T, int TILE, int BK>
 static bool launch_vec(const TcDesc& dd, dim3 grid, const void* A, const void* B, void* C, void* W, hipStream_t s) {
   constexpr int V = 16 / sizeof(T);
   const long long K = (long long)dd.K0 * dd.K1;
-  // whole K-tiles, each inside one K0 run
-  if (K % 16 != 0 || (dd.K1 > 1 && dd.K0 % 16 != 0)) return false;
+  // whole K-tiles, each inside one K0 run (split-K slices are multiples of 32)
+  if (K % BK != 0 || (dd.K1 > 1 && dd.K0 % BK != 0)) return false;
   const int ma = vec_mode(dd.M0, dd.M1, dd.am0, dd.am1, dd.K0, dd.K1, dd.ak0, dd.ak1, dd.batch, dd.abat, A, V);
   const int mb = vec_mode(dd.N0, dd.N1, dd.bn0, dd.bn1, dd.K0, dd.K1, dd.bk0, dd.bk1, dd.batch, dd.bbat, B, V);
   if (ma == 0 || mb == 0) return false;
@@ -643,10 +657,10 @@ static bool launch_vec(const TcDesc& dd, dim3 grid, const void* A, const void* B
   const T* b = (const T*)B;
   T* c = (T*)C;
   T* w = (T*)W;
-  if (ma == 1 && mb == 1) hipLaunchKernelGGL((tc_kernel_vec<T, true, true>), grid, dim3(NT), 0, s, dd, a, b, c, w);
-  else if (ma == 1) hipLaunchKernelGGL((tc_kernel_vec<T, true, false>), grid, dim3(NT), 0, s, dd, a, b, c, w);
-  else if (mb == 1) hipLaunchKernelGGL((tc_kernel_vec<T, false, true>), grid, dim3(NT), 0, s, dd, a, b, c, w);
-  else hipLaunchKernelGGL((tc_kernel_vec<T, false, false>), grid, dim3(NT), 0, s, dd, a, b, c, w);
+  if (ma == 1 && mb == 1) hipLaunchKernelGGL((tc_kernel_vec<T, TILE, BK, true, true>), grid, dim3(NT), 0, s, dd, a, b, c, w);
+  else if (ma == 1) hipLaunchKernelGGL((tc_kernel_vec<T, TILE, BK, true, false>), grid, dim3(NT), 0, s, dd, a, b, c, w);
+  else if (mb == 1) hipLaunchKernelGGL((tc_kernel_vec<T, TILE, BK, false, true>), grid, dim3(NT), 0, s, dd, a, b, c, w);
+  else hipLaunchKernelGGL((tc_kernel_vec<T, TILE, BK, false, false>), grid, dim3(NT), 0, s, dd, a, b, c, w);
   return true;
 }
 
@@ -699,10 +713,14 @@ void tc_launch(Ctx* c, Prec p, const TcDesc& d, const void* A, const void* B, vo
   const dim3 grid_big((unsigned)((M + 127) / 128), (unsigned)((N + 127) / 128), grid.z);
   const bool big = !small && nsplit == 1 && (long long)grid_big.x * grid_big.y * grid_big.z >= TC_BIG_MIN_WG;
   const bool deep = !big && !small && (long long)grid.x * grid.y * grid.z < TC_DEEP_BELOW;
-  // 64 x 64 tiles: the 16-byte-load kernel where the operand layouts allow it
-  const bool vec = !big && !small && !tc_vec_disabled() &&
-                   (p == Prec::F64 ? launch_vec<double>(dd, grid, A, B, C, W, s)
-                                   : p == Prec::F32 ? launch_vec<float>(dd, grid, A, B, C, W, s) : false);
+  // 64 x 64 and 32 x 32 tiles: the 16-byte-load kernel where the operand layouts allow it
+  bool vec = false;
+  if (!big && !tc_vec_disabled()) {
+    if (p == Prec::F64)
+      vec = small ? launch_vec<double, 32, 32>(dd, grid, A, B, C, W, s) : launch_vec<double, 64, 16>(dd, grid, A, B, C, W, s);
+    else if (p == Prec::F32)
+      vec = small ? launch_vec<float, 32, 32>(dd, grid, A, B, C, W, s) : launch_vec<float, 64, 16>(dd, grid, A, B, C, W, s);
+  }
   if (!vec) switch (p) {
     case Prec::F64:
       if (small)
