@@ -36,9 +36,14 @@ def sources():
     return sorted([p for p in CSRC.iterdir() if p.suffix in (".hip", ".cpp")])
 
 
+# per-source flags: the contraction kernels keep MFMA accumulators in ordinary VGPRs (the AGPR form
+# copies every accumulator across each K-loop iteration in the 128 x 128 FP64 kernel)
+EXTRA = {"tc.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"]}
+
+
 def compile_one(src: Path) -> Path:
     obj = BUILD / (src.stem + ".o")
-    cmd = [HIPCC, *COMMON, "-c", str(src), "-o", str(obj)]
+    cmd = [HIPCC, *COMMON, *EXTRA.get(src.name, []), "-c", str(src), "-o", str(obj)]
     if src.suffix == ".cpp":
         cmd = [HIPCC, *COMMON, "-x", "hip", "-c", str(src), "-o", str(obj)]
     r = subprocess.run(cmd, capture_output=True, text=True)

@@ -301,7 +301,7 @@ template <> struct Vec16<double> { typedef d2 type; static constexpr int V = 2; 
 template <> struct Vec16<float> { typedef f4 type; static constexpr int V = 4; };
 
 template <typename T, int TILE, int BK, bool A_MV, bool B_NV>
-__global__ __launch_bounds__(NT, sizeof(T) == 8 ? 3 : 4) void tc_kernel_vec(TcDesc d, const T* __restrict__ A,
+__global__ __launch_bounds__(NT, TILE == 128 ? 1 : (sizeof(T) == 8 ? 3 : 4)) void tc_kernel_vec(TcDesc d, const T* __restrict__ A,
                                                                          const T* __restrict__ B,
                                                                          T* __restrict__ C, T* __restrict__ Wsp) {
   // row pitch of the LDS tiles: 16-byte rows; FP32: the four k-rows a fragment read touches start 16
@@ -401,30 +401,26 @@ __global__ __launch_bounds__(NT, sizeof(T) == 8 ? 3 : 4) void tc_kernel_vec(TcDe
   __syncthreads();
   int buf = 0;
   for (int k0 = kbeg; k0 < K; k0 += BK, buf ^= 1) {
-    T af[BK / 4][FB], bf[BK / 4][FB];
+    // one k-step (4 of K) of MFMAs from the fragments of LDS buffer `buf`
+    auto kstep = [&](int kk) {
+      T af[FB], bf[FB];
 #pragma unroll
-    for (int kk = 0; kk < BK / 4; ++kk) {
+      for (int i = 0; i < FB; ++i) af[i] = As[buf][4 * kk + (lane >> 4)][wm * (TILE / 2) + i * 16 + (lane & 15)];
 #pragma unroll
-      for (int i = 0; i < FB; ++i) af[kk][i] = As[buf][4 * kk + (lane >> 4)][wm * (TILE / 2) + i * 16 + (lane & 15)];
-#pragma unroll
-      for (int j = 0; j < FB; ++j) bf[kk][j] = Bs[buf][4 * kk + (lane >> 4)][wn * (TILE / 2) + j * 16 + (lane & 15)];
-    }
-#pragma unroll
-    for (int kk = 0; kk < BK / 8; ++kk)
+      for (int j = 0; j < FB; ++j) bf[j] = Bs[buf][4 * kk + (lane >> 4)][wn * (TILE / 2) + j * 16 + (lane & 15)];
 #pragma unroll
       for (int i = 0; i < FB; ++i)
 #pragma unroll
-        for (int j = 0; j < FB; ++j) acc[i][j] = MfmaOp<T>::run(af[kk][i], bf[kk][j], acc[i][j]);
+        for (int j = 0; j < FB; ++j) acc[i][j] = MfmaOp<T>::run(af[i], bf[j], acc[i][j]);
+    };
+#pragma unroll
+    for (int kk = 0; kk < BK / 8; ++kk) kstep(kk);
     stage(ra, rb, buf ^ 1);
     // the next loads reuse the staged registers: keep them after the stores (hoisted above, they would
     // need a second register set and a copy that waits for them within the same K-tile)
     __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-    for (int kk = BK / 8; kk < BK / 4; ++kk)
-#pragma unroll
-      for (int i = 0; i < FB; ++i)
-#pragma unroll
-        for (int j = 0; j < FB; ++j) acc[i][j] = MfmaOp<T>::run(af[kk][i], bf[kk][j], acc[i][j]);
+    for (int kk = BK / 8; kk < BK / 4; ++kk) kstep(kk);
     gload(ra, rb, k0 + 2 * BK < K);
     __syncthreads();
   }
@@ -713,13 +709,21 @@ void tc_launch(Ctx* c, Prec p, const TcDesc& d, const void* A, const void* B, vo
   const dim3 grid_big((unsigned)((M + 127) / 128), (unsigned)((N + 127) / 128), grid.z);
   const bool big = !small && nsplit == 1 && (long long)grid_big.x * grid_big.y * grid_big.z >= TC_BIG_MIN_WG;
   const bool deep = !big && !small && (long long)grid.x * grid.y * grid.z < TC_DEEP_BELOW;
-  // 64 x 64 and 32 x 32 tiles: the 16-byte-load kernel where the operand layouts allow it
+  // The 16-byte-load kernel where the operand layouts allow it: 128 x 128 tiles for large outputs, as
+  // above (a wave then owns a 64 x 64 block: four times the MFMA work per fragment read and per barrier
+  // of the 64 x 64 tile; 4096^3 FP64 35.5 -> 62.6 TFLOP/s, FP32 84.5 -> 124.3; at one workgroup per CU
+  // the 64 x 64 tile is still faster), 32 x 32 for small grids, 64 x 64 otherwise
   bool vec = false;
-  if (!big && !tc_vec_disabled()) {
+  const bool vbig = big;
+  if (!tc_vec_disabled()) {
     if (p == Prec::F64)
-      vec = small ? launch_vec<double, 32, 32>(dd, grid, A, B, C, W, s) : launch_vec<double, 64, 16>(dd, grid, A, B, C, W, s);
+      vec = vbig ? launch_vec<double, 128, 16>(dd, grid_big, A, B, C, W, s)
+                 : small ? launch_vec<double, 32, 32>(dd, grid, A, B, C, W, s)
+                         : !big && launch_vec<double, 64, 16>(dd, grid, A, B, C, W, s);
     else if (p == Prec::F32)
-      vec = small ? launch_vec<float, 32, 32>(dd, grid, A, B, C, W, s) : launch_vec<float, 64, 16>(dd, grid, A, B, C, W, s);
+      vec = vbig ? launch_vec<float, 128, 16>(dd, grid_big, A, B, C, W, s)
+                 : small ? launch_vec<float, 32, 32>(dd, grid, A, B, C, W, s)
+                         : !big && launch_vec<float, 64, 16>(dd, grid, A, B, C, W, s);
   }
   if (!vec) switch (p) {
     case Prec::F64:

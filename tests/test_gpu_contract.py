@@ -158,3 +158,34 @@ def test_step_contractions_vector_loads(ctx, prec, spec):
     got = tnad.contract(spec, A, B, precision=prec, ctx=ctx)
     ref = np.einsum(spec, A, B)
     assert _rel(got, ref) < 10 * TOL[prec]
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+@pytest.mark.parametrize("tr", ["NN", "NT", "TN", "TT"])
+def test_gemm_vector_loads_large_tiles(ctx, prec, tr):
+    # 4096 x 4096 outputs (1024 tiles of 128 x 128: the large-tile vector-load kernel)
+    import tnad
+    print(f"case: vector-load gemm 128-tiles {prec} {tr}", flush=True)
+    m, n, k = 4096, 4096, 48
+    rng = np.random.default_rng(11 + len(tr))
+    spec = {"NN": "ik,kj->ij", "NT": "ik,jk->ij", "TN": "ki,kj->ij", "TT": "ki,jk->ij"}[tr]
+    A = rng.standard_normal((m, k) if tr[0] == "N" else (k, m))
+    B = rng.standard_normal((k, n) if tr[1] == "N" else (n, k))
+    got = tnad.contract(spec, A, B, precision=prec, ctx=ctx)
+    ref = np.einsum(spec, A, B, optimize=True)
+    assert _rel(got, ref) < 10 * TOL[prec]
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_enlarged_corner_at_chi128(ctx, prec):
+    # the bench's enlarged corner, ein"ibcl,jkcb->ijlk" at chi = 128 and double-layer bond 16 (a 16384 x 256
+    # output over K = 256, unit-stride i runs of 128 and K0 = 16: the vector-load kernel)
+    import tnad
+    print(f"case: enlarged corner chi=128 {prec}", flush=True)
+    rng = np.random.default_rng(5)
+    chi, Dd = 128, 16
+    X2 = rng.standard_normal((chi, Dd, Dd, chi))
+    a = rng.standard_normal((Dd, Dd, Dd, Dd))
+    got = tnad.contract("ibcl,jkcb->ijlk", X2, a, precision=prec, ctx=ctx)
+    ref = np.einsum("ibcl,jkcb->ijlk", X2, a, optimize=True)
+    assert _rel(got, ref) < 10 * TOL[prec]
