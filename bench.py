@@ -41,8 +41,8 @@ PEAK = {"fp64": 78.6, "fp32": 157.3, "bf16": 2500.0}   # dense matrix TFLOP/s
 # rocprofv3 --pmc runs of `bench.py --kernel-only` (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch, mean of
 # 21 launches), each scaled by what the counter reads for a copy of known size in the same run: the 256 MiB
 # device copy reads FETCH_SIZE = 131089 KiB and WRITE_SIZE = 262144 KiB, so fetch x 2.0, write x 1.0
-# (the FP64->FP32 cast of 32 MiB gives the same fetch factor).  9349 KiB x 2 + 16384 KiB = 35081 KiB.
-TRAFFIC = 35081 * 1024
+# (the FP64->FP32 cast of 32 MiB gives the same fetch factor).  9317.3 KiB x 2 + 16384 KiB = 35019 KiB.
+TRAFFIC = 35019 * 1024
 
 
 def heisenberg_h():
@@ -116,7 +116,7 @@ def roofline_leg(tnad, ctx, chi, Dd):
     for prec in ("fp64", "fp32"):
         ms = tnad.time_contract(spec, ext, precision=prec, reps=20, ctx=ctx)
         roof[prec] = dict(ms=ms, tflops=kflops / (ms * 1e-3) / 1e12)
-    return {"bound": "mfma", "kernel": f"tc_kernel<float> ein\"{spec}\" (fp32, {kflops:.3e} flop/launch)",
+    return {"bound": "mfma", "kernel": f"tc_kernel_vec<float, 64, 16> ein\"{spec}\" (fp32, {kflops:.3e} flop/launch)",
             "achieved": round(roof["fp32"]["tflops"], 3), "peak": PEAK["fp32"], "unit": "TFLOP/s",
             "frac": round(roof["fp32"]["tflops"] / PEAK["fp32"], 4), "traffic": TRAFFIC,
             "avg_launch_ms": round(roof["fp32"]["ms"], 5),
