@@ -125,6 +125,26 @@ def roofline_leg(tnad, ctx, chi, Dd):
                              "avg_launch_ms": round(roof["fp64"]["ms"], 5)}}
 
 
+# LDS bandwidth of one CU: 128 bytes per clock at the 2.4 GHz peak engine clock
+LDS_PEAK_GBS = 128 * 2.4
+
+
+def eigensolver_leg(tnad, ctx, sweeps=3):
+    """The eigensolver's rotation kernel (one workgroup, the packed 192 x 192 FP64 Rayleigh-Ritz matrix in
+    LDS), timed alone with HIP events on the library stream: every round reads and writes the whole packed
+    matrix (4 planes of 96 * 97 / 2 2x2-block entries, 8 bytes each, both ways), which bounds it by the LDS
+    bandwidth of the one CU it runs on."""
+    m = 192
+    ms, rounds = tnad.time_jacobi(m, sweeps, reps=5, ctx=ctx)
+    p = m // 2
+    bytes_round = 2 * 4 * (p * (p + 1) // 2) * 8
+    gbs = bytes_round * rounds / (ms * 1e-3) / 1e9
+    return {"bound": "lds", "kernel": f"jacobi_rot_kernel (m={m}, {sweeps} sweeps = {rounds} rounds, one workgroup)",
+            "achieved": round(gbs, 1), "peak": LDS_PEAK_GBS, "unit": "GB/s (LDS of one CU)",
+            "frac": round(gbs / LDS_PEAK_GBS, 4), "bytes_per_round": bytes_round,
+            "avg_launch_ms": round(ms, 4), "us_per_round": round(ms * 1e3 / rounds, 3)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -157,6 +177,7 @@ def main():
     dev = torch.device("cuda:0")
     if a.kernel_only:
         roofline = roofline_leg(tnad, ctx, chi, Dd)
+        roofline["eigensolver"] = eigensolver_leg(tnad, ctx)
         # a plain copy of known size (256 MiB read + 256 MiB written) to scale FETCH_SIZE / WRITE_SIZE
         x = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device=dev)
         y = torch.empty_like(x)
@@ -204,6 +225,7 @@ def main():
                    iters_fp64=results["plain_fp64"]["iters"], iters_mixed=results["mixed"]["iters"])
 
     roofline = roofline_leg(tnad, ctx, chi, Dd)
+    eig_roof = eigensolver_leg(tnad, ctx)
 
     cpu = None
     if not a.no_cpu_baseline:
@@ -252,6 +274,9 @@ def main():
                                    "other": round(v["ms_step"] - v["stats"]["fwd_ms"] - v["stats"]["bwd_ms"], 3)}
                                for k, v in results.items()},
         "roofline": roofline,
+        # the dominant kernel by time is the eigensolver's one-workgroup Jacobi rotation kernel, bound by the
+        # LDS of the CU it runs on rather than by HBM or the matrix cores: its own roofline
+        "eigensolver_roofline": eig_roof,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

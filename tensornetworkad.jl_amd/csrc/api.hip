@@ -849,4 +849,21 @@ int tnad_time_contract(tnad_ctx* ctx, int precision, const char* ia, const long 
   });
 }
 
+int tnad_time_jacobi(tnad_ctx* ctx, int m, int sweeps, int reps, double* avg_ms, int* rounds) {
+  return guarded(ctx, [&] {
+    if (!avg_ms || m < 2 || m > JACOBI_MAX || sweeps < 1 || sweeps > 60 || reps < 1)
+      throw Error(TNAD_ERR_ARG, "time_jacobi: bad argument");
+    Ctx* c = &ctx->c;
+    DevBuf H(c, (size_t)m * m * 8), W(c, jacobi_work_doubles(m, sweeps) * 8), r(c, 16);
+    k_random_matrix(H.as<double>(), (size_t)m * m, 17u, c->stream);   // symmetrised by the kernel on load
+    *avg_ms = jacobi_rot_time(H.as<double>(), m, sweeps, reps, W.as<double>(), c->stream, r.as<int>());
+    if (rounds) {
+      int hr = 0;
+      c->read_async(&hr, r.ptr, sizeof(int));
+      c->sync();
+      *rounds = hr;
+    }
+  });
+}
+
 }  // extern "C"

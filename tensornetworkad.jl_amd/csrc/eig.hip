@@ -897,6 +897,42 @@ void jacobi_eig_ws(const double* H, int ldh, int m, double* evals, double* V, do
   TNAD_HIP_TRY(hipGetLastError());
 }
 
+float jacobi_rot_time(const double* H, int m, int max_sweeps, int reps, double* Vwork, hipStream_t s,
+                      int* rounds_dev) {
+  if (m < 2 || m > JACOBI_MAX || reps < 1) throw Error(TNAD_ERR_ARG, "jacobi_rot_time: bad sizes");
+  if (max_sweeps < 1 || max_sweeps > JR_MAX_SWEEPS) throw Error(TNAD_ERR_ARG, "jacobi_rot_time: sweep limit out of range");
+  const int mm = m + (m & 1);
+  const size_t P = mm / 2;
+  double* rot = Vwork;
+  double* diag = rot + (size_t)max_sweeps * (mm - 1) * P * 2 + (size_t)m * m;
+  const size_t lds = jacobi_lds(m);
+  const bool next = (long long)P * (P + 1) / 2 > (long long)(JB_MAX - 1) * JR_NT;
+  const void* kern = next ? (const void*)jacobi_rot_kernel<true> : (const void*)jacobi_rot_kernel<false>;
+  TNAD_HIP_TRY(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  auto launch = [&] {
+    if (next)
+      hipLaunchKernelGGL(jacobi_rot_kernel<true>, dim3(1), dim3(JR_NT), lds, s, H, m, m, rot, max_sweeps, diag,
+                         rounds_dev, (int*)nullptr, (const int*)nullptr);
+    else
+      hipLaunchKernelGGL(jacobi_rot_kernel<false>, dim3(1), dim3(JR_NT), lds, s, H, m, m, rot, max_sweeps, diag,
+                         rounds_dev, (int*)nullptr, (const int*)nullptr);
+    TNAD_HIP_TRY(hipGetLastError());
+  };
+  launch();   // warm-up
+  hipEvent_t e0, e1;
+  TNAD_HIP_TRY(hipEventCreate(&e0));
+  TNAD_HIP_TRY(hipEventCreate(&e1));
+  TNAD_HIP_TRY(hipEventRecord(e0, s));
+  for (int r = 0; r < reps; ++r) launch();
+  TNAD_HIP_TRY(hipEventRecord(e1, s));
+  TNAD_HIP_TRY(hipEventSynchronize(e1));
+  float ms = 0.f;
+  TNAD_HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return ms / reps;
+}
+
 void ritz_select(const double* H, int ldh, int m, int nh, double thresh, double* evals, double* V, int* flag,
                  hipStream_t s) {
   if (m < 1 || m > JACOBI_MAX || nh < 1 || nh > m) throw Error(TNAD_ERR_ARG, "ritz_select: size out of range");

@@ -90,6 +90,11 @@ inline void enforce(const Limit& l) {
 // it holds 0 at run time the kernels return at once and evals / V keep what they hold (a fallback
 // that the device decides, without a host round trip).
 size_t jacobi_work_doubles(int m, int max_sweeps);
+// Average device time (ms) of one rotation-kernel launch of the Jacobi solver on H (m x m, ld m; at most
+// max_sweeps sweeps) over reps launches after one warm-up, HIP events on stream s; *rounds_dev (device int)
+// receives the rounds each launch ran.  Vwork as for jacobi_eig_ws.  For roofline measurement.
+float jacobi_rot_time(const double* H, int m, int max_sweeps, int reps, double* Vwork, hipStream_t s,
+                      int* rounds_dev);
 void jacobi_eig_ws(const double* H, int ldh, int m, double* evals, double* V, double* Vwork,
                    int max_sweeps, int* sweeps_out, hipStream_t s, const int* gate = nullptr);
 

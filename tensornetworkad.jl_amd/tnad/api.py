@@ -562,6 +562,15 @@ def time_contract(spec: str, extents: dict, precision="fp64", reps: int = 20,
     return ms.value
 
 
+def time_jacobi(m: int = 192, sweeps: int = 3, reps: int = 5, ctx: Context | None = None):
+    """(average device ms, rounds) of one launch of the Jacobi rotation kernel on a random symmetric m x m
+    matrix, `sweeps` sweeps (HIP events on the context's stream)."""
+    ctx = ctx or default_context()
+    ms, rounds = C.c_double(0.0), C.c_int(0)
+    ctx.check(ctx.lib.tnad_time_jacobi(ctx.h, int(m), int(sweeps), int(reps), C.byref(ms), C.byref(rounds)))
+    return ms.value, rounds.value
+
+
 # ---------------------------------------------------------------------------
 # TRG (src/trg.jl)
 # ---------------------------------------------------------------------------

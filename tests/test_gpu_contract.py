@@ -189,3 +189,14 @@ def test_enlarged_corner_at_chi128(ctx, prec):
     got = tnad.contract("ibcl,jkcb->ijlk", X2, a, precision=prec, ctx=ctx)
     ref = np.einsum("ibcl,jkcb->ijlk", X2, a, optimize=True)
     assert _rel(got, ref) < 10 * TOL[prec]
+
+
+def test_time_jacobi_runs_the_requested_sweeps(ctx):
+    # the eigensolver roofline's timing entry: a random 192 x 192 matrix does not converge in 3 sweeps, so the
+    # rotation kernel runs 3 x 191 rounds; an odd size is padded to even (101 -> 102, 101 rounds per sweep)
+    import tnad
+    print("case: time_jacobi", flush=True)
+    ms, rounds = tnad.time_jacobi(192, 3, reps=2, ctx=ctx)
+    assert rounds == 3 * 191 and 0.0 < ms < 100.0
+    ms, rounds = tnad.time_jacobi(101, 2, reps=1, ctx=ctx)
+    assert rounds == 2 * 101 and ms > 0.0
