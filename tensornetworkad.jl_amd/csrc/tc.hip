@@ -610,6 +610,152 @@ __global__ __launch_bounds__(NT) void tc_kernel_bf16(TcDesc d, const float* __re
   }
 }
 
+// ---------------------------------------------------------------------------
+// BF16 kernel on gfx950's double-K matrix instruction (v_mfma_f32_16x16x32_bf16: eight consecutive k per
+// lane and operand, FP32 accumulation), for the layouts the 16-byte-load kernel accepts: FP32 operands
+// read as float4 vectors (launch_vec's rules, K-tile 32), rounded to BF16 (nearest even) while they are
+// stored into m-major (n-major) LDS tiles, so that each lane fetches its eight k values of a fragment with
+// one 16-byte read.  64 x 64 output tile, 4 waves in 2 x 2, double-buffered LDS, one barrier per K-tile,
+// branch-free loads as in tc_kernel_vec.
+// ---------------------------------------------------------------------------
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+
+template <bool A_MV, bool B_NV>
+__global__ __launch_bounds__(NT, 4) void tc_kernel_vec_bf16(TcDesc d, const float* __restrict__ A,
+                                                            const float* __restrict__ B, float* __restrict__ C,
+                                                            float* __restrict__ Wsp) {
+  constexpr int TILE = 64, BK = 32, V = 4, KP = BK + 8;   // KP: row pitch in bf16 (80 bytes)
+  constexpr int NVEC = TILE * BK / (NT * V);               // float4 vectors per thread and operand
+  constexpr int FB = TILE / 32;
+  static_assert(NVEC == 2, "loader coverage");
+  __shared__ __attribute__((aligned(16))) unsigned short As[2][TILE][KP];
+  __shared__ __attribute__((aligned(16))) unsigned short Bs[2][TILE][KP];
+
+  const int M = d.M0 * d.M1, N = d.N0 * d.N1;
+  const int tile_m = blockIdx.x * TILE, tile_n = blockIdx.y * TILE;
+  const int bz = Wsp ? 0 : blockIdx.z;
+  const int kbeg = Wsp ? blockIdx.z * d.kchunk : 0;
+  const int K = Wsp ? min(d.K0 * d.K1, kbeg + d.kchunk) : d.K0 * d.K1;
+  A += (long long)bz * d.abat;
+  B += (long long)bz * d.bbat;
+  C += (long long)bz * d.cbat;
+
+  const int t = threadIdx.x;
+  const int lane = t & 63, wave = t >> 6;
+  const int wm = wave & 1, wn = wave >> 1;
+
+  int a_m[NVEC], a_k[NVEC], b_n[NVEC], b_k[NVEC];
+  long long a_off[NVEC], b_off[NVEC];
+  bool a_ok[NVEC], b_ok[NVEC];
+  KPos a_kp[NVEC], b_kp[NVEC];
+  const long long a_dk = d.ak1 - (long long)d.K0 * d.ak0, b_dk = d.bk1 - (long long)d.K0 * d.bk0;
+#pragma unroll
+  for (int r = 0; r < NVEC; ++r) {
+    if (A_MV) { a_m[r] = (t % (TILE / V)) * V; a_k[r] = t / (TILE / V) + r * (NT * V / TILE); }
+    else      { a_k[r] = (t % (BK / V)) * V;   a_m[r] = t / (BK / V) + r * (NT * V / BK); }
+    if (B_NV) { b_n[r] = (t % (TILE / V)) * V; b_k[r] = t / (TILE / V) + r * (NT * V / TILE); }
+    else      { b_k[r] = (t % (BK / V)) * V;   b_n[r] = t / (BK / V) + r * (NT * V / BK); }
+    const int m = tile_m + a_m[r], n = tile_n + b_n[r];
+    a_ok[r] = m < M;
+    b_ok[r] = n < N;
+    a_off[r] = a_ok[r] ? off2(m, d.M0, d.am0, d.am1) : 0;
+    b_off[r] = b_ok[r] ? off2(n, d.N0, d.bn0, d.bn1) : 0;
+    a_kp[r].init(kbeg + a_k[r], d.K0, d.ak0, d.ak1);
+    b_kp[r].init(kbeg + b_k[r], d.K0, d.bk0, d.bk1);
+  }
+  auto gload = [&](f4 (&xa)[NVEC], f4 (&xb)[NVEC], bool live) {
+#pragma unroll
+    for (int r = 0; r < NVEC; ++r) {
+      const bool oka = live && a_ok[r], okb = live && b_ok[r];
+      xa[r] = *reinterpret_cast<const f4*>(A + (oka ? a_off[r] + a_kp[r].off : 0));
+      xb[r] = *reinterpret_cast<const f4*>(B + (okb ? b_off[r] + b_kp[r].off : 0));
+      a_kp[r].k0 += BK;
+      a_kp[r].off += (long long)BK * d.ak0;
+      if (a_kp[r].k0 >= d.K0) { a_kp[r].k0 -= d.K0; a_kp[r].off += a_dk; }
+      b_kp[r].k0 += BK;
+      b_kp[r].off += (long long)BK * d.bk0;
+      if (b_kp[r].k0 >= d.K0) { b_kp[r].k0 -= d.K0; b_kp[r].off += b_dk; }
+    }
+  };
+  // a vector along k becomes one 8-byte store of four BF16 values; a vector along m (n) four 2-byte ones
+  auto put = [&](unsigned short (&T)[TILE][KP], int row, int k, const f4& v, bool along_rows) {
+    if (along_rows) {
+#pragma unroll
+      for (int e = 0; e < V; ++e) T[row + e][k] = f2bf(v[e]);
+    } else {
+      const unsigned int lo = (unsigned int)f2bf(v[0]) | ((unsigned int)f2bf(v[1]) << 16);
+      const unsigned int hi = (unsigned int)f2bf(v[2]) | ((unsigned int)f2bf(v[3]) << 16);
+      *reinterpret_cast<uint2*>(&T[row][k]) = make_uint2(lo, hi);
+    }
+  };
+  auto stage = [&](const f4 (&xa)[NVEC], const f4 (&xb)[NVEC], int buf) {
+#pragma unroll
+    for (int r = 0; r < NVEC; ++r) {
+      put(As[buf], a_m[r], a_k[r], xa[r], A_MV);
+      put(Bs[buf], b_n[r], b_k[r], xb[r], B_NV);
+    }
+  };
+
+  f4 acc[FB][FB];
+#pragma unroll
+  for (int i = 0; i < FB; ++i)
+#pragma unroll
+    for (int j = 0; j < FB; ++j) acc[i][j] = f4{0, 0, 0, 0};
+  // one k-step (32 of K): lane l supplies rows (columns) l % 16 of each 16-block, k = 8 (l / 16) .. + 7
+  auto kstep = [&](int buf) {
+    bf8 af[FB], bf[FB];
+#pragma unroll
+    for (int i = 0; i < FB; ++i)
+      af[i] = *reinterpret_cast<const bf8*>(&As[buf][wm * (TILE / 2) + i * 16 + (lane & 15)][8 * (lane >> 4)]);
+#pragma unroll
+    for (int j = 0; j < FB; ++j)
+      bf[j] = *reinterpret_cast<const bf8*>(&Bs[buf][wn * (TILE / 2) + j * 16 + (lane & 15)][8 * (lane >> 4)]);
+#pragma unroll
+    for (int i = 0; i < FB; ++i)
+#pragma unroll
+      for (int j = 0; j < FB; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i], bf[j], acc[i][j], 0, 0, 0);
+  };
+
+  f4 ra[NVEC], rb[NVEC];
+  gload(ra, rb, true);
+  stage(ra, rb, 0);
+  gload(ra, rb, kbeg + BK < K);
+  __syncthreads();
+  int buf = 0;
+  for (int k0 = kbeg; k0 < K; k0 += BK, buf ^= 1) {
+    kstep(buf);
+    stage(ra, rb, buf ^ 1);
+    __builtin_amdgcn_sched_barrier(0);
+    gload(ra, rb, k0 + 2 * BK < K);
+    __syncthreads();
+  }
+
+  const float alpha = (float)d.alpha, beta = (float)d.beta;
+  float* Wz = Wsp ? Wsp + (size_t)blockIdx.z * M * N : nullptr;
+#pragma unroll
+  for (int j = 0; j < FB; ++j) {
+    int n = tile_n + wn * (TILE / 2) + j * 16 + (lane & 15);
+    if (n >= N) continue;
+    long long noff = off2(n, d.N0, d.cn0, d.cn1);
+#pragma unroll
+    for (int i = 0; i < FB; ++i) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        int m = tile_m + wm * (TILE / 2) + i * 16 + 4 * (lane >> 4) + r;
+        if (m >= M) continue;
+        if (Wz) {
+          Wz[m + (size_t)M * n] = acc[i][j][r];
+          continue;
+        }
+        long long o = noff + off2(m, d.M0, d.cm0, d.cm1);
+        float v = alpha * acc[i][j][r];
+        if (d.beta != 0.0) v += beta * C[o];
+        C[o] = v;
+      }
+    }
+  }
+}
+
 }  // namespace
 
 static size_t extent_bytes(long long x0, long long s0, long long x1, long long s1, long long y0, long long t0,
@@ -638,6 +784,26 @@ static int vec_mode(int X0, int X1, long long x0, long long x1, int K0, int K1, 
   if (X0 > 1 && x0 == 1 && X0 % V == 0 && mult(x1, X1) && mult(k0, K0) && mult(k1, K1)) return 1;
   if (K0 > 1 && k0 == 1 && K0 % V == 0 && mult(k1, K1) && mult(x0, X0) && mult(x1, X1)) return 2;
   return 0;
+}
+
+// BF16 operands on the double-K instruction where launch_vec's FP32 rules (K-tile 32) hold
+static bool launch_vec_bf16(const TcDesc& dd, dim3 grid64, const void* A, const void* B, void* C, void* W,
+                            hipStream_t s) {
+  constexpr int V = 4;
+  const long long K = (long long)dd.K0 * dd.K1;
+  if (K % 32 != 0 || (dd.K1 > 1 && dd.K0 % 32 != 0)) return false;
+  const int ma = vec_mode(dd.M0, dd.M1, dd.am0, dd.am1, dd.K0, dd.K1, dd.ak0, dd.ak1, dd.batch, dd.abat, A, V);
+  const int mb = vec_mode(dd.N0, dd.N1, dd.bn0, dd.bn1, dd.K0, dd.K1, dd.bk0, dd.bk1, dd.batch, dd.bbat, B, V);
+  if (ma == 0 || mb == 0) return false;
+  const float* a = (const float*)A;
+  const float* b = (const float*)B;
+  float* c = (float*)C;
+  float* w = (float*)W;
+  if (ma == 1 && mb == 1) hipLaunchKernelGGL((tc_kernel_vec_bf16<true, true>), grid64, dim3(NT), 0, s, dd, a, b, c, w);
+  else if (ma == 1) hipLaunchKernelGGL((tc_kernel_vec_bf16<true, false>), grid64, dim3(NT), 0, s, dd, a, b, c, w);
+  else if (mb == 1) hipLaunchKernelGGL((tc_kernel_vec_bf16<false, true>), grid64, dim3(NT), 0, s, dd, a, b, c, w);
+  else hipLaunchKernelGGL((tc_kernel_vec_bf16<false, false>), grid64, dim3(NT), 0, s, dd, a, b, c, w);
+  return true;
 }
 
 template <typename T, int TILE, int BK>
@@ -724,6 +890,8 @@ void tc_launch(Ctx* c, Prec p, const TcDesc& d, const void* A, const void* B, vo
       vec = vbig ? launch_vec<float, 128, 16>(dd, grid_big, A, B, C, W, s)
                  : small ? launch_vec<float, 32, 32>(dd, grid, A, B, C, W, s)
                          : !big && launch_vec<float, 64, 16>(dd, grid, A, B, C, W, s);
+    else
+      vec = launch_vec_bf16(dd, dim3((unsigned)((M + 63) / 64), (unsigned)((N + 63) / 64), grid.z), A, B, C, W, s);
   }
   if (!vec) switch (p) {
     case Prec::F64:

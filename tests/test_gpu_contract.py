@@ -116,7 +116,7 @@ def test_split_k_strided_bulk_cotangent(ctx, prec):
     assert _rel(got, ref) < 10 * TOL[prec]
 
 
-@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+@pytest.mark.parametrize("prec", ["fp64", "fp32", "bf16"])
 @pytest.mark.parametrize("tr", ["NN", "NT", "TN", "TT"])
 @pytest.mark.parametrize("m,n,k", [(256, 192, 512), (2048, 128, 2048), (136, 72, 48), (64, 64, 16),
                                    (2048, 192, 192), (192, 192, 2048), (96, 40, 64)])
@@ -124,7 +124,8 @@ def test_gemm_vector_loads(ctx, prec, tr, m, n, k):
     # sizes whose unit-stride index and K are multiples of the 16-byte vector and of the K-tile take the
     # vector-load kernel: 64 x 64 tiles (also with split-K at 2048 x 128 x 2048, and with partial output
     # tiles at 136 x 72), 32 x 32 tiles for small grids (2048 x 192 x 192; split-K at 192 x 192 x 2048;
-    # partial tiles at 96 x 40); alpha / beta applied to a strided output
+    # partial tiles at 96 x 40); BF16 on the double-K instruction wherever K is a multiple of 32;
+    # alpha / beta applied to a strided output
     import tnad
     print(f"case: vector-load gemm {prec} {tr} {m}x{n}x{k}", flush=True)
     rng = np.random.default_rng(m + 3 * n + 7 * k + len(tr))
