@@ -310,15 +310,15 @@ void run_ctmrg(tnad_ctx* ctx, tnad_tape& t, const double* C0, const double* T0, 
 
 // ---------------------------------------------------------------------------
 // Device scratch of the entry point that allocates nothing (tnad_symeig_noalloc): one file-scope
-// array per GPU, sized for the full-spectrum Jacobi at n = 192 with the sweep limit tnad_symeig
-// uses (rotation record 60 x 191 x 96 x 2 doubles, unsorted eigenvectors 192 x 192, diagonal,
-// round counter: 2,237,384 doubles, 17.9 MB).  Every call on a GPU shares that GPU's copy, and every
-// word a call reads there is written earlier in the same call.  Host code never passes `scratch`
-// itself to a kernel: its address on each device comes from hipGetSymbolAddress at tnad_create.
+// array per GPU, sized for tnad_symeig's full-spectrum solver (divide and conquer) at n = 192
+// (reflectors, two eigenvector buffers and the merge matrices, 4 x 192 x 192 doubles, plus the
+// tridiagonal matrix, tau and the eigenvalues: 148,232 doubles, 1.19 MB).  Every call on a GPU shares
+// that GPU's copy, and every word a call reads there is written earlier in the same call.  Host code
+// never passes `scratch` itself to a kernel: its address on each device comes from hipGetSymbolAddress
+// at tnad_create.
 // ---------------------------------------------------------------------------
 namespace {
-constexpr int kNoAllocSweeps = 60;   // the sweep limit of tnad_symeig's full-spectrum path (topk_eig)
-constexpr size_t kScratchDoubles = (size_t)kNoAllocSweeps * 191 * 96 * 2 + 192 * 192 + 192 + 8;
+constexpr size_t kScratchDoubles = 4 * 192 * 192 + 4 * 192 + 8;   // syev_dc_work_doubles(192)
 constexpr int kMaxDevices = 64;
 std::mutex g_scratch_mu;
 double* g_scratch[kMaxDevices] = {};   // device address of `scratch` on each device
@@ -473,10 +473,10 @@ int tnad_symeig_noalloc(tnad_ctx* ctx, const double* A, int n, double* evals, do
     require_device(c, evals, "evals");
     require_device(c, evecs, "evecs");
     double* ws = scratch_for(c->device);
-    if (jacobi_work_doubles(n, kNoAllocSweeps) > kScratchDoubles)
+    if (syev_dc_work_doubles(n) > kScratchDoubles)
       throw Error(TNAD_ERR_UNSUPPORTED, "tnad_symeig_noalloc: work exceeds the scratch array");
     // the kernels of tnad_symeig's full-spectrum path, on the scratch array instead of a pool block
-    jacobi_eig_ws(A, n, n, evals, evecs, ws, kNoAllocSweeps, nullptr, c->stream);
+    syev_dc(A, n, n, evals, evecs, ws, c->stream);
   });
 }
 

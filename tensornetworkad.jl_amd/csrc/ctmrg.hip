@@ -431,9 +431,10 @@ int topk_eig(Ctx* c, const double* A, int n, int want, int k, const double* X0, 
              double* X, double* AX, double* theta, EigStats* st) {
   hipStream_t s = c->stream;
   if (n <= JACOBI_MAX) {
-    // full spectrum: exact reference semantics (all n singular values / vectors)
-    DevBuf Vw(c, jacobi_work_doubles(n, 60) * 8);
-    jacobi_eig_ws(A, n, n, theta, X, Vw.as<double>(), 60, nullptr, s);
+    // full spectrum: exact reference semantics (all n singular values / vectors), by divide and conquer
+    // (2-3.5x faster than Jacobi sweeps to convergence at n = 40..192)
+    DevBuf Vw(c, syev_dc_work_doubles(n) * 8);
+    syev_dc(A, n, n, theta, X, Vw.as<double>(), s);
     tc_launch(c, Prec::F64, gemm_desc('N', 'N', n, n, n, n, n, n), A, X, AX, s);
     if (st) { st->iters = 1; st->k = n; st->resid = 0.0; }
     return n;
@@ -477,12 +478,15 @@ int topk_eig(Ctx* c, const double* A, int n, int want, int k, const double* X0, 
     // far from convergence the Rayleigh-Ritz step only has to improve the basis (the span, which
     // is what converges, does not depend on it): two Jacobi sweeps do.  Closer in, three: Jacobi
     // converges quadratically on a nearly diagonal H, and what is left is taken up by the next
-    // iteration's Rayleigh-Ritz (the residual test sees it).  The sweeps run to completion when
-    // the residual stopped falling or the iteration runs long.
+    // iteration's Rayleigh-Ritz (the residual test sees it).  When the residual stopped falling or the
+    // iteration runs long, the Rayleigh-Ritz problem is solved exactly, by divide and conquer (cheaper
+    // than Jacobi sweeps to convergence; two or three sweeps cost less than it).
     const bool rough = it == 1 || prev_rel > 1e-6;
-    const int sweeps = (force_jacobi || it > 8) ? 60 : (rough ? 2 : 3);
-    jacobi_eig_ws(w.H.as<double>(), k, k, theta, w.W.as<double>(), w.Vw.as<double>(), sweeps,
-                  dbg ? w.sweeps.as<int>() : nullptr, s, jac_forced ? nullptr : gate);
+    if (force_jacobi || it > 8)
+      syev_dc(w.H.as<double>(), k, k, theta, w.W.as<double>(), w.Vw.as<double>(), s, jac_forced ? nullptr : gate);
+    else
+      jacobi_eig_ws(w.H.as<double>(), k, k, theta, w.W.as<double>(), w.Vw.as<double>(), rough ? 2 : 3,
+                    dbg ? w.sweeps.as<int>() : nullptr, s, jac_forced ? nullptr : gate);
     dbg_finite(c, "W", w.W.as<double>(), (size_t)k * k);
     dbg_finite(c, "theta", theta, (size_t)k);
     tc_launch(c, Prec::F64, gemm_desc('N', 'N', n, k, k, n, k, n), X, w.W.as<double>(), w.Xr.as<double>(), s);

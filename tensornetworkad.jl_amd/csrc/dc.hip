@@ -1,0 +1,744 @@
+// Dense symmetric eigensolver for m <= 192 (FP64) in three stages, each a HIP kernel:
+//
+//  * trd_kernel : Householder reduction to tridiagonal form in one workgroup of 48 x 16 threads.  The
+//    matrix stays in registers in a cyclic distribution (thread (ty, tx) holds the 4 x 12 elements
+//    (ty + 48 a, tx + 16 b)), both triangles, so the product A v of each step is a per-thread partial
+//    sum over its twelve columns finished by DPP butterflies across the 16 lanes of the same ty (one
+//    row of a wavefront), and the symmetric rank-two update is local.  Per step: the owner
+//    lanes publish column j through LDS, every wavefront forms the reflector from it (norm by a
+//    wavefront reduction, no extra barrier), then p = tau A v through LDS, then the update; two
+//    barriers per step; column blocks that a step cannot touch are not emitted (template block index),
+//    finished row blocks are skipped.
+//    The reflectors (v with v[j+1] = 1, zeros above) and their tau are written for the back
+//    transformation.
+//  * dc_kernel : divide and conquer on the tridiagonal matrix T (Cuppen's method), one workgroup.  T is
+//    torn recursively into halves down to 1 x 1 leaves, T = diag(T1, T2) + |b| u u^T with
+//    u = e_last + sign(b) e_first, and the tree is merged bottom-up, all merges of a level at once:
+//    the children's eigenpairs give D + rho z z^T, z = Q^T u / sqrt 2, rho = 2 |b|; entries with
+//    rho |z_i| below 8 eps max(|D|, rho |z|) deflate, and so does one of two entries whose Givens
+//    rotation (zeroing one z) perturbs the matrix by less than that (Dongarra-Sorensen deflation);
+//    the remaining K roots of the secular equation 1 + rho sum z_i^2 / (d_i - lambda) = 0 are found
+//    in parallel, each as an offset tau from its nearer pole (so lambda - d_i is formed without
+//    cancellation) by the two-pole rational model of the secular function (Li's "middle way"),
+//    safeguarded by bisection on a bracket; z is then recomputed from the roots (Gu-Eisenstat, Lowner's
+//    formula), which makes the eigenvectors (zhat_i / (d_i - lambda_j))_i orthogonal to working
+//    accuracy; the merged eigenvectors are the children's (block-diagonal) eigenvector matrix times
+//    that K x K matrix (deflated columns are copied).  The eigenvector matrices of a level live in
+//    two m x m ping-pong buffers in global memory (L2-resident), a level's K x K matrices in a third.
+//  * trd_back_kernel : W = H_0 H_1 ... H_(m-3) Z, one wavefront per column of Z (no barriers).
+// The eigenpairs are then sorted by |lambda| descending with the sign convention of the Jacobi path.
+#include "tnad_internal.h"
+
+#include <cfloat>
+
+namespace tnad {
+namespace {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+constexpr int TD_GX = 16, TD_GY = 48;       // thread grid: 48 rows x 16 columns of threads (768)
+constexpr int TD_NT = TD_GX * TD_GY;
+constexpr int TD_RB = JACOBI_MAX / TD_GX;   // column elements per thread (12)
+constexpr int TD_RA = JACOBI_MAX / TD_GY;   // row elements per thread (4)
+static_assert(TD_RB * TD_GX == JACOBI_MAX && TD_RA * TD_GY == JACOBI_MAX, "tridiagonalisation tiling");
+
+// DPP lane move of a double (its two 32-bit halves); lanes outside row_mask keep x
+template <int CTRL, int ROWMASK = 0xF>
+__device__ __forceinline__ double dpp_d(double x) {
+  const long long b = __double_as_longlong(x);
+  const int lo = __builtin_amdgcn_update_dpp((int)b, (int)b, CTRL, ROWMASK, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp((int)(b >> 32), (int)(b >> 32), CTRL, ROWMASK, 0xF, false);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// sums over each row of 16 lanes of N values at once, stage by stage (the lane moves of different
+// values interleave); butterflies, so every lane of a row ends with the same bits
+template <int N>
+__device__ __forceinline__ void row16_sums(double (&x)[N]) {
+#pragma unroll
+  for (int a = 0; a < N; ++a) x[a] += dpp_d<0xB1>(x[a]);    // quad_perm (1, 0, 3, 2)
+#pragma unroll
+  for (int a = 0; a < N; ++a) x[a] += dpp_d<0x4E>(x[a]);    // quad_perm (2, 3, 0, 1)
+#pragma unroll
+  for (int a = 0; a < N; ++a) x[a] += dpp_d<0x141>(x[a]);   // row_half_mirror
+#pragma unroll
+  for (int a = 0; a < N; ++a) x[a] += dpp_d<0x140>(x[a]);   // row_mirror
+}
+
+// sum over the wavefront, the same value in every lane
+__device__ __forceinline__ double wave_sum(double x) {
+  double v[1] = {x};
+  row16_sums<1>(v);
+  x = v[0];
+  x += dpp_d<0x142, 0xA>(x);   // row_bcast15: row 1 += row 0, row 3 += row 2
+  x += dpp_d<0x143, 0x8>(x);   // row_bcast31: row 3 += lane 31; lane 63 holds the total
+  const long long b = __double_as_longlong(x);
+  const int lo = __builtin_amdgcn_readlane((int)b, 63), hi = __builtin_amdgcn_readlane((int)(b >> 32), 63);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+struct TrdShared {
+  double col[2][JACOBI_MAX];   // column j of the current matrix (double-buffered by step parity)
+  double p[2][JACOBI_MAX];     // tau A v
+};
+
+// Householder steps j = 16 BJ .. min(16 BJ + 15, m - 3).  Columns below 16 BJ are finished (they lie
+// left of every remaining step's column), so only column blocks b >= BJ are touched; row blocks whose
+// rows all lie at or above j are skipped by a uniform test.
+template <int BJ>
+__device__ __forceinline__ void trd_block(double (&A)[TD_RA][TD_RB], int m, TrdShared& sh, int tx, int ty,
+                                          int lane, int tid, double* __restrict__ Vh, double* __restrict__ tauo,
+                                          double* __restrict__ eo) {
+  const int j0 = BJ * TD_GX, j1 = min(m - 2, j0 + TD_GX);
+  for (int j = j0; j < j1; ++j) {
+    double* col = sh.col[j & 1];
+    double* pv = sh.p[j & 1];
+    if (tx == j - j0) {
+#pragma unroll
+      for (int a = 0; a < TD_RA; ++a) col[ty + TD_GY * a] = A[a][BJ];
+    }
+    __syncthreads();
+    // reflector H = I - tau v v^T with v[j+1] = 1 that maps A[j+1:, j] to (beta, 0, ..., 0)
+    double sg = 0.0;
+    for (int r = j + 2 + lane; r < m; r += 64) {
+      const double x = col[r];
+      sg += x * x;
+    }
+    sg = wave_sum(sg);
+    const double al = col[j + 1];
+    double be = al, tv = 0.0, sc = 0.0;
+    if (sg > 0.0) {
+      const double nr = sqrt(al * al + sg);
+      be = al >= 0.0 ? -nr : nr;
+      const double d = al - be;             // |d| >= |al|: no cancellation
+      const double inv = 1.0 / (be * d);    // tau = (beta - alpha) / beta = -d^2 inv, 1 / d = beta inv
+      tv = -d * d * inv;
+      sc = be * inv;
+    }
+    // v[r] for any r < 192 (every row of col is written each step, so the read needs no guard)
+    auto vat = [&](int r) -> double {
+      const double x = col[r] * sc;
+      return r == j + 1 ? 1.0 : ((r > j + 1 && r < m) ? x : 0.0);
+    };
+    double vc[TD_RB];
+#pragma unroll
+    for (int b = BJ; b < TD_RB; ++b) vc[b] = vat(tx + TD_GX * b);
+    // p = tau A v on the rows below j (finished row blocks are skipped and never stored)
+    double ps[TD_RA];
+#pragma unroll
+    for (int a = 0; a < TD_RA; ++a) {
+      ps[a] = 0.0;
+      if (TD_GY * a + TD_GY - 1 <= j) continue;   // uniform: every row of the block is finished
+#pragma unroll
+      for (int b = BJ; b < TD_RB; ++b) ps[a] = fma(A[a][b], vc[b], ps[a]);
+    }
+    row16_sums<TD_RA>(ps);
+#pragma unroll
+    for (int a = 0; a < TD_RA; ++a) {
+      if (TD_GY * a + TD_GY - 1 <= j) continue;
+      const int r = ty + TD_GY * a;
+      if (tx == 0) pv[r] = (r > j && r < m) ? tv * ps[a] : 0.0;
+    }
+    if (tid < 64) {
+      for (int r = lane; r < m; r += 64) Vh[r + (size_t)j * m] = vat(r);
+      if (lane == 0) {
+        tauo[j] = tv;
+        eo[j] = be;
+      }
+    }
+    __syncthreads();
+    // w = p - (tau / 2) (p . v) v ;  A -= v w^T + w v^T
+    double pw = 0.0;
+    for (int r = j + 1 + lane; r < m; r += 64) pw += pv[r] * vat(r);
+    pw = wave_sum(pw);
+    const double kc = 0.5 * tv * pw;
+    double wc[TD_RB];
+    // p is written this step only for rows in unskipped blocks: columns at or left of j take w = 0
+#pragma unroll
+    for (int b = BJ; b < TD_RB; ++b) {
+      const int c = tx + TD_GX * b;
+      const double x = pv[c] - kc * vc[b];
+      wc[b] = c > j ? x : 0.0;
+    }
+#pragma unroll
+    for (int a = 0; a < TD_RA; ++a) {
+      if (TD_GY * a + TD_GY - 1 <= j) continue;
+      const int r = ty + TD_GY * a;
+      const double vr = vat(r), wr = pv[r] - kc * vr;
+#pragma unroll
+      for (int b = BJ; b < TD_RB; ++b) A[a][b] = fma(-wr, vc[b], fma(-vr, wc[b], A[a][b]));
+    }
+  }
+}
+
+// H (m x m, ld ldh; the upper triangle is read) -> diagonal dout (m), off-diagonal eout (m - 1),
+// reflectors Vh (m x m, column j = v_j) and tauo (m - 2).
+__global__ __launch_bounds__(TD_NT) void trd_kernel(const double* __restrict__ H, int ldh, int m,
+                                                    double* __restrict__ dout, double* __restrict__ eout,
+                                                    double* __restrict__ Vh, double* __restrict__ tauo,
+                                                    const int* __restrict__ gate) {
+  if (gate && *gate == 0) return;
+  __shared__ TrdShared sh;
+  const int tid = threadIdx.x, ty = tid / TD_GX, tx = tid % TD_GX, lane = tid & 63;
+  double A[TD_RA][TD_RB];
+#pragma unroll
+  for (int a = 0; a < TD_RA; ++a)
+#pragma unroll
+    for (int b = 0; b < TD_RB; ++b) {
+      const int r = ty + TD_GY * a, c = tx + TD_GX * b;
+      const bool in = r < m && c < m;
+      const double x = H[in ? min(r, c) + (size_t)max(r, c) * ldh : 0];   // branch-free, clamped address
+      A[a][b] = in ? x : 0.0;
+    }
+  trd_block<0>(A, m, sh, tx, ty, lane, tid, Vh, tauo, eout);
+  trd_block<1>(A, m, sh, tx, ty, lane, tid, Vh, tauo, eout);
+  trd_block<2>(A, m, sh, tx, ty, lane, tid, Vh, tauo, eout);
+  trd_block<3>(A, m, sh, tx, ty, lane, tid, Vh, tauo, eout);
+  trd_block<4>(A, m, sh, tx, ty, lane, tid, Vh, tauo, eout);
+  trd_block<5>(A, m, sh, tx, ty, lane, tid, Vh, tauo, eout);
+  trd_block<6>(A, m, sh, tx, ty, lane, tid, Vh, tauo, eout);
+  trd_block<7>(A, m, sh, tx, ty, lane, tid, Vh, tauo, eout);
+  trd_block<8>(A, m, sh, tx, ty, lane, tid, Vh, tauo, eout);
+  trd_block<9>(A, m, sh, tx, ty, lane, tid, Vh, tauo, eout);
+  trd_block<10>(A, m, sh, tx, ty, lane, tid, Vh, tauo, eout);
+  trd_block<11>(A, m, sh, tx, ty, lane, tid, Vh, tauo, eout);
+#pragma unroll
+  for (int a = 0; a < TD_RA; ++a)
+#pragma unroll
+    for (int b = 0; b < TD_RB; ++b) {
+      const int r = ty + TD_GY * a, c = tx + TD_GX * b;
+      if (r == c && r < m) dout[r] = A[a][b];
+      if (r == m - 1 && c == m - 2) eout[m - 2] = A[a][b];
+    }
+}
+
+// ---------------------------------------------------------------------------------------------------
+// divide and conquer
+// ---------------------------------------------------------------------------------------------------
+constexpr int DC_NT = 1024;
+constexpr int DC_M = JACOBI_MAX;
+constexpr int DC_SEC = 256;   // threads from DC_SEC on work four lanes per root / per entry
+constexpr int DC_BIG = 32;    // levels whose widest merge reaches this use the matrix-core product
+
+struct DcShared {
+  double lam[DC_M];                 // eigenvalue of column g of the current eigenvector matrix
+  double ds[DC_M], zs[DC_M];        // merged node: d and z sorted ascending by d
+  int src[DC_M];                    // column of the sorted entry
+  int nlo[DC_M];                    // lo of the merge node of index g at this level (-1: none)
+  double dK[DC_M], zK[DC_M];        // kept entries (slots lo .. lo + K - 1)
+  double z2[DC_M];                  // rho zK^2
+  int kc[DC_M], ktype[DC_M];        // their columns; 0: top half only, 1: both halves, 2: bottom half only
+  double dvl[DC_M];                 // deflated eigenvalues (slots lo + K .. hi - 1)
+  int dcol[DC_M];                   // their columns
+  int org[DC_M];                    // pole (absolute slot) the root of slot lo + j is measured from
+  double tau[DC_M];                 // root = dK[org] + tau
+  double zh[DC_M], rn[DC_M];        // recomputed z; 1 / |u_j|
+  int upos[DC_M], kcp[DC_M];        // row of kept entry i in the K x K matrix (first its index within
+                                    // its type); column of row p
+  double rc[DC_M], rs[DC_M];        // deflation rotations of a node (slots lo ..)
+  int ra[DC_M], rb[DC_M];
+  int nK[DC_M], nT1[DC_M], nT12[DC_M], nRot[DC_M];   // per node at slot lo
+  double nRho[DC_M];
+  unsigned long long nmd[DC_M], nmz[DC_M];           // per node: bits of max |d|, max |z| (non-negative)
+};
+
+// node of index g at depth dd (recursive halving of [0, m), the lower half the smaller); false when it
+// is a leaf (a single index, then lo = g, hi = g + 1)
+__device__ __forceinline__ bool dc_node(int g, int dd, int m, int& lo, int& hi) {
+  lo = 0;
+  hi = m;
+  for (int l = 0; l < dd; ++l) {
+    if (hi - lo <= 1) return false;
+    const int mid = lo + ((hi - lo) >> 1);
+    if (g < mid) hi = mid;
+    else lo = mid;
+  }
+  return hi - lo >= 2;
+}
+
+__device__ __forceinline__ int dc_depth(int m) {
+  int d = 0;
+  while ((1 << d) < m) ++d;
+  return d;
+}
+
+__device__ __forceinline__ double sum4(double x) {
+  x += __shfl_xor(x, 1);
+  x += __shfl_xor(x, 2);
+  return x;
+}
+
+__device__ __forceinline__ double prod4(double x) {
+  x *= __shfl_xor(x, 1);
+  x *= __shfl_xor(x, 2);
+  return x;
+}
+
+// root j (0 <= j < K) of 1 + rho sum_i zk_i^2 / (dk_i - lambda) = 0, dk ascending, four lanes (q = 0..3)
+// sharing the sums.  Returns the pole o (index into dk) and lambda - dk[o].
+// 1 / x from the hardware estimate and two Newton steps (x finite and nonzero)
+__device__ __forceinline__ double rcp_nr(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  r = fma(r, fma(-x, r, 1.0), r);
+  return fma(r, fma(-x, r, 1.0), r);
+}
+
+// root e in (le, he) of c (djl - e)(djr - e) + s1 (djr - e) + s2 (djl - e) = 0 (the secular function
+// modelled by its two nearest poles and a constant); false when none lies there
+__device__ __forceinline__ bool dc_quad(double c, double djl, double djr, double s1, double s2, double le, double he,
+                                        double& e) {
+  const double qa = c, qb = -(c * (djl + djr) + s1 + s2), qc = c * djl * djr + s1 * djr + s2 * djl;
+  if (qa == 0.0) {
+    if (qb == 0.0) return false;
+    e = -qc / qb;
+    return e > le && e < he;
+  }
+  const double sq = sqrt(fmax(qb * qb - 4.0 * qa * qc, 0.0));
+  const double qq = -0.5 * (qb + copysign(sq, qb));
+  e = qq / qa;
+  if (e > le && e < he) return true;
+  if (qq == 0.0) return false;
+  e = qc / qq;
+  return e > le && e < he;
+}
+
+__device__ void dc_secular(const double* dk, const double* z2, int K, int j, int q, int& o, double& t) {
+  double lo_t, hi_t;
+  if (j < K - 1) {
+    const double mdp = 0.5 * (dk[j] + dk[j + 1]);
+    double f = 0.0;
+    for (int i = q; i < K; i += 4) f += z2[i] * rcp_nr(dk[i] - mdp);
+    f = 1.0 + sum4(f);
+    o = f > 0.0 ? j : j + 1;
+    const double gap = dk[j + 1] - dk[j];
+    if (o == j) { lo_t = 0.0; hi_t = 0.5 * gap; }
+    else { lo_t = -0.5 * gap; hi_t = 0.0; }
+    // first iterate: the two nearest poles exactly, the rest of the sum frozen at the midpoint
+    const double djl = dk[j] - dk[o], djr = dk[j + 1] - dk[o];
+    const double cc = f - z2[j] / (dk[j] - mdp) - z2[j + 1] / (dk[j + 1] - mdp);
+    double e;
+    t = dc_quad(cc, djl, djr, z2[j], z2[j + 1], lo_t, hi_t, e) ? e : 0.5 * (lo_t + hi_t);
+  } else {
+    double zz = 0.0;
+    for (int i = q; i < K; i += 4) zz += z2[i];
+    o = j;
+    lo_t = 0.0;
+    hi_t = sum4(zz);
+    t = 0.5 * (lo_t + hi_t);
+  }
+  const double dorg = dk[o];
+  for (int it = 0; it < 80; ++it) {
+    double f = 0.0, af = 0.0, psp = 0.0, php = 0.0;
+    for (int i = q; i < K; i += 4) {
+      const double r = rcp_nr((dk[i] - dorg) - t);
+      const double w = z2[i] * r;
+      f += w;
+      af += fabs(w);
+      if (i <= j) psp += w * r;
+      else php += w * r;
+    }
+    f = 1.0 + sum4(f);
+    af = sum4(af);
+    psp = sum4(psp);
+    php = sum4(php);
+    if (fabs(f) <= 16.0 * DBL_EPSILON * (1.0 + af) || hi_t - lo_t <= 2.0 * DBL_EPSILON * fmax(fabs(lo_t), fabs(hi_t)))
+      break;
+    if (f > 0.0) hi_t = t;
+    else lo_t = t;
+    const double djl = (dk[j] - dorg) - t;
+    const double s1 = djl * djl * psp;
+    double tn = t;
+    bool ok = false;
+    if (j < K - 1) {
+      const double djr = (dk[j + 1] - dorg) - t;
+      const double s2 = djr * djr * php;
+      const double c = f - s1 / djl - s2 / djr;
+      double e;
+      if (dc_quad(c, djl, djr, s1, s2, lo_t - t, hi_t - t, e)) { tn = t + e; ok = true; }
+    } else {
+      const double c = f - s1 / djl;
+      if (c != 0.0) {
+        const double e = djl + s1 / c;
+        if (e > lo_t - t && e < hi_t - t) { tn = t + e; ok = true; }
+      }
+    }
+    if (!ok || !(tn > lo_t && tn < hi_t)) tn = 0.5 * (lo_t + hi_t);
+    t = tn;
+  }
+}
+
+// one level of merges: Qin holds the children's eigenvectors (block diagonal), Qout receives the
+// merged ones; U (m x m) holds each merge's K x K eigenvector matrix of D + rho z z^T, transposed, in
+// its diagonal block, rows ordered top-half columns, rotated columns, bottom-half columns.
+__device__ void dc_level(DcShared& sh, const double* __restrict__ ev, int m, int dd, double* __restrict__ Qin,
+                         double* __restrict__ Qout, double* __restrict__ U, int tid) {
+  // children's eigenpairs -> z, sorted d of every merge node
+  if (tid < m) {
+    const int g = tid;
+    int lo, hi;
+    if (!dc_node(g, dd, m, lo, hi)) {
+      sh.nlo[g] = -1;
+      Qout[g + (size_t)g * m] = 1.0;   // a leaf passes through
+    } else {
+      sh.nlo[g] = lo;
+      const int mid = lo + ((hi - lo) >> 1);
+      const double be = ev[mid - 1];
+      double z = g < mid ? Qin[(mid - 1) + (size_t)g * m] : (be < 0.0 ? -Qin[mid + (size_t)g * m] : Qin[mid + (size_t)g * m]);
+      z *= 0.70710678118654752440;
+      const double lg = sh.lam[g];
+      int rank = 0;
+      for (int h = lo; h < hi; ++h) {
+        const double lh = sh.lam[h];
+        rank += (lh < lg) || (lh == lg && h < g);
+      }
+      sh.ds[lo + rank] = lg;
+      sh.zs[lo + rank] = z;
+      sh.src[lo + rank] = g;
+      atomicMax(&sh.nmd[lo], (unsigned long long)__double_as_longlong(fabs(lg)));
+      atomicMax(&sh.nmz[lo], (unsigned long long)__double_as_longlong(fabs(z)));
+    }
+  }
+  __syncthreads();
+  // deflation: one thread per merge node scans its sorted entries
+  if (tid < m && sh.nlo[tid] == tid) {
+    const int lo = tid;
+    int lo2, hi;
+    dc_node(lo, dd, m, lo2, hi);
+    const int mid = lo + ((hi - lo) >> 1);
+    const double rho = 2.0 * fabs(ev[mid - 1]);
+    const double dmx = __longlong_as_double((long long)sh.nmd[lo]), zmx = __longlong_as_double((long long)sh.nmz[lo]);
+    const double tol = 8.0 * DBL_EPSILON * fmax(dmx, rho * zmx);
+    int K = 0, nd = 0, nr = 0, c0 = 0, c1 = 0, c2 = 0;
+    bool have = false, prot = false;
+    // keep the pending entry: type 0 top half only, 1 both halves (rotated), 2 bottom half only
+#define DC_KEEP()                                                  \
+  do {                                                             \
+    const int ty = prot ? 1 : (sp < mid ? 0 : 2);                  \
+    sh.dK[lo + K] = dp;                                            \
+    sh.zK[lo + K] = zp;                                            \
+    sh.z2[lo + K] = rho * zp * zp;                                 \
+    sh.kc[lo + K] = sp;                                            \
+    sh.ktype[lo + K] = ty;                                         \
+    sh.upos[lo + K] = ty == 0 ? c0 : (ty == 1 ? c1 : c2);          \
+    c0 += ty == 0;                                                 \
+    c1 += ty == 1;                                                 \
+    c2 += ty == 2;                                                 \
+    ++K;                                                           \
+  } while (0)
+    double dp = 0.0, zp = 0.0;   // the pending entry (the last one not deflated)
+    int sp = 0;
+    double dnx = sh.ds[lo], znx = sh.zs[lo];
+    int snx = sh.src[lo];
+    for (int nj = lo; nj < hi; ++nj) {
+      const double dn = dnx, zn = znx;
+      const int sn = snx;
+      if (nj + 1 < hi) {   // the next entry's loads are in flight while this one is decided
+        dnx = sh.ds[nj + 1];
+        znx = sh.zs[nj + 1];
+        snx = sh.src[nj + 1];
+      }
+      if (rho * fabs(zn) <= tol) {
+        sh.dvl[hi - 1 - nd] = dn;
+        sh.dcol[hi - 1 - nd] = sn;
+        ++nd;
+        continue;
+      }
+      if (have) {
+        const double dt = dn - dp;
+        if (fabs(dt) <= 2.0 * tol) {   // |dt c s| <= |dt| / 2: only close pairs can deflate
+          const double tt = hypot(zn, zp);
+          const double c = zn / tt, sr = -zp / tt;
+          if (fabs(dt * c * sr) <= tol) {
+            // rotate the pending entry away: it deflates, the new one carries z = tt
+            sh.ra[lo + nr] = sp;
+            sh.rb[lo + nr] = sn;
+            sh.rc[lo + nr] = c;
+            sh.rs[lo + nr] = sr;
+            ++nr;
+            sh.dvl[hi - 1 - nd] = dp * c * c + dn * sr * sr;
+            sh.dcol[hi - 1 - nd] = sp;
+            ++nd;
+            dp = dp * sr * sr + dn * c * c;
+            zp = tt;
+            sp = sn;
+            prot = true;
+            continue;
+          }
+        }
+        DC_KEEP();
+      }
+      have = true;
+      dp = dn;
+      zp = zn;
+      sp = sn;
+      prot = false;
+    }
+    if (have) DC_KEEP();
+#undef DC_KEEP
+    sh.nK[lo] = K;
+    sh.nT1[lo] = c0;
+    sh.nT12[lo] = c0 + c1;
+    sh.nRot[lo] = nr;
+    sh.nRho[lo] = rho;
+  }
+  __syncthreads();
+  // deflation rotations on the rows of Qin (row-parallel); secular roots (four lanes each)
+  if (tid < DC_SEC) {
+    const int r = tid;
+    if (r < m && sh.nlo[r] >= 0) {
+      const int lo = sh.nlo[r], nr = sh.nRot[lo];
+      for (int t = 0; t < nr; ++t) {
+        const size_t ia = r + (size_t)sh.ra[lo + t] * m, ib = r + (size_t)sh.rb[lo + t] * m;
+        const double x = Qin[ia], y = Qin[ib], c = sh.rc[lo + t], s = sh.rs[lo + t];
+        Qin[ia] = c * x + s * y;
+        Qin[ib] = -s * x + c * y;
+      }
+    }
+  } else {
+    const int g = (tid - DC_SEC) >> 2, q = tid & 3;
+    if (g < m && sh.nlo[g] >= 0) {
+      const int lo = sh.nlo[g], j = g - lo, K = sh.nK[lo];
+      if (j < K) {
+        int o;
+        double t;
+        dc_secular(sh.dK + lo, sh.z2 + lo, K, j, q, o, t);
+        if (q == 0) {
+          sh.org[lo + j] = lo + o;
+          sh.tau[lo + j] = t;
+        }
+      }
+    }
+  }
+  __syncthreads();
+  // z from the roots (Lowner's formula)
+  if (tid >= DC_SEC) {
+    const int g = (tid - DC_SEC) >> 2, q = tid & 3;
+    if (g < m && sh.nlo[g] >= 0) {
+      const int lo = sh.nlo[g], i = g - lo, K = sh.nK[lo];
+      if (i < K) {
+        const double* dk = sh.dK + lo;
+        const double di = dk[i], rho = sh.nRho[lo];
+        double p = 1.0;
+        for (int jj = q; jj < K; jj += 4) {
+          const double lmd = (sh.dK[sh.org[lo + jj]] - di) + sh.tau[lo + jj];
+          const double den = jj == K - 1 ? rho : (jj < i ? dk[jj] - di : dk[jj + 1] - di);
+          p *= lmd / den;
+        }
+        p = prod4(p);
+        if (q == 0) {
+          sh.zh[lo + i] = copysign(sqrt(fmax(p, 0.0)), sh.zK[lo + i]);
+          const int ty = sh.ktype[lo + i];
+          const int pos = sh.upos[lo + i] + (ty == 0 ? 0 : (ty == 1 ? sh.nT1[lo] : sh.nT12[lo]));
+          sh.upos[lo + i] = pos;
+          sh.kcp[lo + pos] = sh.kc[lo + i];
+        }
+      }
+    }
+  }
+  __syncthreads();
+  // norms of the vectors u_j; the new eigenvalues
+  if (tid >= DC_SEC) {
+    const int g = (tid - DC_SEC) >> 2, q = tid & 3;
+    if (g < m && sh.nlo[g] >= 0) {
+      const int lo = sh.nlo[g], j = g - lo, K = sh.nK[lo];
+      if (j < K) {
+        const double dorg = sh.dK[sh.org[lo + j]], tj = sh.tau[lo + j];
+        double s = 0.0;
+        for (int i = q; i < K; i += 4) {
+          const double u = sh.zh[lo + i] / ((sh.dK[lo + i] - dorg) - tj);
+          s += u * u;
+        }
+        s = sum4(s);
+        if (q == 0) {
+          sh.rn[lo + j] = 1.0 / sqrt(s);
+          sh.lam[lo + j] = dorg + tj;
+        }
+      } else if (q == 0) {
+        sh.lam[g] = sh.dvl[g];
+      }
+    }
+  }
+  __syncthreads();
+  // the K x K eigenvector matrices: items (row r, column t) inside the node blocks
+  const int smax = (m + (1 << dd) - 1) >> dd;   // the widest node of this level
+  for (int it = tid; it < m * smax; it += DC_NT) {
+    const int r = it / smax, t = it % smax;
+    const int lo = sh.nlo[r];
+    if (lo < 0) continue;
+    const int K = sh.nK[lo], i = r - lo;
+    if (i >= K || t >= K) continue;
+    const double u = sh.zh[lo + i] / ((sh.dK[lo + i] - sh.dK[sh.org[lo + t]]) - sh.tau[lo + t]) * sh.rn[lo + t];
+    U[(lo + t) + (size_t)(lo + sh.upos[lo + i]) * m] = u;   // row p = upos of the K x K matrix, stored transposed
+  }
+  if (tid < m) {
+    sh.nmd[tid] = 0;
+    sh.nmz[tid] = 0;
+  }
+  __syncthreads();
+  // Qout = Qin[:, kept] U, deflated columns copied.  Levels of narrow merges: one output per item.
+  if (smax < DC_BIG) {
+    for (int it = tid; it < m * smax; it += DC_NT) {
+      const int r = it / smax, t = it % smax;
+      const int lo = sh.nlo[r];
+      if (lo < 0) continue;
+      int lo2, hi;
+      dc_node(r, dd, m, lo2, hi);
+      if (t >= hi - lo) continue;
+      const int K = sh.nK[lo], c = lo + t;
+      double acc;
+      if (t < K) {
+        acc = 0.0;
+        for (int p = 0; p < K; ++p) acc += Qin[r + (size_t)sh.kcp[lo + p] * m] * U[c + (size_t)(lo + p) * m];
+      } else {
+        acc = Qin[r + (size_t)sh.dcol[c] * m];
+      }
+      Qout[r + (size_t)c * m] = acc;
+    }
+    __syncthreads();
+    return;
+  }
+  // wide merges: 16 x 16 output tiles on the FP64 matrix cores, one wavefront per tile, operands
+  // straight from L2
+  const int wave = tid >> 6, lane = tid & 63, li = lane & 15, lk = lane >> 4;
+  int base = 0;
+  const int nn = 1 << dd;
+  for (int k = 0; k < nn; ++k) {
+    int lo = 0, hi = m;
+    for (int l = dd - 1; l >= 0; --l) {
+      const int mid = lo + ((hi - lo) >> 1);
+      if ((k >> l) & 1) lo = mid;
+      else hi = mid;
+    }
+    const int s = hi - lo;
+    if (s < 2) continue;   // no merge node here (uniform)
+    const int K = sh.nK[lo], mid = lo + (s >> 1);
+    const int rt = (s + 15) >> 4, ct = (K + 15) >> 4, ntile = rt * ct;
+    for (int tl = wave - base; tl < ntile; tl += DC_NT / 64) {
+      if (tl < 0) continue;
+      const int r0 = lo + 16 * (tl % rt), t0 = 16 * (tl / rt);
+      // rows of the top half meet only top-half and rotated columns, the bottom half the others
+      const int p0 = r0 >= mid ? sh.nT1[lo] : 0;
+      const int p1 = r0 + 15 < mid ? sh.nT12[lo] : K;
+      const int ra = r0 + li;   // A-fragment row of this lane (A[li][lk])
+      const int tb = t0 + li;   // B-fragment column of this lane (B[lk][li])
+      d4 acc = {0.0, 0.0, 0.0, 0.0};
+      // four k-steps per pass: all eight operand loads are in flight before the first product
+      for (int p = p0; p < p1; p += 16) {
+        double a[4], b[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int pk = p + 4 * u + lk;
+          const bool okp = pk < p1;
+          a[u] = (okp && ra < hi) ? Qin[ra + (size_t)sh.kcp[lo + (okp ? pk : p0)] * m] : 0.0;
+          b[u] = (okp && tb < K) ? U[(lo + tb) + (size_t)(lo + pk) * m] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u], b[u], acc, 0, 0, 0);
+      }
+      // output lane layout of the 16 x 16 x 4 FP64 product: register q holds row lk + 4 q, column li
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int r = r0 + lk + 4 * q;
+        if (r < hi && tb < K) Qout[r + (size_t)(lo + tb) * m] = acc[q];
+      }
+    }
+    base += ntile;
+    const int nd = s - K;
+    for (int e = tid; e < nd * s; e += DC_NT) {
+      const int r = lo + e % s, t = K + e / s;
+      Qout[r + (size_t)(lo + t) * m] = Qin[r + (size_t)sh.dcol[lo + t] * m];
+    }
+  }
+  __syncthreads();
+}
+
+// tridiagonal (d, e) -> eigenvalues lam (unsorted) and eigenvectors in Qa (m x m); Qb, U: m x m scratch
+__global__ __launch_bounds__(DC_NT) void dc_kernel(const double* __restrict__ d, const double* __restrict__ e, int m,
+                                                   double* __restrict__ Qa, double* __restrict__ Qb,
+                                                   double* __restrict__ U, double* __restrict__ lam_out,
+                                                   const int* __restrict__ gate) {
+  if (gate && *gate == 0) return;
+  __shared__ DcShared sh;
+  const int tid = threadIdx.x;
+  const int depth = dc_depth(m);
+  // the levels alternate between the buffers and end in Qa
+  double* Qin = (depth & 1) ? Qb : Qa;
+  double* Qout = (depth & 1) ? Qa : Qb;
+  for (int idx = tid; idx < m * m; idx += DC_NT) {
+    const int r = idx % m, c = idx / m;
+    Qin[idx] = r == c ? 1.0 : 0.0;
+    Qout[idx] = 0.0;
+  }
+  // leaves: every boundary is a tear, d_i -= |e_(i-1)| + |e_i|
+  if (tid < m) {
+    double v = d[tid];
+    if (tid > 0) v -= fabs(e[tid - 1]);
+    if (tid < m - 1) v -= fabs(e[tid]);
+    sh.lam[tid] = v;
+    sh.nmd[tid] = 0;
+    sh.nmz[tid] = 0;
+  }
+  __syncthreads();
+  for (int dd = depth - 1; dd >= 0; --dd) {
+    dc_level(sh, e, m, dd, Qin, Qout, U, tid);
+    double* t = Qin;
+    Qin = Qout;
+    Qout = t;
+  }
+  if (tid < m) lam_out[tid] = sh.lam[tid];
+}
+
+// W = H_0 H_1 ... H_(m-3) Z in place (Z = W), one wavefront per column
+__global__ __launch_bounds__(64) void trd_back_kernel(const double* __restrict__ Vh, const double* __restrict__ tau,
+                                                      int m, double* __restrict__ Z, const int* __restrict__ gate) {
+  if (gate && *gate == 0) return;
+  const int c = blockIdx.x, lane = threadIdx.x;
+  double* zc = Z + (size_t)c * m;
+  double z[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) z[k] = lane + 64 * k < m ? zc[lane + 64 * k] : 0.0;
+  for (int j = m - 3; j >= 0; --j) {
+    const double t = tau[j];
+    if (t == 0.0) continue;
+    const double* v = Vh + (size_t)j * m;
+    double vv[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) vv[k] = lane + 64 * k < m ? v[lane + 64 * k] : 0.0;
+    double s = vv[0] * z[0] + vv[1] * z[1] + vv[2] * z[2];
+    s = t * wave_sum(s);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) z[k] -= s * vv[k];
+  }
+#pragma unroll
+  for (int k = 0; k < 3; ++k)
+    if (lane + 64 * k < m) zc[lane + 64 * k] = z[k];
+}
+
+}  // namespace
+
+size_t syev_dc_work_doubles(int m) { return 4 * (size_t)m * m + 4 * (size_t)m + 8; }
+
+void syev_dc(const double* H, int ldh, int m, double* evals, double* V, double* work, hipStream_t s,
+             const int* gate) {
+  if (m < 1 || m > JACOBI_MAX) throw Error(TNAD_ERR_ARG, "syev_dc: size out of range");
+  const size_t mm = (size_t)m * m;
+  double* Vh = work;
+  double* Qa = Vh + mm;
+  double* Qb = Qa + mm;
+  double* U = Qb + mm;
+  double* dv = U + mm;
+  double* ev = dv + m;
+  double* tau = ev + m;
+  double* lam = tau + m;
+  hipLaunchKernelGGL(trd_kernel, dim3(1), dim3(TD_NT), 0, s, H, ldh, m, dv, ev, Vh, tau, gate);
+  TNAD_HIP_TRY(hipGetLastError());
+  hipLaunchKernelGGL(dc_kernel, dim3(1), dim3(DC_NT), 0, s, dv, ev, m, Qa, Qb, U, lam, gate);
+  TNAD_HIP_TRY(hipGetLastError());
+  if (m > 2) {
+    hipLaunchKernelGGL(trd_back_kernel, dim3(m), dim3(64), 0, s, Vh, tau, m, Qa, gate);
+    TNAD_HIP_TRY(hipGetLastError());
+  }
+  sort_eigenpairs(lam, Qa, m, evals, V, gate, s);
+}
+
+}  // namespace tnad

@@ -857,6 +857,12 @@ __global__ __launch_bounds__(EIG_NT) void ritz_select_kernel(const double* __res
 }
 }  // namespace
 
+void sort_eigenpairs(const double* lam, const double* Vu, int m, double* evals, double* V, const int* gate,
+                     hipStream_t s) {
+  hipLaunchKernelGGL(jacobi_sort_kernel, dim3(m), dim3(JS_NT), 0, s, lam, Vu, m, evals, V, gate);
+  TNAD_HIP_TRY(hipGetLastError());
+}
+
 static size_t jacobi_lds(int m) {   // four planes of P (P + 1) / 2 blocks, P = ceil(m / 2)
   const int P = (m + 1) / 2;
   return (size_t)(4 * (P * (P + 1) / 2)) * sizeof(double);

@@ -98,6 +98,17 @@ float jacobi_rot_time(const double* H, int m, int max_sweeps, int reps, double* 
 void jacobi_eig_ws(const double* H, int ldh, int m, double* evals, double* V, double* Vwork,
                    int max_sweeps, int* sweeps_out, hipStream_t s, const int* gate = nullptr);
 
+// Full symmetric eigendecomposition of an m x m matrix (m <= JACOBI_MAX, the upper triangle of H is read)
+// by Householder tridiagonalisation, divide and conquer and back-transformation (dc.hip); output as
+// jacobi_eig_ws (sorted by |lambda| descending, sign convention).  work: syev_dc_work_doubles(m) doubles.
+size_t syev_dc_work_doubles(int m);
+void syev_dc(const double* H, int ldh, int m, double* evals, double* V, double* work, hipStream_t s,
+             const int* gate = nullptr);
+// eigenpairs (lam[i], column i of Vu, m x m) sorted by |lambda| descending into evals / V with the sign
+// convention of jacobi_eig_ws; skipped when *gate == 0
+void sort_eigenpairs(const double* lam, const double* Vu, int m, double* evals, double* V, const int* gate,
+                     hipStream_t s);
+
 // Eigenpairs of a nearly diagonal symmetric m x m H for its first nh columns (pairs among the
 // others are not rotated), sorted by |lambda| descending: pairs with min(i, j) < nh coupled more
 // strongly than |H_ij| > thresh |H_ii - H_jj| are rotated away by Jacobi rotations (a few passes),

@@ -644,10 +644,10 @@ void z_heig(Ctx* c, const double* H, int n, double* U, double* lam) {
   if (n < 1 || 2 * n > JACOBI_MAX) throw Error(TNAD_ERR_UNSUPPORTED, "complex eigensolver: n must be 1..96");
   hipStream_t s = c->stream;
   const int m = 2 * n;
-  DevBuf R(c, (size_t)m * m * 8), ev2(c, (size_t)m * 8), V2(c, (size_t)m * m * 8), work(c, jacobi_work_doubles(m, 60) * 8);
+  DevBuf R(c, (size_t)m * m * 8), ev2(c, (size_t)m * 8), V2(c, (size_t)m * m * 8), work(c, syev_dc_work_doubles(m) * 8);
   hipLaunchKernelGGL(realify_kernel, dim3(nb((size_t)m * m)), dim3(NT), 0, s, H, n, R.as<double>());
   TNAD_HIP_TRY(hipGetLastError());
-  jacobi_eig_ws(R.as<double>(), m, m, ev2.as<double>(), V2.as<double>(), work.as<double>(), 60, nullptr, s);
+  syev_dc(R.as<double>(), m, m, ev2.as<double>(), V2.as<double>(), work.as<double>(), s);
   hipLaunchKernelGGL(zeig_extract_kernel, dim3(1), dim3(ZX_NT), 0, s, V2.as<double>(), ev2.as<double>(), n, U, lam);
   TNAD_HIP_TRY(hipGetLastError());
 }

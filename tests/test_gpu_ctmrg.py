@@ -91,6 +91,41 @@ def test_symeig_full_matches_lapack(ctx, n):
     assert np.allclose(A @ U, U * lam[None, :], atol=1e-11 * np.abs(ref).max())
 
 
+def _hard_matrix(kind, n, rng):
+    Q, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    if kind == "graded":        # spectrum over ten decades, as a converged corner matrix's
+        lam = np.sign(rng.standard_normal(n)) * np.logspace(0, -10, n)
+    elif kind == "degenerate":  # exact four-fold multiplets (symmetry-degenerate spectra)
+        lam = np.repeat(rng.standard_normal((n + 3) // 4), 4)[:n]
+    elif kind == "clustered":   # many exactly equal and nearly equal values
+        lam = np.round(rng.standard_normal(n), 1)
+    elif kind == "diagonal":
+        return np.diag(rng.standard_normal(n))
+    elif kind == "zero":
+        return np.zeros((n, n))
+    else:                       # path graph: a tridiagonal matrix with unit couplings
+        return np.diag(np.ones(n - 1), 1) + np.diag(np.ones(n - 1), -1)
+    A = (Q * lam) @ Q.T
+    return 0.5 * (A + A.T)
+
+
+@pytest.mark.parametrize("n", [3, 17, 176, 192])
+@pytest.mark.parametrize("kind", ["graded", "degenerate", "clustered", "diagonal", "zero", "path"])
+def test_symeig_full_hard_spectra(ctx, kind, n):
+    # the full-spectrum solver (Householder tridiagonalisation + divide and conquer) on spectra that
+    # exercise its deflation (exact and near multiplets, tiny couplings) against LAPACK: residual,
+    # orthogonality and eigenvalues to 1e-12 of the spectral norm
+    _say(f"case: symeig full n={n} {kind}")
+    A = _hard_matrix(kind, n, np.random.default_rng(100 + n))
+    lam, U = _symeig(ctx, A, n, n)
+    ref = np.linalg.eigvalsh(A)
+    scale = max(np.abs(ref).max(), 1e-300)
+    assert np.max(np.abs(np.sort(lam) - ref)) <= 1e-12 * scale
+    assert np.allclose(U.T @ U, np.eye(n), atol=1e-12)
+    assert np.max(np.abs(A @ U - U * lam[None, :])) <= 1e-12 * scale
+    assert np.all(np.diff(np.abs(lam)) <= 0.0)   # sorted by |lambda| descending
+
+
 def _symeig(ctx, A, want, k):
     import ctypes as C
     n = A.shape[0]
