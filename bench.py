@@ -125,24 +125,25 @@ def roofline_leg(tnad, ctx, chi, Dd):
                              "avg_launch_ms": round(roof["fp64"]["ms"], 5)}}
 
 
-# LDS bandwidth of one CU: 128 bytes per clock at the 2.4 GHz peak engine clock
-LDS_PEAK_GBS = 128 * 2.4
+# FP64 vector peak of one CU (of 256): the eigensolver's one-workgroup kernels run on one CU
+CU_FP64_GFLOPS = 78.6e3 / 256
 
 
-def eigensolver_leg(tnad, ctx, sweeps=3):
-    """The eigensolver's rotation kernel (one workgroup, the packed 192 x 192 FP64 Rayleigh-Ritz matrix in
-    LDS), timed alone with HIP events on the library stream: every round reads and writes the whole packed
-    matrix (4 planes of 96 * 97 / 2 2x2-block entries, 8 bytes each, both ways), which bounds it by the LDS
-    bandwidth of the one CU it runs on."""
-    m = 192
-    ms, rounds = tnad.time_jacobi(m, sweeps, reps=5, ctx=ctx)
-    p = m // 2
-    bytes_round = 2 * 4 * (p * (p + 1) // 2) * 8
-    gbs = bytes_round * rounds / (ms * 1e-3) / 1e9
-    return {"bound": "lds", "kernel": f"jacobi_rot_kernel (m={m}, {sweeps} sweeps = {rounds} rounds, one workgroup)",
-            "achieved": round(gbs, 1), "peak": LDS_PEAK_GBS, "unit": "GB/s (LDS of one CU)",
-            "frac": round(gbs / LDS_PEAK_GBS, 4), "bytes_per_round": bytes_round,
-            "avg_launch_ms": round(ms, 4), "us_per_round": round(ms * 1e3 / rounds, 3)}
+def eigensolver_leg(tnad, ctx, m=176):
+    """The exact eigensolver of the Rayleigh-Ritz steps (divide and conquer, m = the subspace block at
+    chi = 128), its stages timed alone with HIP events on the library stream.  The Householder
+    tridiagonalisation (one workgroup, the matrix in registers) has 4/3 m^3 algorithmic flops; it is bound by
+    the per-step latency of its two barriers and dependent LDS / reduction / FP64 divide chain, far below the
+    FP64 peak of the one CU it runs on."""
+    trd, dc, back = tnad.time_syev(m, reps=10, ctx=ctx)
+    flops = 4.0 / 3.0 * m ** 3
+    gf = flops / (trd * 1e-3) / 1e9
+    return {"bound": "fp64 (one CU)", "kernel": f"trd_kernel (Householder tridiagonalisation, m={m}, one workgroup)",
+            "achieved": round(gf, 2), "peak": round(CU_FP64_GFLOPS, 1), "unit": "GFLOP/s (FP64 of one CU)",
+            "frac": round(gf / CU_FP64_GFLOPS, 4), "flops": round(flops),
+            "avg_launch_ms": round(trd, 4),
+            "solve_ms": {"tridiagonalisation": round(trd, 4), "divide_and_conquer": round(dc, 4),
+                         "back_transformation_and_sort": round(back, 4)}}
 
 
 def main():

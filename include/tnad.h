@@ -221,12 +221,18 @@ int tnad_contract(tnad_ctx* ctx, int precision, const char* ia, const long long*
 int tnad_time_contract(tnad_ctx* ctx, int precision, const char* ia, const long long* da, const char* ib,
                        const long long* db, const char* ic, const long long* dc, int reps, double* avg_ms);
 
-/* Average device time of one launch of the Jacobi eigensolver's rotation kernel (the Rayleigh-Ritz and
- * full-spectrum solver: one workgroup, the packed m x m FP64 matrix in LDS) on a random symmetric m x m
+/* Average device time of one launch of the Jacobi eigensolver's rotation kernel (one workgroup, the packed
+ * m x m FP64 matrix in LDS; the warm-start codec's solver) on a random symmetric m x m
  * matrix, `sweeps` sweeps unless it converges sooner (*rounds, nullable, receives the rounds one launch
  * ran), HIP events on the context's stream over `reps` launches.  2 <= m <= 192, 1 <= sweeps <= 60.  For
  * roofline measurement; no counterpart in the reference. */
 int tnad_time_jacobi(tnad_ctx* ctx, int m, int sweeps, int reps, double* avg_ms, int* rounds);
+
+/* Roofline timing of the divide-and-conquer symmetric eigensolver (no counterpart in the reference): its
+ * stages on a random symmetric m x m matrix (3 <= m <= 192), averaged over reps runs after one warm-up, HIP
+ * events on the context's stream: stage_ms[0] Householder tridiagonalisation (one workgroup), stage_ms[1]
+ * divide and conquer on the tridiagonal matrix (one workgroup), stage_ms[2] back-transformation and sort. */
+int tnad_time_syev(tnad_ctx* ctx, int m, int reps, double* stage_ms);
 
 /* ---------------------------------------------------------------------------------------------
  * Learned warm start.  The reference starts CTMRG from _initializect_square (src/ctmrg.jl:62-86,

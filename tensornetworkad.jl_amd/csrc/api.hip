@@ -866,4 +866,16 @@ int tnad_time_jacobi(tnad_ctx* ctx, int m, int sweeps, int reps, double* avg_ms,
   });
 }
 
+int tnad_time_syev(tnad_ctx* ctx, int m, int reps, double* stage_ms) {
+  return guarded(ctx, [&] {
+    if (!stage_ms || m < 3 || m > JACOBI_MAX || reps < 1) throw Error(TNAD_ERR_ARG, "time_syev: bad argument");
+    Ctx* c = &ctx->c;
+    DevBuf H(c, (size_t)m * m * 8), W(c, (syev_dc_work_doubles(m) + (size_t)m * m + m) * 8);
+    k_random_matrix(H.as<double>(), (size_t)m * m, 17u, c->stream);   // the upper triangle is read
+    float ms[3];
+    syev_dc_time(H.as<double>(), m, reps, W.as<double>(), c->stream, ms);
+    for (int k = 0; k < 3; ++k) stage_ms[k] = ms[k];
+  });
+}
+
 }  // extern "C"

@@ -52,14 +52,14 @@ struct StepState {
 struct EigStats {
   int iters = 0, k = 0;
   double resid = 0.0;
-  int jacobi = 0;   // Rayleigh-Ritz steps that needed Jacobi sweeps
+  int exact = 0;    // Rayleigh-Ritz steps solved exactly (the selective path declined)
 };
 struct EigBackStats {
   int pcg_iters = 0;
 };
 
 struct EigWork {
-  DevBuf Y, Xr, Yr, H, W, Vw, G, Rinv, nrm, res, info, sweeps, scal;
+  DevBuf Y, Xr, Yr, H, W, Vw, G, Rinv, nrm, res, info, scal;
   EigWork(Ctx* c, int n, int k);
 };
 

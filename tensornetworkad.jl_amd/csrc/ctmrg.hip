@@ -423,9 +423,8 @@ static void orthonormalize(Ctx* c, double* Y, int n, int k, double* X, EigWork& 
 
 EigWork::EigWork(Ctx* c, int n, int k)
     : Y(c, (size_t)n * k * 8), Xr(c, (size_t)n * k * 8), Yr(c, (size_t)n * k * 8), H(c, (size_t)k * k * 8),
-      W(c, (size_t)k * k * 8), Vw(c, jacobi_work_doubles(k, 60) * 8), G(c, (size_t)k * k * 8), Rinv(c, (size_t)k * k * 8),
-      nrm(c, (size_t)k * 8), res(c, (size_t)(k + 2) * 8), info(c, (size_t)k * sizeof(int) + 16), sweeps(c, 16),
-      scal(c, 16 * 8) {}
+      W(c, (size_t)k * k * 8), Vw(c, syev_dc_work_doubles(k) * 8), G(c, (size_t)k * k * 8), Rinv(c, (size_t)k * k * 8),
+      nrm(c, (size_t)k * 8), res(c, (size_t)(k + 2) * 8), info(c, (size_t)k * sizeof(int) + 16), scal(c, 16 * 8) {}
 
 int topk_eig(Ctx* c, const double* A, int n, int want, int k, const double* X0, int maxit, double tol,
              double* X, double* AX, double* theta, EigStats* st) {
@@ -459,8 +458,8 @@ int topk_eig(Ctx* c, const double* A, int n, int want, int k, const double* X0, 
   }();
   int it = 0;
   double rel = 0.0, best = HUGE_VAL, best_mark = HUGE_VAL, prev_rel = HUGE_VAL;
-  int best_it = 0, n_jacobi = 0;
-  bool force_jacobi = false;
+  int best_it = 0, n_exact = 0;
+  bool force_exact = false;
   for (it = 1; it <= maxit; ++it) {
     // Y = A X ; H = X^T Y ; H = W diag(theta) W^T ; Xr = X W ; Yr = Y W
     tc_launch(c, Prec::F64, gemm_desc('N', 'N', n, k, n, n, n, n), A, X, w.Y.as<double>(), s);
@@ -469,47 +468,37 @@ int topk_eig(Ctx* c, const double* A, int n, int want, int k, const double* X0, 
     dbg_finite(c, "H", w.H.as<double>(), (size_t)k * k);
     // Rayleigh-Ritz: once the block is nearly an invariant subspace (H nearly diagonal in the
     // previous Ritz basis) a few rotations of near-degenerate pairs plus second-order perturbation
-    // theory replace the Jacobi sweeps; Jacobi on the first cold iteration, whenever H is not close
-    // enough to diagonal, and whenever the residual stopped falling (by 10% per iteration)
-    // (ritz_select's flag gates the Jacobi kernels on the device: no host round trip)
-    const bool jac_forced = force_jacobi || (it == 1 && !X0);
+    // theory (ritz_select) stand in for the eigendecomposition of H; otherwise H is decomposed exactly,
+    // by divide and conquer: on the first cold iteration, whenever H is not close enough to diagonal,
+    // and whenever the residual stopped falling (by 10% per iteration).  ritz_select's flag gates the
+    // exact solver's kernels on the device: no host round trip.  (Exact solves where two or three
+    // Jacobi sweeps used to improve the basis cut the subspace iterations per evaluation from 35 to 30
+    // on the FP64 path at chi = 128, D = 4, and 17 to 14 on the mixed one, 21% and 10% less time.)
+    const bool exact_forced = force_exact || (it == 1 && !X0);
     int* gate = w.info.as<int>();
-    if (!jac_forced) ritz_select(w.H.as<double>(), k, k, want, 1e-2, theta, w.W.as<double>(), gate, s);
-    // far from convergence the Rayleigh-Ritz step only has to improve the basis (the span, which
-    // is what converges, does not depend on it): two Jacobi sweeps do.  Closer in, three: Jacobi
-    // converges quadratically on a nearly diagonal H, and what is left is taken up by the next
-    // iteration's Rayleigh-Ritz (the residual test sees it).  When the residual stopped falling or the
-    // iteration runs long, the Rayleigh-Ritz problem is solved exactly, by divide and conquer (cheaper
-    // than Jacobi sweeps to convergence; two or three sweeps cost less than it).
-    const bool rough = it == 1 || prev_rel > 1e-6;
-    if (force_jacobi || it > 8)
-      syev_dc(w.H.as<double>(), k, k, theta, w.W.as<double>(), w.Vw.as<double>(), s, jac_forced ? nullptr : gate);
-    else
-      jacobi_eig_ws(w.H.as<double>(), k, k, theta, w.W.as<double>(), w.Vw.as<double>(), rough ? 2 : 3,
-                    dbg ? w.sweeps.as<int>() : nullptr, s, jac_forced ? nullptr : gate);
+    if (!exact_forced) ritz_select(w.H.as<double>(), k, k, want, 1e-2, theta, w.W.as<double>(), gate, s);
+    syev_dc(w.H.as<double>(), k, k, theta, w.W.as<double>(), w.Vw.as<double>(), s, exact_forced ? nullptr : gate);
     dbg_finite(c, "W", w.W.as<double>(), (size_t)k * k);
     dbg_finite(c, "theta", theta, (size_t)k);
     tc_launch(c, Prec::F64, gemm_desc('N', 'N', n, k, k, n, k, n), X, w.W.as<double>(), w.Xr.as<double>(), s);
     tc_launch(c, Prec::F64, gemm_desc('N', 'N', n, k, k, n, k, n), w.Y.as<double>(), w.W.as<double>(),
               w.Yr.as<double>(), s);
     hipLaunchKernelGGL(resid_norms_kernel, dim3(want), dim3(NT), 0, s, w.Xr.as<double>(), w.Yr.as<double>(),
-                       theta, n, w.res.as<double>(), jac_forced ? nullptr : gate);
+                       theta, n, w.res.as<double>(), exact_forced ? nullptr : gate);
     TNAD_HIP_TRY(hipGetLastError());
-    int sweeps_done = 0;
     c->read_async(hres.data(), w.res.ptr, (size_t)(want + 2) * 8);   // residuals, theta_0, flag
-    if (dbg) c->read_async(&sweeps_done, w.sweeps.ptr, sizeof(int));
     c->sync();
     hth[0] = hres[want];
     const int flag = (int)hres[want + 1];
-    const bool jac = flag != 0;   // Jacobi ran (forced, or the selective Rayleigh-Ritz declined)
-    n_jacobi += jac;
+    const bool exact = flag != 0;   // the exact solver ran (forced, or the selective Rayleigh-Ritz declined)
+    n_exact += exact;
     double mx = 0.0;
     for (int j = 0; j < want; ++j) mx = std::max(mx, hres[j]);
     rel = mx / std::max(std::fabs(hth[0]), 1e-300);
     if (dbg)
-      std::fprintf(stderr, "[tnad eig] n=%d k=%d want=%d it=%d rel_resid=%.3e theta0=%.6e rr=%s(%d) sweeps=%d\n", n,
-                   k, want, it, rel, hth[0], jac ? "jacobi" : "select", jac_forced ? -1 : flag, jac ? sweeps_done : 0);
-    force_jacobi = !(rel < 0.9 * prev_rel);
+      std::fprintf(stderr, "[tnad eig] n=%d k=%d want=%d it=%d rel_resid=%.3e theta0=%.6e rr=%s(%d)\n", n, k, want,
+                   it, rel, hth[0], exact ? "exact" : "select", exact_forced ? -1 : flag);
+    force_exact = !(rel < 0.9 * prev_rel);
     prev_rel = rel;
     if (!(rel == rel)) throw Error(TNAD_ERR_NAN, "eigensolver: NaN in Ritz residual");
     if (rel <= tol || it == maxit) break;
@@ -525,7 +514,7 @@ int topk_eig(Ctx* c, const double* A, int n, int want, int k, const double* X0, 
   k_copy(w.Yr.as<double>(), AX, (size_t)n * k, s);
   hipLaunchKernelGGL(canon_sign_kernel, dim3(k), dim3(NT), 0, s, X, AX, n);
   TNAD_HIP_TRY(hipGetLastError());
-  if (st) { st->iters = std::min(it, maxit); st->k = k; st->resid = rel; st->jacobi = n_jacobi; }
+  if (st) { st->iters = std::min(it, maxit); st->k = k; st->resid = rel; st->exact = n_exact; }
   return k;
 }
 

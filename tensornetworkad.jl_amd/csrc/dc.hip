@@ -38,6 +38,7 @@ typedef double d4 __attribute__((ext_vector_type(4)));
 
 constexpr int TD_GX = 16, TD_GY = 48;       // thread grid: 48 rows x 16 columns of threads (768)
 constexpr int TD_NT = TD_GX * TD_GY;
+static_assert(TD_GX == 16 || TD_GX == 32, "row sums span one or two DPP rows");
 constexpr int TD_RB = JACOBI_MAX / TD_GX;   // column elements per thread (12)
 constexpr int TD_RA = JACOBI_MAX / TD_GY;   // row elements per thread (4)
 static_assert(TD_RB * TD_GX == JACOBI_MAX && TD_RA * TD_GY == JACOBI_MAX, "tridiagonalisation tiling");
@@ -82,7 +83,7 @@ struct TrdShared {
   double p[2][JACOBI_MAX];     // tau A v
 };
 
-// Householder steps j = 16 BJ .. min(16 BJ + 15, m - 3).  Columns below 16 BJ are finished (they lie
+// Householder steps j = TD_GX BJ .. min(TD_GX (BJ + 1) - 1, m - 3).  Columns below TD_GX BJ are finished (they lie
 // left of every remaining step's column), so only column blocks b >= BJ are touched; row blocks whose
 // rows all lie at or above j are skipped by a uniform test.
 template <int BJ>
@@ -133,11 +134,15 @@ __device__ __forceinline__ void trd_block(double (&A)[TD_RA][TD_RB], int m, TrdS
       for (int b = BJ; b < TD_RB; ++b) ps[a] = fma(A[a][b], vc[b], ps[a]);
     }
     row16_sums<TD_RA>(ps);
+    if constexpr (TD_GX == 32) {   // rows of 32 lanes: row_bcast15 adds row 0 into row 1, row 2 into row 3
+#pragma unroll
+      for (int a = 0; a < TD_RA; ++a) ps[a] += dpp_d<0x142, 0xA>(ps[a]);
+    }
 #pragma unroll
     for (int a = 0; a < TD_RA; ++a) {
       if (TD_GY * a + TD_GY - 1 <= j) continue;
       const int r = ty + TD_GY * a;
-      if (tx == 0) pv[r] = (r > j && r < m) ? tv * ps[a] : 0.0;
+      if (tx == TD_GX - 16) pv[r] = (r > j && r < m) ? tv * ps[a] : 0.0;   // a lane that holds the row total
     }
     if (tid < 64) {
       for (int r = lane; r < m; r += 64) Vh[r + (size_t)j * m] = vat(r);
@@ -171,6 +176,15 @@ __device__ __forceinline__ void trd_block(double (&A)[TD_RA][TD_RB], int m, TrdS
   }
 }
 
+// the column blocks BJ, BJ + 1, ..., TD_RB - 1 in turn
+template <int BJ>
+__device__ __forceinline__ void trd_blocks(double (&A)[TD_RA][TD_RB], int m, TrdShared& sh, int tx, int ty,
+                                           int lane, int tid, double* __restrict__ Vh, double* __restrict__ tauo,
+                                           double* __restrict__ eo) {
+  trd_block<BJ>(A, m, sh, tx, ty, lane, tid, Vh, tauo, eo);
+  if constexpr (BJ + 1 < TD_RB) trd_blocks<BJ + 1>(A, m, sh, tx, ty, lane, tid, Vh, tauo, eo);
+}
+
 // H (m x m, ld ldh; the upper triangle is read) -> diagonal dout (m), off-diagonal eout (m - 1),
 // reflectors Vh (m x m, column j = v_j) and tauo (m - 2).
 __global__ __launch_bounds__(TD_NT) void trd_kernel(const double* __restrict__ H, int ldh, int m,
@@ -190,18 +204,7 @@ __global__ __launch_bounds__(TD_NT) void trd_kernel(const double* __restrict__ H
       const double x = H[in ? min(r, c) + (size_t)max(r, c) * ldh : 0];   // branch-free, clamped address
       A[a][b] = in ? x : 0.0;
     }
-  trd_block<0>(A, m, sh, tx, ty, lane, tid, Vh, tauo, eout);
-  trd_block<1>(A, m, sh, tx, ty, lane, tid, Vh, tauo, eout);
-  trd_block<2>(A, m, sh, tx, ty, lane, tid, Vh, tauo, eout);
-  trd_block<3>(A, m, sh, tx, ty, lane, tid, Vh, tauo, eout);
-  trd_block<4>(A, m, sh, tx, ty, lane, tid, Vh, tauo, eout);
-  trd_block<5>(A, m, sh, tx, ty, lane, tid, Vh, tauo, eout);
-  trd_block<6>(A, m, sh, tx, ty, lane, tid, Vh, tauo, eout);
-  trd_block<7>(A, m, sh, tx, ty, lane, tid, Vh, tauo, eout);
-  trd_block<8>(A, m, sh, tx, ty, lane, tid, Vh, tauo, eout);
-  trd_block<9>(A, m, sh, tx, ty, lane, tid, Vh, tauo, eout);
-  trd_block<10>(A, m, sh, tx, ty, lane, tid, Vh, tauo, eout);
-  trd_block<11>(A, m, sh, tx, ty, lane, tid, Vh, tauo, eout);
+  trd_blocks<0>(A, m, sh, tx, ty, lane, tid, Vh, tauo, eout);
 #pragma unroll
   for (int a = 0; a < TD_RA; ++a)
 #pragma unroll
@@ -739,6 +742,43 @@ void syev_dc(const double* H, int ldh, int m, double* evals, double* V, double* 
     TNAD_HIP_TRY(hipGetLastError());
   }
   sort_eigenpairs(lam, Qa, m, evals, V, gate, s);
+}
+
+void syev_dc_time(const double* H, int m, int reps, double* work, hipStream_t s, float ms[3]) {
+  if (m < 3 || m > JACOBI_MAX || reps < 1) throw Error(TNAD_ERR_ARG, "syev_dc_time: bad sizes");
+  const size_t mm = (size_t)m * m;
+  double* Vh = work;
+  double* Qa = Vh + mm;
+  double* Qb = Qa + mm;
+  double* U = Qb + mm;
+  double* dv = U + mm;
+  double* ev = dv + m;
+  double* tau = ev + m;
+  double* lam = tau + m;
+  double* V = lam + m;   // m x m sorted eigenvectors
+  double* evals = V + mm;
+  hipEvent_t e[4];
+  for (auto& x : e) TNAD_HIP_TRY(hipEventCreate(&x));
+  float acc[3] = {0.f, 0.f, 0.f};
+  for (int r = 0; r <= reps; ++r) {   // the first run warms up
+    TNAD_HIP_TRY(hipEventRecord(e[0], s));
+    hipLaunchKernelGGL(trd_kernel, dim3(1), dim3(TD_NT), 0, s, H, m, m, dv, ev, Vh, tau, (const int*)nullptr);
+    TNAD_HIP_TRY(hipEventRecord(e[1], s));
+    hipLaunchKernelGGL(dc_kernel, dim3(1), dim3(DC_NT), 0, s, dv, ev, m, Qa, Qb, U, lam, (const int*)nullptr);
+    TNAD_HIP_TRY(hipEventRecord(e[2], s));
+    hipLaunchKernelGGL(trd_back_kernel, dim3(m), dim3(64), 0, s, Vh, tau, m, Qa, (const int*)nullptr);
+    sort_eigenpairs(lam, Qa, m, evals, V, nullptr, s);
+    TNAD_HIP_TRY(hipEventRecord(e[3], s));
+    TNAD_HIP_TRY(hipEventSynchronize(e[3]));
+    if (r == 0) continue;
+    for (int k = 0; k < 3; ++k) {
+      float t = 0.f;
+      TNAD_HIP_TRY(hipEventElapsedTime(&t, e[k], e[k + 1]));
+      acc[k] += t;
+    }
+  }
+  for (auto& x : e) (void)hipEventDestroy(x);
+  for (int k = 0; k < 3; ++k) ms[k] = acc[k] / reps;
 }
 
 }  // namespace tnad

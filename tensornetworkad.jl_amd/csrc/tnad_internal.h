@@ -104,6 +104,10 @@ void jacobi_eig_ws(const double* H, int ldh, int m, double* evals, double* V, do
 size_t syev_dc_work_doubles(int m);
 void syev_dc(const double* H, int ldh, int m, double* evals, double* V, double* work, hipStream_t s,
              const int* gate = nullptr);
+// Average device times (ms) of syev_dc's stages on H (m x m, ld m; 3 <= m <= 192) over reps runs after one
+// warm-up, HIP events on stream s: ms[0] tridiagonalisation, ms[1] divide and conquer, ms[2]
+// back-transformation and sort.  work: syev_dc_work_doubles(m) + m * m + m doubles.  For roofline measurement.
+void syev_dc_time(const double* H, int m, int reps, double* work, hipStream_t s, float ms[3]);
 // eigenpairs (lam[i], column i of Vu, m x m) sorted by |lambda| descending into evals / V with the sign
 // convention of jacobi_eig_ws; skipped when *gate == 0
 void sort_eigenpairs(const double* lam, const double* Vu, int m, double* evals, double* V, const int* gate,

@@ -89,6 +89,7 @@ SIGNATURES = {
     "tnad_contract": (I, [P, I, C.c_char_p, P, P, C.c_char_p, P, P, C.c_char_p, P, D, D, P]),
     "tnad_time_contract": (I, [P, I, C.c_char_p, P, C.c_char_p, P, C.c_char_p, P, I, PD]),
     "tnad_time_jacobi": (I, [P, I, I, I, PD, PI]),
+    "tnad_time_syev": (I, [P, I, I, PD]),
     "tnad_mlp_create": (I, [P, I, I, I, I, C.c_ulonglong, C.POINTER(P)]),
     "tnad_mlp_free": (I, [P]),
     "tnad_mlp_num_params": (I, [P, C.POINTER(C.c_longlong)]),

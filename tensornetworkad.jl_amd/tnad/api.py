@@ -562,6 +562,15 @@ def time_contract(spec: str, extents: dict, precision="fp64", reps: int = 20,
     return ms.value
 
 
+def time_syev(m: int = 176, reps: int = 10, ctx: Context | None = None):
+    """Average device ms of the divide-and-conquer eigensolver's stages on a random symmetric m x m matrix:
+    (tridiagonalisation, divide and conquer, back-transformation + sort), HIP events on the context's stream."""
+    ctx = ctx or default_context()
+    ms = (C.c_double * 3)()
+    ctx.check(ctx.lib.tnad_time_syev(ctx.h, int(m), int(reps), ms))
+    return tuple(ms)
+
+
 def time_jacobi(m: int = 192, sweeps: int = 3, reps: int = 5, ctx: Context | None = None):
     """(average device ms, rounds) of one launch of the Jacobi rotation kernel on a random symmetric m x m
     matrix, `sweeps` sweeps (HIP events on the context's stream)."""

@@ -193,7 +193,7 @@ def test_enlarged_corner_at_chi128(ctx, prec):
 
 
 def test_time_jacobi_runs_the_requested_sweeps(ctx):
-    # the eigensolver roofline's timing entry: a random 192 x 192 matrix does not converge in 3 sweeps, so the
+    # the Jacobi timing entry: a random 192 x 192 matrix does not converge in 3 sweeps, so the
     # rotation kernel runs 3 x 191 rounds; an odd size is padded to even (101 -> 102, 101 rounds per sweep)
     import tnad
     print("case: time_jacobi", flush=True)
@@ -201,3 +201,15 @@ def test_time_jacobi_runs_the_requested_sweeps(ctx):
     assert rounds == 3 * 191 and 0.0 < ms < 100.0
     ms, rounds = tnad.time_jacobi(101, 2, reps=1, ctx=ctx)
     assert rounds == 2 * 101 and ms > 0.0
+
+
+def test_time_syev_reports_its_three_stages(ctx):
+    # the eigensolver roofline's timing entry: positive device times for the tridiagonalisation, the divide
+    # and conquer and the back-transformation of one solve; sizes below 3 are refused
+    import tnad
+    print("case: time_syev", flush=True)
+    for m in (176, 37):
+        t = tnad.time_syev(m, reps=2, ctx=ctx)
+        assert len(t) == 3 and all(0.0 < x < 100.0 for x in t)
+    with pytest.raises(tnad.TnadError):
+        tnad.time_syev(2, reps=1, ctx=ctx)
