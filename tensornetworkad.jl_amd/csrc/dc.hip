@@ -222,6 +222,7 @@ constexpr int DC_NT = 1024;
 constexpr int DC_M = JACOBI_MAX;
 constexpr int DC_SEC = 256;   // threads from DC_SEC on work four lanes per root / per entry
 constexpr int DC_BIG = 32;    // levels whose widest merge reaches this use the matrix-core product
+constexpr int DC_ONE_LANE = 16;   // levels whose merges are at most this wide solve one root per lane
 
 struct DcShared {
   double lam[DC_M];                 // eigenvalue of column g of the current eigenvector matrix
@@ -265,15 +266,23 @@ __device__ __forceinline__ int dc_depth(int m) {
   return d;
 }
 
-__device__ __forceinline__ double sum4(double x) {
-  x += __shfl_xor(x, 1);
-  x += __shfl_xor(x, 2);
+// sums / products over the L lanes (L = 1 or 4, aligned) that share a root (butterflies: the same
+// bits in all of them)
+template <int L>
+__device__ __forceinline__ double sumL(double x) {
+  if constexpr (L == 4) {
+    x += __shfl_xor(x, 1);
+    x += __shfl_xor(x, 2);
+  }
   return x;
 }
 
-__device__ __forceinline__ double prod4(double x) {
-  x *= __shfl_xor(x, 1);
-  x *= __shfl_xor(x, 2);
+template <int L>
+__device__ __forceinline__ double prodL(double x) {
+  if constexpr (L == 4) {
+    x *= __shfl_xor(x, 1);
+    x *= __shfl_xor(x, 2);
+  }
   return x;
 }
 
@@ -305,13 +314,18 @@ __device__ __forceinline__ bool dc_quad(double c, double djl, double djr, double
   return e > le && e < he;
 }
 
+template <int L>
 __device__ void dc_secular(const double* dk, const double* z2, int K, int j, int q, int& o, double& t) {
+  // fn(i, d_i, rho z_i^2) for every term of this lane
+  auto terms = [&](auto&& fn) {
+    for (int i = q; i < K; i += L) fn(i, dk[i], z2[i]);
+  };
   double lo_t, hi_t;
   if (j < K - 1) {
     const double mdp = 0.5 * (dk[j] + dk[j + 1]);
     double f = 0.0;
-    for (int i = q; i < K; i += 4) f += z2[i] * rcp_nr(dk[i] - mdp);
-    f = 1.0 + sum4(f);
+    terms([&](int, double d, double w) { f += w * rcp_nr(d - mdp); });
+    f = 1.0 + sumL<L>(f);
     o = f > 0.0 ? j : j + 1;
     const double gap = dk[j + 1] - dk[j];
     if (o == j) { lo_t = 0.0; hi_t = 0.5 * gap; }
@@ -323,37 +337,37 @@ __device__ void dc_secular(const double* dk, const double* z2, int K, int j, int
     t = dc_quad(cc, djl, djr, z2[j], z2[j + 1], lo_t, hi_t, e) ? e : 0.5 * (lo_t + hi_t);
   } else {
     double zz = 0.0;
-    for (int i = q; i < K; i += 4) zz += z2[i];
+    terms([&](int, double, double w) { zz += w; });
     o = j;
     lo_t = 0.0;
-    hi_t = sum4(zz);
+    hi_t = sumL<L>(zz);
     t = 0.5 * (lo_t + hi_t);
   }
-  const double dorg = dk[o];
+  const double dorg = dk[o], dj = dk[j] - dorg, dj1 = j < K - 1 ? dk[j + 1] - dorg : 0.0;
   for (int it = 0; it < 80; ++it) {
     double f = 0.0, af = 0.0, psp = 0.0, php = 0.0;
-    for (int i = q; i < K; i += 4) {
-      const double r = rcp_nr((dk[i] - dorg) - t);
-      const double w = z2[i] * r;
+    terms([&](int i, double d, double z) {
+      const double r = rcp_nr((d - dorg) - t);
+      const double w = z * r;
       f += w;
       af += fabs(w);
       if (i <= j) psp += w * r;
       else php += w * r;
-    }
-    f = 1.0 + sum4(f);
-    af = sum4(af);
-    psp = sum4(psp);
-    php = sum4(php);
+    });
+    f = 1.0 + sumL<L>(f);
+    af = sumL<L>(af);
+    psp = sumL<L>(psp);
+    php = sumL<L>(php);
     if (fabs(f) <= 16.0 * DBL_EPSILON * (1.0 + af) || hi_t - lo_t <= 2.0 * DBL_EPSILON * fmax(fabs(lo_t), fabs(hi_t)))
       break;
     if (f > 0.0) hi_t = t;
     else lo_t = t;
-    const double djl = (dk[j] - dorg) - t;
+    const double djl = dj - t;
     const double s1 = djl * djl * psp;
     double tn = t;
     bool ok = false;
     if (j < K - 1) {
-      const double djr = (dk[j + 1] - dorg) - t;
+      const double djr = dj1 - t;
       const double s2 = djr * djr * php;
       const double c = f - s1 / djl - s2 / djr;
       double e;
@@ -368,6 +382,231 @@ __device__ void dc_secular(const double* dk, const double* z2, int K, int j, int
     if (!ok || !(tn > lo_t && tn < hi_t)) tn = 0.5 * (lo_t + hi_t);
     t = tn;
   }
+}
+
+// state of a merge node's deflation scan over its sorted entries: the kept entries so far (K, per type
+// c0 top half only, c1 both halves (rotated), c2 bottom half only), the deflated ones (nd), the rotations
+// (nr) and the pending entry (the last one not deflated, which a close successor may still rotate away)
+struct DcScan {
+  int K = 0, nd = 0, nr = 0, c0 = 0, c1 = 0, c2 = 0;
+  bool have = false, prot = false;
+  double dp = 0.0, zp = 0.0;
+  int sp = 0;
+};
+
+// deflation threshold of the node at lo: 8 eps max(max |d|, rho max |z|)
+__device__ __forceinline__ double dc_tol(const DcShared& sh, int lo, double rho) {
+  const double dmx = __longlong_as_double((long long)sh.nmd[lo]), zmx = __longlong_as_double((long long)sh.nmz[lo]);
+  return 8.0 * DBL_EPSILON * fmax(dmx, rho * zmx);
+}
+
+// keep entry (d, z, column src, rotated?) as kept entry number st.K
+__device__ __forceinline__ void dc_keep(DcShared& sh, int lo, int mid, double rho, double d, double z, int src,
+                                        bool rot, DcScan& st) {
+  const int ty = rot ? 1 : (src < mid ? 0 : 2);
+  sh.dK[lo + st.K] = d;
+  sh.zK[lo + st.K] = z;
+  sh.z2[lo + st.K] = rho * z * z;
+  sh.kc[lo + st.K] = src;
+  sh.ktype[lo + st.K] = ty;
+  sh.upos[lo + st.K] = ty == 0 ? st.c0 : (ty == 1 ? st.c1 : st.c2);
+  st.c0 += ty == 0;
+  st.c1 += ty == 1;
+  st.c2 += ty == 2;
+  ++st.K;
+}
+
+// sequential scan of the sorted entries from .. hi - 1 (one lane): small z deflates; a close pending /
+// new pair is rotated (Givens on the two columns, recorded for the rows of Q) so that the pending entry
+// deflates
+__device__ __forceinline__ void dc_scan_seq(DcShared& sh, int lo, int hi, int mid, double rho, double tol, int from, DcScan& st) {
+  double dnx = sh.ds[from], znx = sh.zs[from];
+  int snx = sh.src[from];
+  for (int nj = from; nj < hi; ++nj) {
+    const double dn = dnx, zn = znx;
+    const int sn = snx;
+    if (nj + 1 < hi) {   // the next entry's loads are in flight while this one is decided
+      dnx = sh.ds[nj + 1];
+      znx = sh.zs[nj + 1];
+      snx = sh.src[nj + 1];
+    }
+    if (rho * fabs(zn) <= tol) {
+      sh.dvl[hi - 1 - st.nd] = dn;
+      sh.dcol[hi - 1 - st.nd] = sn;
+      ++st.nd;
+      continue;
+    }
+    if (st.have) {
+      const double dt = dn - st.dp;
+      if (fabs(dt) <= 2.0 * tol) {   // |dt c s| <= |dt| / 2: only close pairs can deflate
+        const double tt = hypot(zn, st.zp);
+        const double c = zn / tt, sr = -st.zp / tt;
+        if (fabs(dt * c * sr) <= tol) {
+          sh.ra[lo + st.nr] = st.sp;
+          sh.rb[lo + st.nr] = sn;
+          sh.rc[lo + st.nr] = c;
+          sh.rs[lo + st.nr] = sr;
+          ++st.nr;
+          sh.dvl[hi - 1 - st.nd] = st.dp * c * c + dn * sr * sr;
+          sh.dcol[hi - 1 - st.nd] = st.sp;
+          ++st.nd;
+          st.dp = st.dp * sr * sr + dn * c * c;
+          st.zp = tt;
+          st.sp = sn;
+          st.prot = true;
+          continue;
+        }
+      }
+      dc_keep(sh, lo, mid, rho, st.dp, st.zp, st.sp, st.prot, st);
+    }
+    st.have = true;
+    st.dp = dn;
+    st.zp = zn;
+    st.sp = sn;
+    st.prot = false;
+  }
+}
+
+__device__ __forceinline__ void dc_scan_finish(DcShared& sh, int lo, int mid, double rho, DcScan& st) {
+  if (st.have) dc_keep(sh, lo, mid, rho, st.dp, st.zp, st.sp, st.prot, st);
+  sh.nK[lo] = st.K;
+  sh.nT1[lo] = st.c0;
+  sh.nT12[lo] = st.c0 + st.c1;
+  sh.nRot[lo] = st.nr;
+  sh.nRho[lo] = rho;
+}
+
+// the scan of the node [lo, hi) by one wavefront: chunks of 64 entries are decided by ballots as long as
+// no entry of a chunk lies within 2 tol of its predecessor (then nothing rotates: every entry that is not
+// deflated is kept, in order); from the first chunk with such a pair on, lane 0 scans sequentially
+// from the state the chunks before it left (the same state the sequential scan would have reached)
+__device__ __forceinline__ void dc_scan_wave(DcShared& sh, const double* __restrict__ ev, int lo, int hi, int lane) {
+  const int mid = lo + ((hi - lo) >> 1);
+  const double rho = 2.0 * fabs(ev[mid - 1]);
+  const double tol = dc_tol(sh, lo, rho);
+  DcScan st;   // the same in every lane
+  const unsigned long long lt = (1ull << lane) - 1ull;
+  int h0 = lo;
+  for (; h0 < hi; h0 += 64) {
+    const int h = h0 + lane;
+    const bool in = h < hi;
+    const double dn = in ? sh.ds[h] : 0.0, zn = in ? sh.zs[h] : 0.0;
+    const int sn = in ? sh.src[h] : 0;
+    const bool defl = in && rho * fabs(zn) <= tol;
+    const bool kept = in && !defl;
+    const unsigned long long mk = __ballot(kept), md = __ballot(defl);
+    const unsigned long long below = mk & lt;
+    const int pl = below ? 63 - __clzll(below) : 0;
+    const double dl = __shfl(dn, pl);
+    const bool hasprev = below != 0 || st.have;
+    const double dprev = below != 0 ? dl : st.dp;
+    if (__ballot(kept && hasprev && fabs(dn - dprev) <= 2.0 * tol)) break;   // uniform
+    if (mk) {
+      const int last = 63 - __clzll(mk);
+      if (st.have) {   // the pending entry of the chunks before: kept
+        if (lane == 0) dc_keep(sh, lo, mid, rho, st.dp, st.zp, st.sp, false, st);
+        else {
+          const int ty = st.sp < mid ? 0 : 2;
+          st.c0 += ty == 0;
+          st.c2 += ty == 2;
+          ++st.K;
+        }
+      }
+      const bool top = sn < mid;
+      const unsigned long long mk1 = mk & ~(1ull << last);   // kept now; the last one becomes pending
+      const unsigned long long m0 = __ballot(kept && top) & mk1, m2 = __ballot(kept && !top) & mk1;
+      if (kept && lane != last) {
+        const int K = st.K + __popcll(mk1 & lt);
+        sh.dK[lo + K] = dn;
+        sh.zK[lo + K] = zn;
+        sh.z2[lo + K] = rho * zn * zn;
+        sh.kc[lo + K] = sn;
+        sh.ktype[lo + K] = top ? 0 : 2;
+        sh.upos[lo + K] = top ? st.c0 + __popcll(m0 & lt) : st.c2 + __popcll(m2 & lt);
+      }
+      st.K += __popcll(mk1);
+      st.c0 += __popcll(m0);
+      st.c2 += __popcll(m2);
+      st.have = true;
+      st.prot = false;
+      st.dp = __shfl(dn, last);
+      st.zp = __shfl(zn, last);
+      st.sp = __shfl(sn, last);
+    }
+    if (defl) {
+      const int e = hi - 1 - (st.nd + __popcll(md & lt));
+      sh.dvl[e] = dn;
+      sh.dcol[e] = sn;
+    }
+    st.nd += __popcll(md);
+  }
+  if (lane == 0) {
+    if (h0 < hi) dc_scan_seq(sh, lo, hi, mid, rho, tol, h0, st);
+    dc_scan_finish(sh, lo, mid, rho, st);
+  }
+}
+
+// the roots of every merge of a level (threads DC_SEC.., L lanes per root), then z from the roots
+// (Lowner's formula) and the order of the kept columns, then the norms of the vectors u_j and the new
+// eigenvalues; three barriers
+template <int L>
+__device__ void dc_roots(DcShared& sh, int m, int tid) {
+  const int g = (tid - DC_SEC) / L, q = (tid - DC_SEC) % L;
+  const bool mine = tid >= DC_SEC && g < m && sh.nlo[g] >= 0;
+  if (mine) {
+    const int lo = sh.nlo[g], j = g - lo, K = sh.nK[lo];
+    if (j < K) {
+      int o;
+      double t;
+      dc_secular<L>(sh.dK + lo, sh.z2 + lo, K, j, q, o, t);
+      if (q == 0) {
+        sh.org[lo + j] = lo + o;
+        sh.tau[lo + j] = t;
+      }
+    }
+  }
+  __syncthreads();
+  if (mine) {
+    const int lo = sh.nlo[g], i = g - lo, K = sh.nK[lo];
+    if (i < K) {
+      const double* dk = sh.dK + lo;
+      const double di = dk[i], rho = sh.nRho[lo];
+      double p = 1.0;
+      for (int jj = q; jj < K; jj += L) {
+        const double lmd = (sh.dK[sh.org[lo + jj]] - di) + sh.tau[lo + jj];
+        const double den = jj == K - 1 ? rho : (jj < i ? dk[jj] - di : dk[jj + 1] - di);
+        p *= lmd / den;
+      }
+      p = prodL<L>(p);
+      if (q == 0) {
+        sh.zh[lo + i] = copysign(sqrt(fmax(p, 0.0)), sh.zK[lo + i]);
+        const int ty = sh.ktype[lo + i];
+        const int pos = sh.upos[lo + i] + (ty == 0 ? 0 : (ty == 1 ? sh.nT1[lo] : sh.nT12[lo]));
+        sh.upos[lo + i] = pos;
+        sh.kcp[lo + pos] = sh.kc[lo + i];
+      }
+    }
+  }
+  __syncthreads();
+  if (mine) {
+    const int lo = sh.nlo[g], j = g - lo, K = sh.nK[lo];
+    if (j < K) {
+      const double dorg = sh.dK[sh.org[lo + j]], tj = sh.tau[lo + j];
+      double s = 0.0;
+      for (int i = q; i < K; i += L) {
+        const double u = sh.zh[lo + i] / ((sh.dK[lo + i] - dorg) - tj);
+        s += u * u;
+      }
+      s = sumL<L>(s);
+      if (q == 0) {
+        sh.rn[lo + j] = 1.0 / sqrt(s);
+        sh.lam[lo + j] = dorg + tj;
+      }
+    } else if (q == 0) {
+      sh.lam[g] = sh.dvl[g];
+    }
+  }
+  __syncthreads();
 }
 
 // one level of merges: Qin holds the children's eigenvectors (block diagonal), Qout receives the
@@ -402,90 +641,30 @@ __device__ void dc_level(DcShared& sh, const double* __restrict__ ev, int m, int
     }
   }
   __syncthreads();
-  // deflation: one thread per merge node scans its sorted entries
-  if (tid < m && sh.nlo[tid] == tid) {
-    const int lo = tid;
-    int lo2, hi;
-    dc_node(lo, dd, m, lo2, hi);
-    const int mid = lo + ((hi - lo) >> 1);
-    const double rho = 2.0 * fabs(ev[mid - 1]);
-    const double dmx = __longlong_as_double((long long)sh.nmd[lo]), zmx = __longlong_as_double((long long)sh.nmz[lo]);
-    const double tol = 8.0 * DBL_EPSILON * fmax(dmx, rho * zmx);
-    int K = 0, nd = 0, nr = 0, c0 = 0, c1 = 0, c2 = 0;
-    bool have = false, prot = false;
-    // keep the pending entry: type 0 top half only, 1 both halves (rotated), 2 bottom half only
-#define DC_KEEP()                                                  \
-  do {                                                             \
-    const int ty = prot ? 1 : (sp < mid ? 0 : 2);                  \
-    sh.dK[lo + K] = dp;                                            \
-    sh.zK[lo + K] = zp;                                            \
-    sh.z2[lo + K] = rho * zp * zp;                                 \
-    sh.kc[lo + K] = sp;                                            \
-    sh.ktype[lo + K] = ty;                                         \
-    sh.upos[lo + K] = ty == 0 ? c0 : (ty == 1 ? c1 : c2);          \
-    c0 += ty == 0;                                                 \
-    c1 += ty == 1;                                                 \
-    c2 += ty == 2;                                                 \
-    ++K;                                                           \
-  } while (0)
-    double dp = 0.0, zp = 0.0;   // the pending entry (the last one not deflated)
-    int sp = 0;
-    double dnx = sh.ds[lo], znx = sh.zs[lo];
-    int snx = sh.src[lo];
-    for (int nj = lo; nj < hi; ++nj) {
-      const double dn = dnx, zn = znx;
-      const int sn = snx;
-      if (nj + 1 < hi) {   // the next entry's loads are in flight while this one is decided
-        dnx = sh.ds[nj + 1];
-        znx = sh.zs[nj + 1];
-        snx = sh.src[nj + 1];
+  // deflation: a wavefront per merge node while a level has at most 16 of them (ballots decide whole chunks
+  // of 64 entries at once until a close pair needs the sequential scan), else one thread per node
+  if (dd <= 4) {
+    const int wave = tid >> 6, lane = tid & 63;
+    if (wave < (1 << dd)) {
+      int lo = 0, hi = m;
+      for (int l = dd - 1; l >= 0; --l) {
+        const int md = lo + ((hi - lo) >> 1);
+        if ((wave >> l) & 1) lo = md;
+        else hi = md;
       }
-      if (rho * fabs(zn) <= tol) {
-        sh.dvl[hi - 1 - nd] = dn;
-        sh.dcol[hi - 1 - nd] = sn;
-        ++nd;
-        continue;
-      }
-      if (have) {
-        const double dt = dn - dp;
-        if (fabs(dt) <= 2.0 * tol) {   // |dt c s| <= |dt| / 2: only close pairs can deflate
-          const double tt = hypot(zn, zp);
-          const double c = zn / tt, sr = -zp / tt;
-          if (fabs(dt * c * sr) <= tol) {
-            // rotate the pending entry away: it deflates, the new one carries z = tt
-            sh.ra[lo + nr] = sp;
-            sh.rb[lo + nr] = sn;
-            sh.rc[lo + nr] = c;
-            sh.rs[lo + nr] = sr;
-            ++nr;
-            sh.dvl[hi - 1 - nd] = dp * c * c + dn * sr * sr;
-            sh.dcol[hi - 1 - nd] = sp;
-            ++nd;
-            dp = dp * sr * sr + dn * c * c;
-            zp = tt;
-            sp = sn;
-            prot = true;
-            continue;
-          }
-        }
-        DC_KEEP();
-      }
-      have = true;
-      dp = dn;
-      zp = zn;
-      sp = sn;
-      prot = false;
+      if (hi - lo >= 2) dc_scan_wave(sh, ev, lo, hi, lane);
     }
-    if (have) DC_KEEP();
-#undef DC_KEEP
-    sh.nK[lo] = K;
-    sh.nT1[lo] = c0;
-    sh.nT12[lo] = c0 + c1;
-    sh.nRot[lo] = nr;
-    sh.nRho[lo] = rho;
+  } else if (tid < m && sh.nlo[tid] == tid) {
+    int lo2, hi;
+    dc_node(tid, dd, m, lo2, hi);
+    DcScan st;
+    const int mid = tid + ((hi - tid) >> 1);
+    const double rho = 2.0 * fabs(ev[mid - 1]);
+    dc_scan_seq(sh, tid, hi, mid, rho, dc_tol(sh, tid, rho), tid, st);
+    dc_scan_finish(sh, tid, mid, rho, st);
   }
   __syncthreads();
-  // deflation rotations on the rows of Qin (row-parallel); secular roots (four lanes each)
+  // deflation rotations on the rows of Qin (row-parallel); the secular roots, z from the roots, norms
   if (tid < DC_SEC) {
     const int r = tid;
     if (r < m && sh.nlo[r] >= 0) {
@@ -497,73 +676,12 @@ __device__ void dc_level(DcShared& sh, const double* __restrict__ ev, int m, int
         Qin[ib] = -s * x + c * y;
       }
     }
-  } else {
-    const int g = (tid - DC_SEC) >> 2, q = tid & 3;
-    if (g < m && sh.nlo[g] >= 0) {
-      const int lo = sh.nlo[g], j = g - lo, K = sh.nK[lo];
-      if (j < K) {
-        int o;
-        double t;
-        dc_secular(sh.dK + lo, sh.z2 + lo, K, j, q, o, t);
-        if (q == 0) {
-          sh.org[lo + j] = lo + o;
-          sh.tau[lo + j] = t;
-        }
-      }
-    }
   }
-  __syncthreads();
-  // z from the roots (Lowner's formula)
-  if (tid >= DC_SEC) {
-    const int g = (tid - DC_SEC) >> 2, q = tid & 3;
-    if (g < m && sh.nlo[g] >= 0) {
-      const int lo = sh.nlo[g], i = g - lo, K = sh.nK[lo];
-      if (i < K) {
-        const double* dk = sh.dK + lo;
-        const double di = dk[i], rho = sh.nRho[lo];
-        double p = 1.0;
-        for (int jj = q; jj < K; jj += 4) {
-          const double lmd = (sh.dK[sh.org[lo + jj]] - di) + sh.tau[lo + jj];
-          const double den = jj == K - 1 ? rho : (jj < i ? dk[jj] - di : dk[jj + 1] - di);
-          p *= lmd / den;
-        }
-        p = prod4(p);
-        if (q == 0) {
-          sh.zh[lo + i] = copysign(sqrt(fmax(p, 0.0)), sh.zK[lo + i]);
-          const int ty = sh.ktype[lo + i];
-          const int pos = sh.upos[lo + i] + (ty == 0 ? 0 : (ty == 1 ? sh.nT1[lo] : sh.nT12[lo]));
-          sh.upos[lo + i] = pos;
-          sh.kcp[lo + pos] = sh.kc[lo + i];
-        }
-      }
-    }
-  }
-  __syncthreads();
-  // norms of the vectors u_j; the new eigenvalues
-  if (tid >= DC_SEC) {
-    const int g = (tid - DC_SEC) >> 2, q = tid & 3;
-    if (g < m && sh.nlo[g] >= 0) {
-      const int lo = sh.nlo[g], j = g - lo, K = sh.nK[lo];
-      if (j < K) {
-        const double dorg = sh.dK[sh.org[lo + j]], tj = sh.tau[lo + j];
-        double s = 0.0;
-        for (int i = q; i < K; i += 4) {
-          const double u = sh.zh[lo + i] / ((sh.dK[lo + i] - dorg) - tj);
-          s += u * u;
-        }
-        s = sum4(s);
-        if (q == 0) {
-          sh.rn[lo + j] = 1.0 / sqrt(s);
-          sh.lam[lo + j] = dorg + tj;
-        }
-      } else if (q == 0) {
-        sh.lam[g] = sh.dvl[g];
-      }
-    }
-  }
-  __syncthreads();
-  // the K x K eigenvector matrices: items (row r, column t) inside the node blocks
   const int smax = (m + (1 << dd) - 1) >> dd;   // the widest node of this level
+  // narrow merges: one lane per root; wide ones: four lanes share each root's sums
+  if (smax <= DC_ONE_LANE) dc_roots<1>(sh, m, tid);
+  else dc_roots<4>(sh, m, tid);
+  // the K x K eigenvector matrices: items (row r, column t) inside the node blocks
   for (int it = tid; it < m * smax; it += DC_NT) {
     const int r = it / smax, t = it % smax;
     const int lo = sh.nlo[r];
@@ -625,18 +743,18 @@ __device__ void dc_level(DcShared& sh, const double* __restrict__ ev, int m, int
       const int ra = r0 + li;   // A-fragment row of this lane (A[li][lk])
       const int tb = t0 + li;   // B-fragment column of this lane (B[lk][li])
       d4 acc = {0.0, 0.0, 0.0, 0.0};
-      // four k-steps per pass: all eight operand loads are in flight before the first product
-      for (int p = p0; p < p1; p += 16) {
-        double a[4], b[4];
+      // eight k-steps per pass: all sixteen operand loads are in flight before the first product
+      for (int p = p0; p < p1; p += 32) {
+        double a[8], b[8];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
+        for (int u = 0; u < 8; ++u) {
           const int pk = p + 4 * u + lk;
           const bool okp = pk < p1;
           a[u] = (okp && ra < hi) ? Qin[ra + (size_t)sh.kcp[lo + (okp ? pk : p0)] * m] : 0.0;
           b[u] = (okp && tb < K) ? U[(lo + tb) + (size_t)(lo + pk) * m] : 0.0;
         }
 #pragma unroll
-        for (int u = 0; u < 4; ++u) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u], b[u], acc, 0, 0, 0);
+        for (int u = 0; u < 8; ++u) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u], b[u], acc, 0, 0, 0);
       }
       // output lane layout of the 16 x 16 x 4 FP64 product: register q holds row lk + 4 q, column li
 #pragma unroll
