@@ -129,7 +129,7 @@ def roofline_leg(tnad, ctx, chi, Dd):
 CU_FP64_GFLOPS = 78.6e3 / 256
 
 
-def eigensolver_leg(tnad, ctx, m=176):
+def eigensolver_leg(tnad, ctx, m=192):
     """The exact eigensolver of the Rayleigh-Ritz steps (divide and conquer, m = the subspace block at
     chi = 128), its stages timed alone with HIP events on the library stream.  The Householder
     tridiagonalisation (one workgroup, the matrix in registers) has 4/3 m^3 algorithmic flops; it is bound by

@@ -60,7 +60,7 @@ typedef struct tnad_opts {
   int precision;       /* contraction precision of the low-precision phase (TNAD_PREC_*) */
   double switch_tol;   /* precision != FP64: switch to FP64 once |vals - oldvals| <= switch_tol */
   int bwd_precision;   /* contraction precision of the reverse sweep */
-  int eig_block;       /* subspace block size k for chi*D > 192 (0 = 2.25 chi, capped at 176 for chi <= 128, else at 192) */
+  int eig_block;       /* subspace block size k for chi*D > 192 (0 = 2.25 chi, capped at 192) */
   int eig_max_iters;   /* cap on subspace iterations per step, cold start (stops at eig_tol) */
   int eig_warm;        /* 1: start each step's eigensolver from the previous step's Ritz vectors */
   int eig_warm_iters;  /* cap on subspace iterations per step, warm start (stops at eig_tol) */
@@ -121,8 +121,8 @@ int tnad_svd_back_sym(tnad_ctx* ctx, const double* A, int n, int chi, int k, con
 
 /* ctmrgstep (src/ctmrg.jl:126-153): one step; vals has chi*D entries, s ./ s[1] sorted descending:
  * all of them when chi*D <= 192 (full spectrum, as the reference), otherwise the k largest followed by
- * zeros, k = eig_block or 2.25 chi capped at 176 (chi <= 128) or 192 (the chi kept values, resolved to
- * the eigensolver tolerance, then the eigensolver's guard block, whose Ritz values are approximate). */
+ * zeros, k = eig_block or 2.25 chi capped at 192 (the chi kept values, resolved to the eigensolver
+ * tolerance, then the eigensolver's guard block, whose Ritz values are approximate). */
 int tnad_ctmrgstep(tnad_ctx* ctx, const double* a, int D, int chi, const double* C, const double* T,
                    const tnad_opts* opts, double* Cout, double* Tout, double* vals);
 

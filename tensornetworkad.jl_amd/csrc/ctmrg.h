@@ -74,15 +74,12 @@ int topk_eig(Ctx* c, const double* A, int n, int want, int k, const double* X0, 
 // Subspace block of a step: n (full spectrum) for n <= 192; else the chi kept columns plus a guard of
 // 1.25 chi (2.25 chi in all), or eig_block when set.  The guard sets the convergence rate
 // |lambda_(k+1) / lambda_chi| per iteration: on random normal D=3 bulks at chi=64 a block of 96
-// (1.5 chi) took ~40 iterations per step, 144 about 2.5x fewer and 25% less time.  The automatic block
-// is capped at 176 up to chi = 128 (a guard of at least 3/8 chi), at 192 above: a Rayleigh-Ritz
-// step's Jacobi sweep costs ~k^3, and at chi = 128, D = 4 the energy + gradient ran 12% (mixed
-// precision) and 9% (FP64) faster at k = 176 than at 192 (168: 3-5% slower than 176; 184: slower
-// than 192, its Jacobi threads owning five 2x2 blocks instead of four).
+// (1.5 chi) took ~40 iterations per step, 144 about 2.5x fewer and 25% less time.  Capped at 192 (the
+// exact Rayleigh-Ritz solver's limit); at chi = 128, D = 4 the energy + gradient runs 3% (mixed
+// precision) and 4% (FP64) faster at k = 192 than at 176, and 6% / 18% faster than at 160.
 inline int step_k(const StepDims& d, const StepOpts& o) {
   if (d.n <= JACOBI_MAX) return d.n;
   int k = o.eig_block > 0 ? o.eig_block : 2 * d.chi + d.chi / 4;
-  if (o.eig_block <= 0 && k > 176 && d.chi <= 128) k = 176;
   if (k < d.chi) k = d.chi;
   if (k > JACOBI_MAX) k = JACOBI_MAX;
   if (k > d.n) k = d.n;

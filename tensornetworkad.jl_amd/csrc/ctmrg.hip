@@ -562,7 +562,7 @@ void ctmrg_step(Ctx* c, const StepDims& d, const StepOpts& o, const double* a64,
     k_sym_add(cp64.as<double>(), n, M, s);
   }
   // u, s, v = svd(cpmat) -> top-k eigenpairs by |lambda|        (src/ctmrg.jl:136)
-  // block size: n (full spectrum) or 2.25 chi guard-extended (eig_block if set), capped at 176 / 192
+  // block size: n (full spectrum) or 2.25 chi guard-extended (eig_block if set), capped at 192
   const int kreq = step_k(d, o);
   DevBuf Xtmp, AXtmp, thtmp;
   double *X = nullptr, *AX = nullptr, *theta = nullptr;

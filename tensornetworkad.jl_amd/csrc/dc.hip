@@ -1,8 +1,8 @@
 // Dense symmetric eigensolver for m <= 192 (FP64) in three stages, each a HIP kernel:
 //
-//  * trd_kernel : Householder reduction to tridiagonal form in one workgroup of 48 x 16 threads.  The
-//    matrix stays in registers in a cyclic distribution (thread (ty, tx) holds the 4 x 12 elements
-//    (ty + 48 a, tx + 16 b)), both triangles, so the product A v of each step is a per-thread partial
+//  * trd_kernel : Householder reduction to tridiagonal form in one workgroup of 32 x 16 threads.  The
+//    matrix stays in registers in a cyclic distribution (thread (ty, tx) holds the 6 x 12 elements
+//    (ty + 32 a, tx + 16 b)), both triangles, so the product A v of each step is a per-thread partial
 //    sum over its twelve columns finished by DPP butterflies across the 16 lanes of the same ty (one
 //    row of a wavefront), and the symmetric rank-two update is local.  Per step: the owner
 //    lanes publish column j through LDS, every wavefront forms the reflector from it (norm by a
@@ -36,7 +36,7 @@ namespace {
 
 typedef double d4 __attribute__((ext_vector_type(4)));
 
-constexpr int TD_GX = 16, TD_GY = 48;       // thread grid: 48 rows x 16 columns of threads (768)
+constexpr int TD_GX = 16, TD_GY = 32;       // thread grid: 32 rows x 16 columns of threads (512)
 constexpr int TD_NT = TD_GX * TD_GY;
 static_assert(TD_GX == 16 || TD_GX == 32, "row sums span one or two DPP rows");
 constexpr int TD_RB = JACOBI_MAX / TD_GX;   // column elements per thread (12)
@@ -121,17 +121,17 @@ __device__ __forceinline__ void trd_block(double (&A)[TD_RA][TD_RB], int m, TrdS
       const double x = col[r] * sc;
       return r == j + 1 ? 1.0 : ((r > j + 1 && r < m) ? x : 0.0);
     };
-    double vc[TD_RB];
+    // p = tau A v on the rows below j (finished row blocks are skipped and never stored); column by column,
+    // so only the matrix, the row sums and one v entry are live
+    double ps[TD_RA] = {};
 #pragma unroll
-    for (int b = BJ; b < TD_RB; ++b) vc[b] = vat(tx + TD_GX * b);
-    // p = tau A v on the rows below j (finished row blocks are skipped and never stored)
-    double ps[TD_RA];
+    for (int b = BJ; b < TD_RB; ++b) {
+      const double vcb = vat(tx + TD_GX * b);
 #pragma unroll
-    for (int a = 0; a < TD_RA; ++a) {
-      ps[a] = 0.0;
-      if (TD_GY * a + TD_GY - 1 <= j) continue;   // uniform: every row of the block is finished
-#pragma unroll
-      for (int b = BJ; b < TD_RB; ++b) ps[a] = fma(A[a][b], vc[b], ps[a]);
+      for (int a = 0; a < TD_RA; ++a) {
+        if (TD_GY * a + TD_GY - 1 <= j) continue;   // uniform: every row of the block is finished
+        ps[a] = fma(A[a][b], vcb, ps[a]);
+      }
     }
     row16_sums<TD_RA>(ps);
     if constexpr (TD_GX == 32) {   // rows of 32 lanes: row_bcast15 adds row 0 into row 1, row 2 into row 3
@@ -157,21 +157,24 @@ __device__ __forceinline__ void trd_block(double (&A)[TD_RA][TD_RB], int m, TrdS
     for (int r = j + 1 + lane; r < m; r += 64) pw += pv[r] * vat(r);
     pw = wave_sum(pw);
     const double kc = 0.5 * tv * pw;
-    double wc[TD_RB];
-    // p is written this step only for rows in unskipped blocks: columns at or left of j take w = 0
+    // the thread's rows of v and w, then the update column by column (p is written this step only for
+    // rows in unskipped blocks: columns at or left of j take w = 0)
+    double vr[TD_RA], wr[TD_RA];
+#pragma unroll
+    for (int a = 0; a < TD_RA; ++a) {
+      const int r = ty + TD_GY * a;
+      vr[a] = vat(r);
+      wr[a] = TD_GY * a + TD_GY - 1 <= j ? 0.0 : pv[r] - kc * vr[a];
+    }
 #pragma unroll
     for (int b = BJ; b < TD_RB; ++b) {
       const int c = tx + TD_GX * b;
-      const double x = pv[c] - kc * vc[b];
-      wc[b] = c > j ? x : 0.0;
-    }
+      const double vcb = vat(c), x = pv[c] - kc * vcb, wcb = c > j ? x : 0.0;
 #pragma unroll
-    for (int a = 0; a < TD_RA; ++a) {
-      if (TD_GY * a + TD_GY - 1 <= j) continue;
-      const int r = ty + TD_GY * a;
-      const double vr = vat(r), wr = pv[r] - kc * vr;
-#pragma unroll
-      for (int b = BJ; b < TD_RB; ++b) A[a][b] = fma(-wr, vc[b], fma(-vr, wc[b], A[a][b]));
+      for (int a = 0; a < TD_RA; ++a) {
+        if (TD_GY * a + TD_GY - 1 <= j) continue;
+        A[a][b] = fma(-wr[a], vcb, fma(-vr[a], wcb, A[a][b]));
+      }
     }
   }
 }
@@ -809,6 +812,7 @@ __global__ __launch_bounds__(DC_NT) void dc_kernel(const double* __restrict__ d,
   if (tid < m) lam_out[tid] = sh.lam[tid];
 }
 
+// W = H_0 H_1 ... H_(m-3) Z in place (Z = W), one wavefront per column
 // W = H_0 H_1 ... H_(m-3) Z in place (Z = W), one wavefront per column
 __global__ __launch_bounds__(64) void trd_back_kernel(const double* __restrict__ Vh, const double* __restrict__ tau,
                                                       int m, double* __restrict__ Z, const int* __restrict__ gate) {
