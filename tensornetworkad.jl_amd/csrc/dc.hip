@@ -813,7 +813,6 @@ __global__ __launch_bounds__(DC_NT) void dc_kernel(const double* __restrict__ d,
 }
 
 // W = H_0 H_1 ... H_(m-3) Z in place (Z = W), one wavefront per column
-// W = H_0 H_1 ... H_(m-3) Z in place (Z = W), one wavefront per column
 __global__ __launch_bounds__(64) void trd_back_kernel(const double* __restrict__ Vh, const double* __restrict__ tau,
                                                       int m, double* __restrict__ Z, const int* __restrict__ gate) {
   if (gate && *gate == 0) return;
