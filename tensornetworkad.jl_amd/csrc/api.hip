@@ -318,7 +318,7 @@ void run_ctmrg(tnad_ctx* ctx, tnad_tape& t, const double* C0, const double* T0, 
 // at tnad_create.
 // ---------------------------------------------------------------------------
 namespace {
-constexpr size_t kScratchDoubles = 4 * 192 * 192 + 4 * 192 + 8;   // syev_dc_work_doubles(192)
+constexpr size_t kScratchDoubles = 4 * 192 * 192 + 6 * 192 + 48;   // syev_dc_work_doubles(192)
 constexpr int kMaxDevices = 64;
 std::mutex g_scratch_mu;
 double* g_scratch[kMaxDevices] = {};   // device address of `scratch` on each device

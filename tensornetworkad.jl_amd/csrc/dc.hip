@@ -1,4 +1,4 @@
-// Dense symmetric eigensolver for m <= 192 (FP64) in three stages, each a HIP kernel:
+// Dense symmetric eigensolver for m <= 192 (FP64) in four stages, each a HIP kernel:
 //
 //  * trd_kernel : Householder reduction to tridiagonal form in one workgroup of 32 x 16 threads.  The
 //    matrix stays in registers in a cyclic distribution (thread (ty, tx) holds the 6 x 12 elements
@@ -25,7 +25,12 @@
 //    accuracy; the merged eigenvectors are the children's (block-diagonal) eigenvector matrix times
 //    that K x K matrix (deflated columns are copied).  The eigenvector matrices of a level live in
 //    two m x m ping-pong buffers in global memory (L2-resident), a level's K x K matrices in a third.
-//  * trd_back_kernel : W = H_0 H_1 ... H_(m-3) Z, one wavefront per column of Z (no barriers).
+//    The top merge stops short of its product: it leaves the children's eigenvectors Z in Qb and its
+//    m x m merge matrix M (the K x K matrix scattered to the rows of its columns of Z, a 1 for each
+//    deflated column) in U, since H (Z M) = (H Z) M lets the product run over the whole chip.
+//  * trd_back_kernel : Z <- H_0 H_1 ... H_(m-3) Z, one wavefront per column of Z (no barriers), four
+//    reflectors per wavefront reduction.
+//  * dc_final_kernel : W = Z M, a 16 x 16 tile of W per workgroup on the FP64 matrix cores.
 // The eigenpairs are then sorted by |lambda| descending with the sign convention of the Jacobi path.
 #include "tnad_internal.h"
 
@@ -38,17 +43,24 @@ typedef double d4 __attribute__((ext_vector_type(4)));
 
 constexpr int TD_GX = 16, TD_GY = 32;       // thread grid: 32 rows x 16 columns of threads (512)
 constexpr int TD_NT = TD_GX * TD_GY;
-static_assert(TD_GX == 16 || TD_GX == 32, "row sums span one or two DPP rows");
+static_assert(TD_GX == 8 || TD_GX == 16 || TD_GX == 32, "row sums span half, one or two DPP rows");
 constexpr int TD_RB = JACOBI_MAX / TD_GX;   // column elements per thread (12)
 constexpr int TD_RA = JACOBI_MAX / TD_GY;   // row elements per thread (4)
 static_assert(TD_RB * TD_GX == JACOBI_MAX && TD_RA * TD_GY == JACOBI_MAX, "tridiagonalisation tiling");
 
-// DPP lane move of a double (its two 32-bit halves); lanes outside row_mask keep x
+// DPP lane move of a double (its two 32-bit halves); lanes outside row_mask keep x.  With every row enabled
+// no old value is needed, so the move writes a fresh register (no copy of x ahead of it).
 template <int CTRL, int ROWMASK = 0xF>
 __device__ __forceinline__ double dpp_d(double x) {
   const long long b = __double_as_longlong(x);
-  const int lo = __builtin_amdgcn_update_dpp((int)b, (int)b, CTRL, ROWMASK, 0xF, false);
-  const int hi = __builtin_amdgcn_update_dpp((int)(b >> 32), (int)(b >> 32), CTRL, ROWMASK, 0xF, false);
+  int lo, hi;
+  if constexpr (ROWMASK == 0xF) {
+    lo = __builtin_amdgcn_mov_dpp((int)b, CTRL, 0xF, 0xF, false);
+    hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), CTRL, 0xF, 0xF, false);
+  } else {
+    lo = __builtin_amdgcn_update_dpp((int)b, (int)b, CTRL, ROWMASK, 0xF, false);
+    hi = __builtin_amdgcn_update_dpp((int)(b >> 32), (int)(b >> 32), CTRL, ROWMASK, 0xF, false);
+  }
   return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
@@ -64,6 +76,22 @@ __device__ __forceinline__ void row16_sums(double (&x)[N]) {
   for (int a = 0; a < N; ++a) x[a] += dpp_d<0x141>(x[a]);   // row_half_mirror
 #pragma unroll
   for (int a = 0; a < N; ++a) x[a] += dpp_d<0x140>(x[a]);   // row_mirror
+}
+
+// totals of N values over the wavefront, in every lane (the N butterflies interleave)
+template <int N>
+__device__ __forceinline__ void wave_sums(double (&x)[N]) {
+  row16_sums<N>(x);
+#pragma unroll
+  for (int k = 0; k < N; ++k) x[k] += dpp_d<0x142, 0xA>(x[k]);
+#pragma unroll
+  for (int k = 0; k < N; ++k) x[k] += dpp_d<0x143, 0x8>(x[k]);
+#pragma unroll
+  for (int k = 0; k < N; ++k) {
+    const long long b = __double_as_longlong(x[k]);
+    const int lo = __builtin_amdgcn_readlane((int)b, 63), hi = __builtin_amdgcn_readlane((int)(b >> 32), 63);
+    x[k] = __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+  }
 }
 
 // sum over the wavefront, the same value in every lane
@@ -85,18 +113,20 @@ struct TrdShared {
 
 // Householder steps j = TD_GX BJ .. min(TD_GX (BJ + 1) - 1, m - 3).  Columns below TD_GX BJ are finished (they lie
 // left of every remaining step's column), so only column blocks b >= BJ are touched; row blocks whose
-// rows all lie at or above j are skipped by a uniform test.
+// rows all lie at or above j are skipped: those finished before the block's first step statically (A0),
+// the others by a uniform test.
 template <int BJ>
 __device__ __forceinline__ void trd_block(double (&A)[TD_RA][TD_RB], int m, TrdShared& sh, int tx, int ty,
                                           int lane, int tid, double* __restrict__ Vh, double* __restrict__ tauo,
                                           double* __restrict__ eo) {
+  constexpr int A0 = (BJ * TD_GX + 1) / TD_GY;   // row blocks a < A0 end at or above row BJ TD_GX
   const int j0 = BJ * TD_GX, j1 = min(m - 2, j0 + TD_GX);
   for (int j = j0; j < j1; ++j) {
     double* col = sh.col[j & 1];
     double* pv = sh.p[j & 1];
     if (tx == j - j0) {
 #pragma unroll
-      for (int a = 0; a < TD_RA; ++a) col[ty + TD_GY * a] = A[a][BJ];
+      for (int a = A0; a < TD_RA; ++a) col[ty + TD_GY * a] = A[a][BJ];
     }
     __syncthreads();
     // reflector H = I - tau v v^T with v[j+1] = 1 that maps A[j+1:, j] to (beta, 0, ..., 0)
@@ -116,7 +146,7 @@ __device__ __forceinline__ void trd_block(double (&A)[TD_RA][TD_RB], int m, TrdS
       tv = -d * d * inv;
       sc = be * inv;
     }
-    // v[r] for any r < 192 (every row of col is written each step, so the read needs no guard)
+    // v[r] for any r < 192 (rows of finished blocks hold stale values, which the selects drop)
     auto vat = [&](int r) -> double {
       const double x = col[r] * sc;
       return r == j + 1 ? 1.0 : ((r > j + 1 && r < m) ? x : 0.0);
@@ -128,21 +158,31 @@ __device__ __forceinline__ void trd_block(double (&A)[TD_RA][TD_RB], int m, TrdS
     for (int b = BJ; b < TD_RB; ++b) {
       const double vcb = vat(tx + TD_GX * b);
 #pragma unroll
-      for (int a = 0; a < TD_RA; ++a) {
+      for (int a = A0; a < TD_RA; ++a) {
         if (TD_GY * a + TD_GY - 1 <= j) continue;   // uniform: every row of the block is finished
         ps[a] = fma(A[a][b], vcb, ps[a]);
       }
     }
-    row16_sums<TD_RA>(ps);
-    if constexpr (TD_GX == 32) {   // rows of 32 lanes: row_bcast15 adds row 0 into row 1, row 2 into row 3
+    // row sums over the TD_GX lanes of each row: pairs, quads, half rows (8 lanes), rows (16), row pairs (32)
 #pragma unroll
-      for (int a = 0; a < TD_RA; ++a) ps[a] += dpp_d<0x142, 0xA>(ps[a]);
+    for (int a = A0; a < TD_RA; ++a) ps[a] += dpp_d<0xB1>(ps[a]);
+#pragma unroll
+    for (int a = A0; a < TD_RA; ++a) ps[a] += dpp_d<0x4E>(ps[a]);
+#pragma unroll
+    for (int a = A0; a < TD_RA; ++a) ps[a] += dpp_d<0x141>(ps[a]);
+    if constexpr (TD_GX >= 16) {
+#pragma unroll
+      for (int a = A0; a < TD_RA; ++a) ps[a] += dpp_d<0x140>(ps[a]);
+    }
+    if constexpr (TD_GX == 32) {   // row_bcast15 adds row 0 into row 1, row 2 into row 3
+#pragma unroll
+      for (int a = A0; a < TD_RA; ++a) ps[a] += dpp_d<0x142, 0xA>(ps[a]);
     }
 #pragma unroll
-    for (int a = 0; a < TD_RA; ++a) {
+    for (int a = A0; a < TD_RA; ++a) {
       if (TD_GY * a + TD_GY - 1 <= j) continue;
       const int r = ty + TD_GY * a;
-      if (tx == TD_GX - 16) pv[r] = (r > j && r < m) ? tv * ps[a] : 0.0;   // a lane that holds the row total
+      if (tx == (TD_GX == 32 ? 16 : 0)) pv[r] = (r > j && r < m) ? tv * ps[a] : 0.0;   // a lane that holds the row total
     }
     if (tid < 64) {
       for (int r = lane; r < m; r += 64) Vh[r + (size_t)j * m] = vat(r);
@@ -161,7 +201,7 @@ __device__ __forceinline__ void trd_block(double (&A)[TD_RA][TD_RB], int m, TrdS
     // rows in unskipped blocks: columns at or left of j take w = 0)
     double vr[TD_RA], wr[TD_RA];
 #pragma unroll
-    for (int a = 0; a < TD_RA; ++a) {
+    for (int a = A0; a < TD_RA; ++a) {
       const int r = ty + TD_GY * a;
       vr[a] = vat(r);
       wr[a] = TD_GY * a + TD_GY - 1 <= j ? 0.0 : pv[r] - kc * vr[a];
@@ -171,7 +211,7 @@ __device__ __forceinline__ void trd_block(double (&A)[TD_RA][TD_RB], int m, TrdS
       const int c = tx + TD_GX * b;
       const double vcb = vat(c), x = pv[c] - kc * vcb, wcb = c > j ? x : 0.0;
 #pragma unroll
-      for (int a = 0; a < TD_RA; ++a) {
+      for (int a = A0; a < TD_RA; ++a) {
         if (TD_GY * a + TD_GY - 1 <= j) continue;
         A[a][b] = fma(-wr[a], vcb, fma(-vr[a], wcb, A[a][b]));
       }
@@ -618,6 +658,8 @@ __device__ void dc_roots(DcShared& sh, int m, int tid) {
 __device__ void dc_level(DcShared& sh, const double* __restrict__ ev, int m, int dd, double* __restrict__ Qin,
                          double* __restrict__ Qout, double* __restrict__ U, int tid) {
   // children's eigenpairs -> z, sorted d of every merge node
+  if (dd == 0)   // the top merge leaves its merge matrix in U (zero outside its entries)
+    for (int idx = tid; idx < m * m; idx += DC_NT) U[idx] = 0.0;
   if (tid < m) {
     const int g = tid;
     int lo, hi;
@@ -692,11 +734,18 @@ __device__ void dc_level(DcShared& sh, const double* __restrict__ ev, int m, int
     const int K = sh.nK[lo], i = r - lo;
     if (i >= K || t >= K) continue;
     const double u = sh.zh[lo + i] / ((sh.dK[lo + i] - sh.dK[sh.org[lo + t]]) - sh.tau[lo + t]) * sh.rn[lo + t];
-    U[(lo + t) + (size_t)(lo + sh.upos[lo + i]) * m] = u;   // row p = upos of the K x K matrix, stored transposed
+    if (dd == 0) U[t + (size_t)sh.kc[i] * m] = u;   // M(kc_i, t), stored transposed
+    else U[(lo + t) + (size_t)(lo + sh.upos[lo + i]) * m] = u;   // row p = upos of the K x K matrix, stored transposed
   }
   if (tid < m) {
     sh.nmd[tid] = 0;
     sh.nmz[tid] = 0;
+  }
+  if (dd == 0) {   // deflated columns: M(dcol_t, t) = 1; the product Qin M is left to dc_final_kernel
+    const int K = sh.nK[0];
+    if (tid >= K && tid < m) U[tid + (size_t)sh.dcol[tid] * m] = 1.0;
+    __syncthreads();
+    return;
   }
   __syncthreads();
   // Qout = Qin[:, kept] U, deflated columns copied.  Levels of narrow merges: one output per item.
@@ -745,19 +794,26 @@ __device__ void dc_level(DcShared& sh, const double* __restrict__ ev, int m, int
       const int p1 = r0 + 15 < mid ? sh.nT12[lo] : K;
       const int ra = r0 + li;   // A-fragment row of this lane (A[li][lk])
       const int tb = t0 + li;   // B-fragment column of this lane (B[lk][li])
+      // clamped addresses, no branches: rows and columns beyond the node only feed outputs that are not
+      // stored, and the k-steps beyond p1 are zeroed through b
+      const int rac = min(ra, hi - 1), tbc = min(tb, K - 1);
       d4 acc = {0.0, 0.0, 0.0, 0.0};
-      // eight k-steps per pass: all sixteen operand loads are in flight before the first product
+      // eight k-steps per pass: the column indices first (one LDS wait), then all sixteen operand loads in
+      // flight before the first product
       for (int p = p0; p < p1; p += 32) {
+        int kk[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) kk[u] = sh.kcp[lo + min(p + 4 * u + lk, p1 - 1)];
         double a[8], b[8];
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
-          const int pk = p + 4 * u + lk;
-          const bool okp = pk < p1;
-          a[u] = (okp && ra < hi) ? Qin[ra + (size_t)sh.kcp[lo + (okp ? pk : p0)] * m] : 0.0;
-          b[u] = (okp && tb < K) ? U[(lo + tb) + (size_t)(lo + pk) * m] : 0.0;
+          a[u] = Qin[rac + (size_t)kk[u] * m];
+          b[u] = U[(lo + tbc) + (size_t)(lo + min(p + 4 * u + lk, p1 - 1)) * m];
         }
+        __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-        for (int u = 0; u < 8; ++u) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u], b[u], acc, 0, 0, 0);
+        for (int u = 0; u < 8; ++u)
+          acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u], p + 4 * u + lk < p1 ? b[u] : 0.0, acc, 0, 0, 0);
       }
       // output lane layout of the 16 x 16 x 4 FP64 product: register q holds row lk + 4 q, column li
 #pragma unroll
@@ -776,7 +832,8 @@ __device__ void dc_level(DcShared& sh, const double* __restrict__ ev, int m, int
   __syncthreads();
 }
 
-// tridiagonal (d, e) -> eigenvalues lam (unsorted) and eigenvectors in Qa (m x m); Qb, U: m x m scratch
+// tridiagonal (d, e) -> eigenvalues lam (unsorted); the eigenvectors are Qb M, with the top merge's matrix M
+// stored transposed in U (m x m each); Qa: m x m scratch
 __global__ __launch_bounds__(DC_NT) void dc_kernel(const double* __restrict__ d, const double* __restrict__ e, int m,
                                                    double* __restrict__ Qa, double* __restrict__ Qb,
                                                    double* __restrict__ U, double* __restrict__ lam_out,
@@ -785,9 +842,10 @@ __global__ __launch_bounds__(DC_NT) void dc_kernel(const double* __restrict__ d,
   __shared__ DcShared sh;
   const int tid = threadIdx.x;
   const int depth = dc_depth(m);
-  // the levels alternate between the buffers and end in Qa
-  double* Qin = (depth & 1) ? Qb : Qa;
-  double* Qout = (depth & 1) ? Qa : Qb;
+  // the levels alternate between the buffers; the top merge reads Qb (as does dc_final_kernel when m = 1)
+  const bool odd = depth > 0 && ((depth - 1) & 1);
+  double* Qin = odd ? Qa : Qb;
+  double* Qout = odd ? Qb : Qa;
   for (int idx = tid; idx < m * m; idx += DC_NT) {
     const int r = idx % m, c = idx / m;
     Qin[idx] = r == c ? 1.0 : 0.0;
@@ -805,90 +863,250 @@ __global__ __launch_bounds__(DC_NT) void dc_kernel(const double* __restrict__ d,
   __syncthreads();
   for (int dd = depth - 1; dd >= 0; --dd) {
     dc_level(sh, e, m, dd, Qin, Qout, U, tid);
+    if (dd == 0) break;
     double* t = Qin;
     Qin = Qout;
     Qout = t;
   }
+  if (depth == 0 && tid == 0) U[0] = 1.0;   // m = 1: Qb = 1, M = 1
   if (tid < m) lam_out[tid] = sh.lam[tid];
 }
 
-// W = H_0 H_1 ... H_(m-3) Z in place (Z = W), one wavefront per column
-__global__ __launch_bounds__(64) void trd_back_kernel(const double* __restrict__ Vh, const double* __restrict__ tau,
-                                                      int m, double* __restrict__ Z, const int* __restrict__ gate) {
+// W = Z M for the top merge: Z = Qb (the children's eigenvectors, deflation-rotated and back-transformed),
+// M from U (transposed); a 16 x 16 tile of W per wavefront on the FP64 matrix cores, over the whole chip
+__global__ __launch_bounds__(64) void dc_final_kernel(const double* __restrict__ Z, const double* __restrict__ Mt,
+                                                      int m, double* __restrict__ W, const int* __restrict__ gate) {
+  if (gate && *gate == 0) return;
+  const int lane = threadIdx.x, li = lane & 15, lk = lane >> 4;
+  const int r0 = 16 * blockIdx.x, t0 = 16 * blockIdx.y;
+  const int ra = r0 + li, tb = t0 + li;
+  // clamped addresses, no branches: rows and columns beyond m only feed outputs that are not stored, and
+  // the k-steps beyond m are zeroed through b
+  const int rac = min(ra, m - 1), tbc = min(tb, m - 1);
+  d4 acc = {0.0, 0.0, 0.0, 0.0};
+  for (int p = 0; p < m; p += 64) {
+    double a[16], b[16];
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+      const int pk = min(p + 4 * u + lk, m - 1);
+      a[u] = Z[rac + (size_t)pk * m];
+      b[u] = Mt[tbc + (size_t)pk * m];
+    }
+    __builtin_amdgcn_sched_barrier(0);   // all 32 loads in flight before the first product
+#pragma unroll
+    for (int u = 0; u < 16; ++u)
+      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u], p + 4 * u + lk < m ? b[u] : 0.0, acc, 0, 0, 0);
+  }
+  // register q holds row lk + 4 q, column li
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int r = r0 + lk + 4 * q;
+    if (r < m && tb < m) W[r + (size_t)tb * m] = acc[q];
+  }
+}
+
+// Z <- H_0 H_1 ... H_(m-3) Z in place, one wavefront per column.  The reflectors are applied four at a
+// time: with s_q = tau_q (u_q . z - sum_(p<q) s_p u_q . u_p) for the block's reflectors u_0..u_3 in the
+// order of application, z - sum_q s_q u_q is the product of the four; the four products with z share one
+// reduction, and the six among the u come from trd_gram_kernel (lanes 0..5 of one register per block).
+// Each lane needs only its own entries of the reflectors (rows lane, lane + 64, lane + 128), so they are
+// prefetched in chunks of 16 reflectors into registers, one chunk ahead (two register sets, the loop
+// unrolled by two); the reflector count is padded to whole pairs of chunks with tau = 0.
+constexpr int BK_Q = 16;
+
+// the six products u_q . u_p (p < q) among each block of four reflectors, in the order the back-transformation
+// uses them (u1.u0, u2.u0, u2.u1, u3.u0, u3.u1, u3.u2), once for all columns: one wavefront per block, over
+// the padded reflector count (a padding reflector is column 0 again, finite, with tau = 0)
+__global__ __launch_bounds__(64) void trd_gram_kernel(const double* __restrict__ Vh, int m, double* __restrict__ gram,
+                                                      const int* __restrict__ gate) {
+  if (gate && *gate == 0) return;
+  const int b = blockIdx.x, lane = threadIdx.x;
+  double u[4][3];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int j = m - 3 - (4 * b + q);
+    const double* v = Vh + (size_t)(j >= 0 ? j : 0) * m;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const int r = lane + 64 * k;
+      const double x = v[r < m ? r : m - 1];
+      u[q][k] = r < m ? x : 0.0;
+    }
+  }
+  double d[6];
+  int n = 0;
+#pragma unroll
+  for (int q = 1; q < 4; ++q)
+#pragma unroll
+    for (int p = 0; p < q; ++p, ++n) d[n] = u[q][0] * u[p][0] + u[q][1] * u[p][1] + u[q][2] * u[p][2];
+  wave_sums<6>(d);
+  if (lane < 6) {
+    double x = d[0];
+#pragma unroll
+    for (int k = 1; k < 6; ++k) x = lane == k ? d[k] : x;
+    gram[6 * b + lane] = x;
+  }
+}
+
+// lane k of a double held by lanes 0..5
+__device__ __forceinline__ double lane_d(double x, int k) {
+  const long long b = __double_as_longlong(x);
+  const int lo = __builtin_amdgcn_readlane((int)b, k), hi = __builtin_amdgcn_readlane((int)(b >> 32), k);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+__device__ __forceinline__ void back_load(const double* __restrict__ Vh, const double* __restrict__ tau,
+                                          const double* __restrict__ gram, int m, int q0, int lane,
+                                          double (&u)[BK_Q][3], double (&t)[BK_Q], double (&g)[BK_Q / 4]) {
+  const int nblk = ((m - 2 + 2 * BK_Q - 1) / (2 * BK_Q)) * (BK_Q / 2);   // blocks trd_gram_kernel wrote
+#pragma unroll
+  for (int b = 0; b < BK_Q / 4; ++b)   // lane k: product k (a prefetch past the last block rereads it, unused)
+    g[b] = gram[6 * min(q0 / 4 + b, nblk - 1) + (lane < 6 ? lane : 5)];
+  // raw loads from clamped addresses (no select on a loaded value here, which would wait for it): a padding
+  // reflector gets tau = 0, so whatever it holds has weight 0, and rows >= m are masked where they are used
+#pragma unroll
+  for (int i = 0; i < BK_Q; ++i) {
+    const int j = m - 3 - (q0 + i);   // reflectors in the order of application
+    const bool ok = j >= 0;
+    const double tj = tau[ok ? j : 0];
+    t[i] = ok ? tj : 0.0;
+    const double* v = Vh + (size_t)(ok ? j : 0) * m;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const int r = lane + 64 * k;
+      u[i][k] = v[r < m ? r : m - 1];
+    }
+  }
+}
+
+__device__ __forceinline__ void back_apply(double (&z)[3], const double (&uraw)[BK_Q][3], const double (&t)[BK_Q],
+                                           const double (&g)[BK_Q / 4], const bool (&rk)[3]) {
+#pragma unroll
+  for (int b = 0; b < BK_Q; b += 4) {
+    double u[BK_Q][3];
+#pragma unroll
+    for (int q = b; q < b + 4; ++q)
+#pragma unroll
+      for (int k = 0; k < 3; ++k) u[q][k] = rk[k] ? uraw[q][k] : 0.0;
+    double d[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) d[q] = u[b + q][0] * z[0] + u[b + q][1] * z[1] + u[b + q][2] * z[2];
+    wave_sums<4>(d);
+    const double gb = g[b / 4];
+    const double g10 = lane_d(gb, 0), g20 = lane_d(gb, 1), g21 = lane_d(gb, 2), g30 = lane_d(gb, 3),
+                 g31 = lane_d(gb, 4), g32 = lane_d(gb, 5);
+    const double s0 = t[b] * d[0];
+    const double s1 = t[b + 1] * (d[1] - s0 * g10);
+    const double s2 = t[b + 2] * (d[2] - s0 * g20 - s1 * g21);
+    const double s3 = t[b + 3] * (d[3] - s0 * g30 - s1 * g31 - s2 * g32);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) z[k] -= s0 * u[b][k] + s1 * u[b + 1][k] + s2 * u[b + 2][k] + s3 * u[b + 3][k];
+  }
+}
+
+// one wavefront per SIMD is all the grid has, so the scheduler may spend every register on the prefetch
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 1))) void trd_back_kernel(
+    const double* __restrict__ Vh, const double* __restrict__ tau, const double* __restrict__ gram, int m,
+    double* __restrict__ Z, const int* __restrict__ gate) {
   if (gate && *gate == 0) return;
   const int c = blockIdx.x, lane = threadIdx.x;
   double* zc = Z + (size_t)c * m;
   double z[3];
 #pragma unroll
   for (int k = 0; k < 3; ++k) z[k] = lane + 64 * k < m ? zc[lane + 64 * k] : 0.0;
-  for (int j = m - 3; j >= 0; --j) {
-    const double t = tau[j];
-    if (t == 0.0) continue;
-    const double* v = Vh + (size_t)j * m;
-    double vv[3];
-#pragma unroll
-    for (int k = 0; k < 3; ++k) vv[k] = lane + 64 * k < m ? v[lane + 64 * k] : 0.0;
-    double s = vv[0] * z[0] + vv[1] * z[1] + vv[2] * z[2];
-    s = t * wave_sum(s);
-#pragma unroll
-    for (int k = 0; k < 3; ++k) z[k] -= s * vv[k];
+  const int nr = m - 2;
+  const bool rk[3] = {lane < m, lane + 64 < m, lane + 128 < m};
+  double ua[BK_Q][3], ta[BK_Q], ga[BK_Q / 4], ub[BK_Q][3], tb[BK_Q], gb[BK_Q / 4];
+  back_load(Vh, tau, gram, m, 0, lane, ua, ta, ga);
+  // whole pairs of chunks, no exit between a chunk's loads and their use (a conditional one would let the
+  // compiler sink the loads past the other chunk's work)
+  // (the scheduling barriers keep the machine scheduler from moving the loads down to their uses)
+  for (int q = 0; q < nr; q += 2 * BK_Q) {
+    back_load(Vh, tau, gram, m, q + BK_Q, lane, ub, tb, gb);
+    __builtin_amdgcn_sched_barrier(0);
+    back_apply(z, ua, ta, ga, rk);
+    back_load(Vh, tau, gram, m, q + 2 * BK_Q, lane, ua, ta, ga);
+    __builtin_amdgcn_sched_barrier(0);
+    back_apply(z, ub, tb, gb, rk);
   }
 #pragma unroll
   for (int k = 0; k < 3; ++k)
     if (lane + 64 * k < m) zc[lane + 64 * k] = z[k];
 }
 
+// the work buffer: Vh, Qa, Qb, U (m x m each), d, e, tau, lambda (m each), the reflector block products
+struct DcWork {
+  double *Vh, *Qa, *Qb, *U, *dv, *ev, *tau, *lam, *gram;
+};
+
+DcWork dc_work(double* w, int m) {
+  const size_t mm = (size_t)m * m;
+  DcWork d;
+  d.Vh = w;
+  d.Qa = d.Vh + mm;
+  d.Qb = d.Qa + mm;
+  d.U = d.Qb + mm;
+  d.dv = d.U + mm;
+  d.ev = d.dv + m;
+  d.tau = d.ev + m;
+  d.lam = d.tau + m;
+  d.gram = d.lam + m;   // 6 per block of four reflectors, the count padded to 32: at most 1.5 m + 44
+  return d;
+}
+
+// blocks of four reflectors that trd_back_kernel reads (m - 2 padded to a multiple of 32)
+int dc_gram_blocks(int m) { return m > 2 ? ((m - 2 + 2 * BK_Q - 1) / (2 * BK_Q)) * (BK_Q / 2) : 0; }
+
+// everything after the tridiagonalisation: divide and conquer, block products, back-transformation of the
+// children's eigenvectors, the top merge's product (into Qa)
+void dc_stages(const DcWork& w, int m, const int* gate, hipStream_t s) {
+  hipLaunchKernelGGL(dc_kernel, dim3(1), dim3(DC_NT), 0, s, w.dv, w.ev, m, w.Qa, w.Qb, w.U, w.lam, gate);
+  TNAD_HIP_TRY(hipGetLastError());
+  if (m > 2) {
+    hipLaunchKernelGGL(trd_gram_kernel, dim3(dc_gram_blocks(m)), dim3(64), 0, s, w.Vh, m, w.gram, gate);
+    TNAD_HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(trd_back_kernel, dim3(m), dim3(64), 0, s, w.Vh, w.tau, w.gram, m, w.Qb, gate);
+    TNAD_HIP_TRY(hipGetLastError());
+  }
+  const int nt = (m + 15) / 16;
+  hipLaunchKernelGGL(dc_final_kernel, dim3(nt, nt), dim3(64), 0, s, w.Qb, w.U, m, w.Qa, gate);
+  TNAD_HIP_TRY(hipGetLastError());
+}
+
 }  // namespace
 
-size_t syev_dc_work_doubles(int m) { return 4 * (size_t)m * m + 4 * (size_t)m + 8; }
+size_t syev_dc_work_doubles(int m) { return 4 * (size_t)m * m + 6 * (size_t)m + 48; }
 
 void syev_dc(const double* H, int ldh, int m, double* evals, double* V, double* work, hipStream_t s,
              const int* gate) {
   if (m < 1 || m > JACOBI_MAX) throw Error(TNAD_ERR_ARG, "syev_dc: size out of range");
-  const size_t mm = (size_t)m * m;
-  double* Vh = work;
-  double* Qa = Vh + mm;
-  double* Qb = Qa + mm;
-  double* U = Qb + mm;
-  double* dv = U + mm;
-  double* ev = dv + m;
-  double* tau = ev + m;
-  double* lam = tau + m;
-  hipLaunchKernelGGL(trd_kernel, dim3(1), dim3(TD_NT), 0, s, H, ldh, m, dv, ev, Vh, tau, gate);
+  const DcWork w = dc_work(work, m);
+  hipLaunchKernelGGL(trd_kernel, dim3(1), dim3(TD_NT), 0, s, H, ldh, m, w.dv, w.ev, w.Vh, w.tau, gate);
   TNAD_HIP_TRY(hipGetLastError());
-  hipLaunchKernelGGL(dc_kernel, dim3(1), dim3(DC_NT), 0, s, dv, ev, m, Qa, Qb, U, lam, gate);
-  TNAD_HIP_TRY(hipGetLastError());
-  if (m > 2) {
-    hipLaunchKernelGGL(trd_back_kernel, dim3(m), dim3(64), 0, s, Vh, tau, m, Qa, gate);
-    TNAD_HIP_TRY(hipGetLastError());
-  }
-  sort_eigenpairs(lam, Qa, m, evals, V, gate, s);
+  dc_stages(w, m, gate, s);
+  sort_eigenpairs(w.lam, w.Qa, m, evals, V, gate, s);
 }
 
 void syev_dc_time(const double* H, int m, int reps, double* work, hipStream_t s, float ms[3]) {
   if (m < 3 || m > JACOBI_MAX || reps < 1) throw Error(TNAD_ERR_ARG, "syev_dc_time: bad sizes");
-  const size_t mm = (size_t)m * m;
-  double* Vh = work;
-  double* Qa = Vh + mm;
-  double* Qb = Qa + mm;
-  double* U = Qb + mm;
-  double* dv = U + mm;
-  double* ev = dv + m;
-  double* tau = ev + m;
-  double* lam = tau + m;
-  double* V = lam + m;   // m x m sorted eigenvectors
-  double* evals = V + mm;
+  const DcWork w = dc_work(work, m);
+  double* V = work + syev_dc_work_doubles(m);   // m x m sorted eigenvectors
+  double* evals = V + (size_t)m * m;
   hipEvent_t e[4];
   for (auto& x : e) TNAD_HIP_TRY(hipEventCreate(&x));
   float acc[3] = {0.f, 0.f, 0.f};
   for (int r = 0; r <= reps; ++r) {   // the first run warms up
     TNAD_HIP_TRY(hipEventRecord(e[0], s));
-    hipLaunchKernelGGL(trd_kernel, dim3(1), dim3(TD_NT), 0, s, H, m, m, dv, ev, Vh, tau, (const int*)nullptr);
+    hipLaunchKernelGGL(trd_kernel, dim3(1), dim3(TD_NT), 0, s, H, m, m, w.dv, w.ev, w.Vh, w.tau, (const int*)nullptr);
     TNAD_HIP_TRY(hipEventRecord(e[1], s));
-    hipLaunchKernelGGL(dc_kernel, dim3(1), dim3(DC_NT), 0, s, dv, ev, m, Qa, Qb, U, lam, (const int*)nullptr);
+    hipLaunchKernelGGL(dc_kernel, dim3(1), dim3(DC_NT), 0, s, w.dv, w.ev, m, w.Qa, w.Qb, w.U, w.lam,
+                       (const int*)nullptr);
     TNAD_HIP_TRY(hipEventRecord(e[2], s));
-    hipLaunchKernelGGL(trd_back_kernel, dim3(m), dim3(64), 0, s, Vh, tau, m, Qa, (const int*)nullptr);
-    sort_eigenpairs(lam, Qa, m, evals, V, nullptr, s);
+    hipLaunchKernelGGL(trd_gram_kernel, dim3(dc_gram_blocks(m)), dim3(64), 0, s, w.Vh, m, w.gram, (const int*)nullptr);
+    hipLaunchKernelGGL(trd_back_kernel, dim3(m), dim3(64), 0, s, w.Vh, w.tau, w.gram, m, w.Qb, (const int*)nullptr);
+    hipLaunchKernelGGL(dc_final_kernel, dim3((m + 15) / 16, (m + 15) / 16), dim3(64), 0, s, w.Qb, w.U, m, w.Qa,
+                       (const int*)nullptr);
+    sort_eigenpairs(w.lam, w.Qa, m, evals, V, nullptr, s);
     TNAD_HIP_TRY(hipEventRecord(e[3], s));
     TNAD_HIP_TRY(hipEventSynchronize(e[3]));
     if (r == 0) continue;

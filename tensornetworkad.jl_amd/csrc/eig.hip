@@ -467,21 +467,23 @@ __global__ __launch_bounds__(JS_NT) void jacobi_sort_kernel(const double* __rest
 }
 
 // Cholesky (with diagonal shift) + triangular inverse of an m x m SPD matrix, m <= 192,
-// in one workgroup of 32 x 32 threads.  Thread (ty, tx) holds the 6 x 6 elements
-// (ty + 32a, tx + 32b) in registers (cyclic distribution, so every step keeps all
-// threads busy).  One sweep of m elimination steps forms both L and B = L^-1: step jj
+// in one workgroup of 16 x 16 threads (one wavefront per SIMD: the per-step costs that every
+// thread pays, the pivot's reciprocal square root and the scaling of its column entries, are
+// then paid by 4 wavefronts instead of 16; 158 -> 140 us at m = 192).  Thread (ty, tx) holds
+// the 12 x 12 elements (ty + 16a, tx + 16b) in registers (cyclic distribution, so every step
+// keeps all threads busy).  One sweep of m elimination steps forms both L and B = L^-1: step jj
 // broadcasts column jj of the Schur complement and row jj of B (both through LDS, one
 // barrier per step, double-buffered); every thread then updates its part of the trailing
 // Schur complement (A -= l l^T) and of the rows of B below jj (B[i,:] -= L[i,jj] B[jj,:],
 // forward substitution for L B = I), and the owners finish column jj of L and row jj of B.
 // Only L's current column is ever needed, so finished columns of L are dropped, and the
-// rows of B in a 32-row block are final when the block's steps are done: they go straight
-// to the output.  The steps run in six 32-column blocks whose block index is a template
+// rows of B in a 16-row block are final when the block's steps are done: they go straight
+// to the output.  The steps run in twelve 16-column blocks whose block index is a template
 // parameter, so every register access is static and blocks that a step cannot touch
 // (finished columns, the upper triangle) are not even emitted.
 // Output Rinv = L^-T (upper triangular, column-major, zeros below the diagonal) so
 // that X = Y Rinv; piv (nullable) receives diag(L).
-constexpr int CH_G = 32;                  // thread grid edge
+constexpr int CH_G = 16;                  // thread grid edge
 constexpr int CH_R = JACOBI_MAX / CH_G;   // elements per thread per dimension
 static_assert(CH_R * CH_G == JACOBI_MAX, "Cholesky tiling");
 
@@ -500,7 +502,7 @@ __device__ __forceinline__ double rsqrt_refined(double d) {
   return y;
 }
 
-// elimination steps jj = 32 BJ .. min(32 BJ + 31, m - 1).  The owner column broadcasts the whole
+// elimination steps jj = 16 BJ .. min(16 BJ + 15, m - 1).  The owner column broadcasts the whole
 // column jj (zeros above the diagonal and in the padding), so the update needs no bounds tests: its
 // only stray products land in row jj above the diagonal (the unused upper triangle) and in column
 // jj, which the owner rewrites.  Likewise row jj of B is zero right of its diagonal, its stray
@@ -568,7 +570,7 @@ __device__ __forceinline__ void chol_block(double (&A)[CH_R][CH_R], double (&B)[
       for (int b = 0; b <= BJ; ++b) B[BJ][b] = xr[b];
     }
   }
-  // rows i = ty + 32 BJ of B are final: Rinv(j, i) = B(i, j), zero above B's diagonal
+  // rows i = ty + 16 BJ of B are final: Rinv(j, i) = B(i, j), zero above B's diagonal
   const int i = ty + CH_G * BJ;
   if (i < m) {
 #pragma unroll
@@ -577,6 +579,15 @@ __device__ __forceinline__ void chol_block(double (&A)[CH_R][CH_R], double (&B)[
       if (j < m) Rinv[j + (size_t)m * i] = (b <= BJ && j <= i) ? B[BJ][b <= BJ ? b : 0] : 0.0;
     }
   }
+}
+
+// the column blocks BJ, BJ + 1, ..., CH_R - 1 in turn
+template <int BJ>
+__device__ __forceinline__ void chol_blocks(double (&A)[CH_R][CH_R], double (&B)[CH_R][CH_R], int m,
+                                            const CholShared& sh, int tx, int ty, int t, double* __restrict__ piv,
+                                            double* __restrict__ Rinv) {
+  chol_block<BJ>(A, B, m, sh, tx, ty, t, piv, Rinv);
+  if constexpr (BJ + 1 < CH_R) chol_blocks<BJ + 1>(A, B, m, sh, tx, ty, t, piv, Rinv);
 }
 
 __global__ __launch_bounds__(CH_G * CH_G) void chol_inv_kernel(const double* __restrict__ G, int m,
@@ -602,12 +613,7 @@ __global__ __launch_bounds__(CH_G * CH_G) void chol_inv_kernel(const double* __r
   }
   if (t == 0) { bad = 0; pmin_s = HUGE_VAL; }
   __syncthreads();
-  chol_block<0>(A, B, m, sh, tx, ty, t, piv, Rinv);
-  chol_block<1>(A, B, m, sh, tx, ty, t, piv, Rinv);
-  chol_block<2>(A, B, m, sh, tx, ty, t, piv, Rinv);
-  chol_block<3>(A, B, m, sh, tx, ty, t, piv, Rinv);
-  chol_block<4>(A, B, m, sh, tx, ty, t, piv, Rinv);
-  chol_block<5>(A, B, m, sh, tx, ty, t, piv, Rinv);
+  chol_blocks<0>(A, B, m, sh, tx, ty, t, piv, Rinv);
   if (t == 0) {
     if (info) *info = bad;
     if (pmin_out) *pmin_out = pmin_s;
