@@ -109,7 +109,7 @@ int tnad_symeig(tnad_ctx* ctx, const double* A, int n, int want, int k, double* 
 /* tnad_symeig for n <= 192 (all n eigenpairs, sorted by |lambda| descending) with no allocation: no
  * counterpart in the reference; for callers that cannot allocate (code under HIP graph capture, tight
  * serving loops).  A (n x n), evals (n) and evecs (n x n) must be device memory on the context's GPU; the
- * work space (at most 1.19 MB, n = 192) is a file-scope device array of which each GPU holds one copy
+ * work space (at most 1.20 MB, n = 192) is a file-scope device array of which each GPU holds one copy
  * from load to process exit.  Calls that overlap on one GPU (two contexts used from two threads) race on
  * that array.  n > 192: TNAD_ERR_UNSUPPORTED. */
 int tnad_symeig_noalloc(tnad_ctx* ctx, const double* A, int n, double* evals, double* evecs);

@@ -312,13 +312,14 @@ void run_ctmrg(tnad_ctx* ctx, tnad_tape& t, const double* C0, const double* T0, 
 // Device scratch of the entry point that allocates nothing (tnad_symeig_noalloc): one file-scope
 // array per GPU, sized for tnad_symeig's full-spectrum solver (divide and conquer) at n = 192
 // (reflectors, two eigenvector buffers and the merge matrices, 4 x 192 x 192 doubles, plus the
-// tridiagonal matrix, tau and the eigenvalues: 148,232 doubles, 1.19 MB).  Every call on a GPU shares
+// tridiagonal matrix, tau, the eigenvalues, the reflector block products and the top merge's vectors:
+// 150,008 doubles, 1.20 MB).  Every call on a GPU shares
 // that GPU's copy, and every word a call reads there is written earlier in the same call.  Host code
 // never passes `scratch` itself to a kernel: its address on each device comes from hipGetSymbolAddress
 // at tnad_create.
 // ---------------------------------------------------------------------------
 namespace {
-constexpr size_t kScratchDoubles = 4 * 192 * 192 + 6 * 192 + 48;   // syev_dc_work_doubles(192)
+constexpr size_t kScratchDoubles = 4 * 192 * 192 + 13 * 192 + 56;   // syev_dc_work_doubles(192)
 constexpr int kMaxDevices = 64;
 std::mutex g_scratch_mu;
 double* g_scratch[kMaxDevices] = {};   // device address of `scratch` on each device

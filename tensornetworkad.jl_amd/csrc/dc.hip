@@ -5,10 +5,11 @@
 //    (ty + 32 a, tx + 16 b)), both triangles, so the product A v of each step is a per-thread partial
 //    sum over its twelve columns finished by DPP butterflies across the 16 lanes of the same ty (one
 //    row of a wavefront), and the symmetric rank-two update is local.  Per step: the owner
-//    lanes publish column j through LDS, every wavefront forms the reflector from it (norm by a
-//    wavefront reduction, no extra barrier), then p = tau A v through LDS, then the update; two
-//    barriers per step; column blocks that a step cannot touch are not emitted (template block index),
-//    finished row blocks are skipped.
+//    lanes publish column j through LDS, wavefront 0 forms the reflector from it and publishes v and
+//    tau (the other wavefronts wait at the barrier instead of repeating its reductions, square root
+//    and division), then p = tau A v through LDS, then the update; three barriers per step; column
+//    blocks that a step cannot touch are not emitted (template block index), finished row blocks are
+//    skipped.
 //    The reflectors (v with v[j+1] = 1, zeros above) and their tau are written for the back
 //    transformation.
 //  * dc_kernel : divide and conquer on the tridiagonal matrix T (Cuppen's method), one workgroup.  T is
@@ -108,13 +109,17 @@ __device__ __forceinline__ double wave_sum(double x) {
 
 struct TrdShared {
   double col[2][JACOBI_MAX];   // column j of the current matrix (double-buffered by step parity)
+  double v[2][JACOBI_MAX];     // the reflector v (v[j+1] = 1, zeros elsewhere outside j+1..m-1)
   double p[2][JACOBI_MAX];     // tau A v
+  double tv[2];                // tau
 };
 
 // Householder steps j = TD_GX BJ .. min(TD_GX (BJ + 1) - 1, m - 3).  Columns below TD_GX BJ are finished (they lie
 // left of every remaining step's column), so only column blocks b >= BJ are touched; row blocks whose
 // rows all lie at or above j are skipped: those finished before the block's first step statically (A0),
-// the others by a uniform test.
+// the others by a uniform test.  Per step: the owner lanes publish column j, wavefront 0 forms the
+// reflector and publishes v and tau, every thread forms its share of p = tau A v, then the update; three
+// barriers.
 template <int BJ>
 __device__ __forceinline__ void trd_block(double (&A)[TD_RA][TD_RB], int m, TrdShared& sh, int tx, int ty,
                                           int lane, int tid, double* __restrict__ Vh, double* __restrict__ tauo,
@@ -122,41 +127,55 @@ __device__ __forceinline__ void trd_block(double (&A)[TD_RA][TD_RB], int m, TrdS
   constexpr int A0 = (BJ * TD_GX + 1) / TD_GY;   // row blocks a < A0 end at or above row BJ TD_GX
   const int j0 = BJ * TD_GX, j1 = min(m - 2, j0 + TD_GX);
   for (int j = j0; j < j1; ++j) {
-    double* col = sh.col[j & 1];
-    double* pv = sh.p[j & 1];
+    const int buf = j & 1;
+    double* col = sh.col[buf];
+    double* vv = sh.v[buf];
+    double* pv = sh.p[buf];
     if (tx == j - j0) {
 #pragma unroll
       for (int a = A0; a < TD_RA; ++a) col[ty + TD_GY * a] = A[a][BJ];
     }
     __syncthreads();
-    // reflector H = I - tau v v^T with v[j+1] = 1 that maps A[j+1:, j] to (beta, 0, ..., 0)
-    double sg = 0.0;
-    for (int r = j + 2 + lane; r < m; r += 64) {
-      const double x = col[r];
-      sg += x * x;
+    if (tid < 64) {   // the reflector H = I - tau v v^T with v[j+1] = 1 that maps A[j+1:, j] to (beta, 0, ..., 0)
+      double sg = 0.0;
+      for (int r = j + 2 + lane; r < m; r += 64) {
+        const double x = col[r];
+        sg += x * x;
+      }
+      sg = wave_sum(sg);
+      const double al = col[j + 1];
+      double be = al, tv = 0.0, sc = 0.0;
+      if (sg > 0.0) {
+        const double nr = sqrt(al * al + sg);
+        be = al >= 0.0 ? -nr : nr;
+        const double d = al - be;             // |d| >= |al|: no cancellation
+        const double inv = 1.0 / (be * d);    // tau = (beta - alpha) / beta = -d^2 inv, 1 / d = beta inv
+        tv = -d * d * inv;
+        sc = be * inv;
+      }
+      // rows of col at or above j may hold stale values: the selects drop them
+#pragma unroll
+      for (int k = 0; k < JACOBI_MAX / 64; ++k) {
+        const int r = lane + 64 * k;
+        const double x = col[r] * sc;
+        const double v = r == j + 1 ? 1.0 : ((r > j + 1 && r < m) ? x : 0.0);
+        vv[r] = v;
+        if (r < m) Vh[r + (size_t)j * m] = v;
+      }
+      if (lane == 0) {
+        sh.tv[buf] = tv;
+        tauo[j] = tv;
+        eo[j] = be;
+      }
     }
-    sg = wave_sum(sg);
-    const double al = col[j + 1];
-    double be = al, tv = 0.0, sc = 0.0;
-    if (sg > 0.0) {
-      const double nr = sqrt(al * al + sg);
-      be = al >= 0.0 ? -nr : nr;
-      const double d = al - be;             // |d| >= |al|: no cancellation
-      const double inv = 1.0 / (be * d);    // tau = (beta - alpha) / beta = -d^2 inv, 1 / d = beta inv
-      tv = -d * d * inv;
-      sc = be * inv;
-    }
-    // v[r] for any r < 192 (rows of finished blocks hold stale values, which the selects drop)
-    auto vat = [&](int r) -> double {
-      const double x = col[r] * sc;
-      return r == j + 1 ? 1.0 : ((r > j + 1 && r < m) ? x : 0.0);
-    };
+    __syncthreads();
+    const double tv = sh.tv[buf];
     // p = tau A v on the rows below j (finished row blocks are skipped and never stored); column by column,
     // so only the matrix, the row sums and one v entry are live
     double ps[TD_RA] = {};
 #pragma unroll
     for (int b = BJ; b < TD_RB; ++b) {
-      const double vcb = vat(tx + TD_GX * b);
+      const double vcb = vv[tx + TD_GX * b];
 #pragma unroll
       for (int a = A0; a < TD_RA; ++a) {
         if (TD_GY * a + TD_GY - 1 <= j) continue;   // uniform: every row of the block is finished
@@ -184,17 +203,10 @@ __device__ __forceinline__ void trd_block(double (&A)[TD_RA][TD_RB], int m, TrdS
       const int r = ty + TD_GY * a;
       if (tx == (TD_GX == 32 ? 16 : 0)) pv[r] = (r > j && r < m) ? tv * ps[a] : 0.0;   // a lane that holds the row total
     }
-    if (tid < 64) {
-      for (int r = lane; r < m; r += 64) Vh[r + (size_t)j * m] = vat(r);
-      if (lane == 0) {
-        tauo[j] = tv;
-        eo[j] = be;
-      }
-    }
     __syncthreads();
     // w = p - (tau / 2) (p . v) v ;  A -= v w^T + w v^T
     double pw = 0.0;
-    for (int r = j + 1 + lane; r < m; r += 64) pw += pv[r] * vat(r);
+    for (int r = j + 1 + lane; r < m; r += 64) pw += pv[r] * vv[r];
     pw = wave_sum(pw);
     const double kc = 0.5 * tv * pw;
     // the thread's rows of v and w, then the update column by column (p is written this step only for
@@ -203,13 +215,13 @@ __device__ __forceinline__ void trd_block(double (&A)[TD_RA][TD_RB], int m, TrdS
 #pragma unroll
     for (int a = A0; a < TD_RA; ++a) {
       const int r = ty + TD_GY * a;
-      vr[a] = vat(r);
+      vr[a] = vv[r];
       wr[a] = TD_GY * a + TD_GY - 1 <= j ? 0.0 : pv[r] - kc * vr[a];
     }
 #pragma unroll
     for (int b = BJ; b < TD_RB; ++b) {
       const int c = tx + TD_GX * b;
-      const double vcb = vat(c), x = pv[c] - kc * vcb, wcb = c > j ? x : 0.0;
+      const double vcb = vv[c], x = pv[c] - kc * vcb, wcb = c > j ? x : 0.0;
 #pragma unroll
       for (int a = A0; a < TD_RA; ++a) {
         if (TD_GY * a + TD_GY - 1 <= j) continue;
@@ -316,6 +328,8 @@ __device__ __forceinline__ double sumL(double x) {
   if constexpr (L == 4) {
     x += __shfl_xor(x, 1);
     x += __shfl_xor(x, 2);
+  } else if constexpr (L == 64) {
+    x = wave_sum(x);
   }
   return x;
 }
@@ -325,6 +339,9 @@ __device__ __forceinline__ double prodL(double x) {
   if constexpr (L == 4) {
     x *= __shfl_xor(x, 1);
     x *= __shfl_xor(x, 2);
+  } else if constexpr (L == 64) {
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) x *= __shfl_xor(x, o);
   }
   return x;
 }
@@ -342,18 +359,19 @@ __device__ __forceinline__ double rcp_nr(double x) {
 // modelled by its two nearest poles and a constant); false when none lies there
 __device__ __forceinline__ bool dc_quad(double c, double djl, double djr, double s1, double s2, double le, double he,
                                         double& e) {
+  // (quotients by the refined reciprocal: an iterate needs no correctly rounded division)
   const double qa = c, qb = -(c * (djl + djr) + s1 + s2), qc = c * djl * djr + s1 * djr + s2 * djl;
   if (qa == 0.0) {
     if (qb == 0.0) return false;
-    e = -qc / qb;
+    e = -qc * rcp_nr(qb);
     return e > le && e < he;
   }
   const double sq = sqrt(fmax(qb * qb - 4.0 * qa * qc, 0.0));
   const double qq = -0.5 * (qb + copysign(sq, qb));
-  e = qq / qa;
+  e = qq * rcp_nr(qa);
   if (e > le && e < he) return true;
   if (qq == 0.0) return false;
-  e = qc / qq;
+  e = qc * rcp_nr(qq);
   return e > le && e < he;
 }
 
@@ -388,15 +406,33 @@ __device__ void dc_secular(const double* dk, const double* z2, int K, int j, int
   }
   const double dorg = dk[o], dj = dk[j] - dorg, dj1 = j < K - 1 ? dk[j + 1] - dorg : 0.0;
   for (int it = 0; it < 80; ++it) {
-    double f = 0.0, af = 0.0, psp = 0.0, php = 0.0;
-    terms([&](int i, double d, double z) {
-      const double r = rcp_nr((d - dorg) - t);
-      const double w = z * r;
+    // two terms per pass into separate sums, so the two reciprocal chains overlap
+    double f = 0.0, af = 0.0, psp = 0.0, php = 0.0, f2 = 0.0, af2 = 0.0, psp2 = 0.0, php2 = 0.0;
+    int i = q;
+    for (; i + L < K; i += 2 * L) {
+      const double r0 = rcp_nr((dk[i] - dorg) - t), r1 = rcp_nr((dk[i + L] - dorg) - t);
+      const double w0 = z2[i] * r0, w1 = z2[i + L] * r1;
+      f += w0;
+      f2 += w1;
+      af += fabs(w0);
+      af2 += fabs(w1);
+      if (i <= j) psp += w0 * r0;
+      else php += w0 * r0;
+      if (i + L <= j) psp2 += w1 * r1;
+      else php2 += w1 * r1;
+    }
+    if (i < K) {
+      const double r = rcp_nr((dk[i] - dorg) - t);
+      const double w = z2[i] * r;
       f += w;
       af += fabs(w);
       if (i <= j) psp += w * r;
       else php += w * r;
-    });
+    }
+    f += f2;
+    af += af2;
+    psp += psp2;
+    php += php2;
     f = 1.0 + sumL<L>(f);
     af = sumL<L>(af);
     psp = sumL<L>(psp);
@@ -412,7 +448,7 @@ __device__ void dc_secular(const double* dk, const double* z2, int K, int j, int
     if (j < K - 1) {
       const double djr = dj1 - t;
       const double s2 = djr * djr * php;
-      const double c = f - s1 / djl - s2 / djr;
+      const double c = f - s1 * rcp_nr(djl) - s2 * rcp_nr(djr);
       double e;
       if (dc_quad(c, djl, djr, s1, s2, lo_t - t, hi_t - t, e)) { tn = t + e; ok = true; }
     } else {
@@ -652,14 +688,22 @@ __device__ void dc_roots(DcShared& sh, int m, int tid) {
   __syncthreads();
 }
 
+// the top merge as dc_kernel leaves it (after its deflation and rotations) for dc_sec_kernel, dc_zh_kernel
+// and dc_final_kernel: the K kept poles (ascending), their z and rho z^2, their columns of Z, the deflated
+// columns (slots K..m-1), and per root its pole, offset and recomputed z
+struct DcTop {
+  double *dK, *zK, *z2, *tau, *zh;
+  int *kc, *dcol, *org, *K;
+  double* rho;
+};
+
 // one level of merges: Qin holds the children's eigenvectors (block diagonal), Qout receives the
 // merged ones; U (m x m) holds each merge's K x K eigenvector matrix of D + rho z z^T, transposed, in
 // its diagonal block, rows ordered top-half columns, rotated columns, bottom-half columns.
 __device__ void dc_level(DcShared& sh, const double* __restrict__ ev, int m, int dd, double* __restrict__ Qin,
-                         double* __restrict__ Qout, double* __restrict__ U, int tid) {
+                         double* __restrict__ Qout, double* __restrict__ U, const DcTop& top,
+                         double* __restrict__ lam_out, int tid) {
   // children's eigenpairs -> z, sorted d of every merge node
-  if (dd == 0)   // the top merge leaves its merge matrix in U (zero outside its entries)
-    for (int idx = tid; idx < m * m; idx += DC_NT) U[idx] = 0.0;
   if (tid < m) {
     const int g = tid;
     int lo, hi;
@@ -722,6 +766,25 @@ __device__ void dc_level(DcShared& sh, const double* __restrict__ ev, int m, int
       }
     }
   }
+  if (dd == 0) {   // the top merge goes on over the whole chip (dc_sec_kernel, dc_zh_kernel, dc_final_kernel)
+    const int K = sh.nK[0];
+    if (tid < m) {
+      if (tid < K) {
+        top.dK[tid] = sh.dK[tid];
+        top.zK[tid] = sh.zK[tid];
+        top.z2[tid] = sh.z2[tid];
+        top.kc[tid] = sh.kc[tid];
+      } else {
+        top.dcol[tid] = sh.dcol[tid];
+        lam_out[tid] = sh.dvl[tid];
+      }
+    }
+    if (tid == 0) {
+      *top.K = K;
+      *top.rho = sh.nRho[0];
+    }
+    return;
+  }
   const int smax = (m + (1 << dd) - 1) >> dd;   // the widest node of this level
   // narrow merges: one lane per root; wide ones: four lanes share each root's sums
   if (smax <= DC_ONE_LANE) dc_roots<1>(sh, m, tid);
@@ -734,18 +797,11 @@ __device__ void dc_level(DcShared& sh, const double* __restrict__ ev, int m, int
     const int K = sh.nK[lo], i = r - lo;
     if (i >= K || t >= K) continue;
     const double u = sh.zh[lo + i] / ((sh.dK[lo + i] - sh.dK[sh.org[lo + t]]) - sh.tau[lo + t]) * sh.rn[lo + t];
-    if (dd == 0) U[t + (size_t)sh.kc[i] * m] = u;   // M(kc_i, t), stored transposed
-    else U[(lo + t) + (size_t)(lo + sh.upos[lo + i]) * m] = u;   // row p = upos of the K x K matrix, stored transposed
+    U[(lo + t) + (size_t)(lo + sh.upos[lo + i]) * m] = u;   // row p = upos of the K x K matrix, stored transposed
   }
   if (tid < m) {
     sh.nmd[tid] = 0;
     sh.nmz[tid] = 0;
-  }
-  if (dd == 0) {   // deflated columns: M(dcol_t, t) = 1; the product Qin M is left to dc_final_kernel
-    const int K = sh.nK[0];
-    if (tid >= K && tid < m) U[tid + (size_t)sh.dcol[tid] * m] = 1.0;
-    __syncthreads();
-    return;
   }
   __syncthreads();
   // Qout = Qin[:, kept] U, deflated columns copied.  Levels of narrow merges: one output per item.
@@ -836,7 +892,7 @@ __device__ void dc_level(DcShared& sh, const double* __restrict__ ev, int m, int
 // stored transposed in U (m x m each); Qa: m x m scratch
 __global__ __launch_bounds__(DC_NT) void dc_kernel(const double* __restrict__ d, const double* __restrict__ e, int m,
                                                    double* __restrict__ Qa, double* __restrict__ Qb,
-                                                   double* __restrict__ U, double* __restrict__ lam_out,
+                                                   double* __restrict__ U, DcTop top, double* __restrict__ lam_out,
                                                    const int* __restrict__ gate) {
   if (gate && *gate == 0) return;
   __shared__ DcShared sh;
@@ -862,48 +918,116 @@ __global__ __launch_bounds__(DC_NT) void dc_kernel(const double* __restrict__ d,
   }
   __syncthreads();
   for (int dd = depth - 1; dd >= 0; --dd) {
-    dc_level(sh, e, m, dd, Qin, Qout, U, tid);
+    dc_level(sh, e, m, dd, Qin, Qout, U, top, lam_out, tid);
     if (dd == 0) break;
     double* t = Qin;
     Qin = Qout;
     Qout = t;
   }
-  if (depth == 0 && tid == 0) U[0] = 1.0;   // m = 1: Qb = 1, M = 1
-  if (tid < m) lam_out[tid] = sh.lam[tid];
+  if (depth == 0 && tid == 0) {   // m = 1: Qb = 1, one deflated column
+    *top.K = 0;
+    top.dcol[0] = 0;
+    lam_out[0] = sh.lam[0];
+  }
 }
 
-// W = Z M for the top merge: Z = Qb (the children's eigenvectors, deflation-rotated and back-transformed),
-// M from U (transposed); a 16 x 16 tile of W per wavefront on the FP64 matrix cores, over the whole chip
-__global__ __launch_bounds__(64) void dc_final_kernel(const double* __restrict__ Z, const double* __restrict__ Mt,
-                                                      int m, double* __restrict__ W, const int* __restrict__ gate) {
+// the top merge's secular roots, one wavefront per root (all 64 lanes share its sums): root j's pole and
+// offset, and its eigenvalue
+__global__ __launch_bounds__(64) void dc_sec_kernel(DcTop top, int m, double* __restrict__ lam_out,
+                                                    const int* __restrict__ gate) {
   if (gate && *gate == 0) return;
+  const int j = blockIdx.x, lane = threadIdx.x;
+  const int K = *top.K;
+  if (j >= K) return;
+  __shared__ double dk[JACOBI_MAX], z2[JACOBI_MAX];
+  for (int i = lane; i < K; i += 64) {
+    dk[i] = top.dK[i];
+    z2[i] = top.z2[i];
+  }
+  __syncthreads();
+  int o;
+  double t;
+  dc_secular<64>(dk, z2, K, j, lane, o, t);
+  if (lane == 0) {
+    top.org[j] = o;
+    top.tau[j] = t;
+    lam_out[j] = dk[o] + t;
+  }
+}
+
+// the top merge's z recomputed from its roots (Lowner's formula), one wavefront per entry
+__global__ __launch_bounds__(64) void dc_zh_kernel(DcTop top, int m, const int* __restrict__ gate) {
+  if (gate && *gate == 0) return;
+  const int i = blockIdx.x, lane = threadIdx.x;
+  const int K = *top.K;
+  if (i >= K) return;
+  const double rho = *top.rho, di = top.dK[i];
+  double p = 1.0;
+  for (int jj = lane; jj < K; jj += 64) {
+    const double lmd = (top.dK[top.org[jj]] - di) + top.tau[jj];
+    const double den = jj == K - 1 ? rho : (jj < i ? top.dK[jj] - di : top.dK[jj + 1] - di);
+    p *= lmd / den;
+  }
+  p = prodL<64>(p);
+  if (lane == 0) top.zh[i] = copysign(sqrt(fmax(p, 0.0)), top.zK[i]);
+}
+
+// W = Z M for the top merge, Z = Qb (the children's eigenvectors, deflation-rotated and back-transformed):
+// column t < K of M holds the normalised eigenvector zhat_i / (d_i - lambda_t) of the merge on the rows kc_i,
+// a deflated column t >= K is column dcol_t of Z.  A 16 x 16 tile of W per workgroup on the FP64 matrix
+// cores, over the whole chip; each workgroup forms the norms of its 16 columns and the entries of M it needs.
+__global__ __launch_bounds__(64) void dc_final_kernel(const double* __restrict__ Z, DcTop top, int m,
+                                                      double* __restrict__ W, const int* __restrict__ gate) {
+  if (gate && *gate == 0) return;
+  __shared__ double dk[JACOBI_MAX], zh[JACOBI_MAX];
+  __shared__ int kc[JACOBI_MAX];
   const int lane = threadIdx.x, li = lane & 15, lk = lane >> 4;
   const int r0 = 16 * blockIdx.x, t0 = 16 * blockIdx.y;
-  const int ra = r0 + li, tb = t0 + li;
-  // clamped addresses, no branches: rows and columns beyond m only feed outputs that are not stored, and
-  // the k-steps beyond m are zeroed through b
-  const int rac = min(ra, m - 1), tbc = min(tb, m - 1);
+  const int K = *top.K;
+  for (int i = lane; i < K; i += 64) {
+    dk[i] = top.dK[i];
+    zh[i] = top.zh[i];
+    kc[i] = top.kc[i];
+  }
+  __syncthreads();
+  // this lane's column t = t0 + li: its root's pole and offset, and its norm (four lanes per column)
+  const int tb = t0 + li;
+  const bool kept = tb < K;
+  const double dorg = kept ? dk[top.org[tb]] : 0.0, tt = kept ? top.tau[tb] : 0.0;
+  double sq = 0.0;
+  if (kept)
+    for (int i = lk; i < K; i += 4) {
+      const double u = zh[i] / ((dk[i] - dorg) - tt);
+      sq += u * u;
+    }
+  sq += __shfl_xor(sq, 16);
+  sq += __shfl_xor(sq, 32);
+  const double rn = kept ? 1.0 / sqrt(sq) : 0.0;
+  const int ra = r0 + li;
+  const int rac = min(ra, m - 1);   // rows beyond m only feed outputs that are not stored
   d4 acc = {0.0, 0.0, 0.0, 0.0};
-  for (int p = 0; p < m; p += 64) {
+  for (int p = 0; p < K; p += 64) {
     double a[16], b[16];
 #pragma unroll
     for (int u = 0; u < 16; ++u) {
-      const int pk = min(p + 4 * u + lk, m - 1);
-      a[u] = Z[rac + (size_t)pk * m];
-      b[u] = Mt[tbc + (size_t)pk * m];
+      const int i = min(p + 4 * u + lk, K - 1);
+      a[u] = Z[rac + (size_t)kc[i] * m];
+      b[u] = zh[i] / ((dk[i] - dorg) - tt) * rn;
     }
-    __builtin_amdgcn_sched_barrier(0);   // all 32 loads in flight before the first product
+    __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
     for (int u = 0; u < 16; ++u)
-      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u], p + 4 * u + lk < m ? b[u] : 0.0, acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u], (kept && p + 4 * u + lk < K) ? b[u] : 0.0, acc, 0, 0, 0);
   }
-  // register q holds row lk + 4 q, column li
+  // register q holds row lk + 4 q, column li; deflated columns are copied
+  const int src = kept ? 0 : (tb < m ? top.dcol[tb] : 0);
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
     const int r = r0 + lk + 4 * q;
-    if (r < m && tb < m) W[r + (size_t)tb * m] = acc[q];
+    if (r < m && tb < m) W[r + (size_t)tb * m] = kept ? acc[q] : Z[r + (size_t)src * m];
   }
 }
+
 
 // Z <- H_0 H_1 ... H_(m-3) Z in place, one wavefront per column.  The reflectors are applied four at a
 // time: with s_q = tau_q (u_q . z - sum_(p<q) s_p u_q . u_p) for the block's reflectors u_0..u_3 in the
@@ -1037,6 +1161,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 1))) void
 // the work buffer: Vh, Qa, Qb, U (m x m each), d, e, tau, lambda (m each), the reflector block products
 struct DcWork {
   double *Vh, *Qa, *Qb, *U, *dv, *ev, *tau, *lam, *gram;
+  DcTop top;
 };
 
 DcWork dc_work(double* w, int m) {
@@ -1051,17 +1176,40 @@ DcWork dc_work(double* w, int m) {
   d.tau = d.ev + m;
   d.lam = d.tau + m;
   d.gram = d.lam + m;   // 6 per block of four reflectors, the count padded to 32: at most 1.5 m + 44
+  double* t = d.gram + 2 * (size_t)m + 48;   // the top merge: 5 m doubles, rho, then 3 m + 1 ints
+  d.top.dK = t;
+  d.top.zK = t + m;
+  d.top.z2 = t + 2 * m;
+  d.top.tau = t + 3 * m;
+  d.top.zh = t + 4 * m;
+  d.top.rho = t + 5 * m;
+  int* ti = reinterpret_cast<int*>(t + 5 * m + 1);
+  d.top.kc = ti;
+  d.top.dcol = ti + m;
+  d.top.org = ti + 2 * m;
+  d.top.K = ti + 3 * m;
   return d;
 }
 
 // blocks of four reflectors that trd_back_kernel reads (m - 2 padded to a multiple of 32)
 int dc_gram_blocks(int m) { return m > 2 ? ((m - 2 + 2 * BK_Q - 1) / (2 * BK_Q)) * (BK_Q / 2) : 0; }
 
-// everything after the tridiagonalisation: divide and conquer, block products, back-transformation of the
-// children's eigenvectors, the top merge's product (into Qa)
-void dc_stages(const DcWork& w, int m, const int* gate, hipStream_t s) {
-  hipLaunchKernelGGL(dc_kernel, dim3(1), dim3(DC_NT), 0, s, w.dv, w.ev, m, w.Qa, w.Qb, w.U, w.lam, gate);
+// the divide and conquer: the merge tree below the top merge in one workgroup, the top merge's roots and z
+// over the whole chip
+void dc_solve(const DcWork& w, int m, const int* gate, hipStream_t s) {
+  hipLaunchKernelGGL(dc_kernel, dim3(1), dim3(DC_NT), 0, s, w.dv, w.ev, m, w.Qa, w.Qb, w.U, w.top, w.lam, gate);
   TNAD_HIP_TRY(hipGetLastError());
+  if (m > 1) {
+    hipLaunchKernelGGL(dc_sec_kernel, dim3(m), dim3(64), 0, s, w.top, m, w.lam, gate);
+    TNAD_HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(dc_zh_kernel, dim3(m), dim3(64), 0, s, w.top, m, gate);
+    TNAD_HIP_TRY(hipGetLastError());
+  }
+}
+
+// the eigenvectors: block products and back-transformation of the children's eigenvectors, then the top
+// merge's product (into Qa)
+void dc_vectors(const DcWork& w, int m, const int* gate, hipStream_t s) {
   if (m > 2) {
     hipLaunchKernelGGL(trd_gram_kernel, dim3(dc_gram_blocks(m)), dim3(64), 0, s, w.Vh, m, w.gram, gate);
     TNAD_HIP_TRY(hipGetLastError());
@@ -1069,13 +1217,13 @@ void dc_stages(const DcWork& w, int m, const int* gate, hipStream_t s) {
     TNAD_HIP_TRY(hipGetLastError());
   }
   const int nt = (m + 15) / 16;
-  hipLaunchKernelGGL(dc_final_kernel, dim3(nt, nt), dim3(64), 0, s, w.Qb, w.U, m, w.Qa, gate);
+  hipLaunchKernelGGL(dc_final_kernel, dim3(nt, nt), dim3(64), 0, s, w.Qb, w.top, m, w.Qa, gate);
   TNAD_HIP_TRY(hipGetLastError());
 }
 
 }  // namespace
 
-size_t syev_dc_work_doubles(int m) { return 4 * (size_t)m * m + 6 * (size_t)m + 48; }
+size_t syev_dc_work_doubles(int m) { return 4 * (size_t)m * m + 13 * (size_t)m + 56; }
 
 void syev_dc(const double* H, int ldh, int m, double* evals, double* V, double* work, hipStream_t s,
              const int* gate) {
@@ -1083,7 +1231,8 @@ void syev_dc(const double* H, int ldh, int m, double* evals, double* V, double* 
   const DcWork w = dc_work(work, m);
   hipLaunchKernelGGL(trd_kernel, dim3(1), dim3(TD_NT), 0, s, H, ldh, m, w.dv, w.ev, w.Vh, w.tau, gate);
   TNAD_HIP_TRY(hipGetLastError());
-  dc_stages(w, m, gate, s);
+  dc_solve(w, m, gate, s);
+  dc_vectors(w, m, gate, s);
   sort_eigenpairs(w.lam, w.Qa, m, evals, V, gate, s);
 }
 
@@ -1099,13 +1248,9 @@ void syev_dc_time(const double* H, int m, int reps, double* work, hipStream_t s,
     TNAD_HIP_TRY(hipEventRecord(e[0], s));
     hipLaunchKernelGGL(trd_kernel, dim3(1), dim3(TD_NT), 0, s, H, m, m, w.dv, w.ev, w.Vh, w.tau, (const int*)nullptr);
     TNAD_HIP_TRY(hipEventRecord(e[1], s));
-    hipLaunchKernelGGL(dc_kernel, dim3(1), dim3(DC_NT), 0, s, w.dv, w.ev, m, w.Qa, w.Qb, w.U, w.lam,
-                       (const int*)nullptr);
+    dc_solve(w, m, nullptr, s);
     TNAD_HIP_TRY(hipEventRecord(e[2], s));
-    hipLaunchKernelGGL(trd_gram_kernel, dim3(dc_gram_blocks(m)), dim3(64), 0, s, w.Vh, m, w.gram, (const int*)nullptr);
-    hipLaunchKernelGGL(trd_back_kernel, dim3(m), dim3(64), 0, s, w.Vh, w.tau, w.gram, m, w.Qb, (const int*)nullptr);
-    hipLaunchKernelGGL(dc_final_kernel, dim3((m + 15) / 16, (m + 15) / 16), dim3(64), 0, s, w.Qb, w.U, m, w.Qa,
-                       (const int*)nullptr);
+    dc_vectors(w, m, nullptr, s);
     sort_eigenpairs(w.lam, w.Qa, m, evals, V, nullptr, s);
     TNAD_HIP_TRY(hipEventRecord(e[3], s));
     TNAD_HIP_TRY(hipEventSynchronize(e[3]));
