@@ -490,8 +490,13 @@ static_assert(CH_R * CH_G == JACOBI_MAX, "Cholesky tiling");
 struct CholShared {
   double* cb;     // 2 x JACOBI_MAX broadcast buffers: column jj of the Schur complement
   double* rb;     // 2 x JACOBI_MAX broadcast buffers: row jj of B before its scaling
-  int* bad;
-  double* pmin;   // smallest pivot so far (thread 0 only)
+};
+
+// the first non-positive pivot (1-based, 0: none) and the smallest pivot so far, kept by thread 0 in
+// registers (every thread sees the same pivots)
+struct CholState {
+  int bad = 0;
+  double pmin = HUGE_VAL;
 };
 
 // 1 / sqrt(d) for a positive normal d: the hardware estimate refined by two Newton steps
@@ -510,8 +515,8 @@ __device__ __forceinline__ double rsqrt_refined(double d) {
 // After the block's steps its rows of B are final and are written to Rinv.
 template <int BJ>
 __device__ __forceinline__ void chol_block(double (&A)[CH_R][CH_R], double (&B)[CH_R][CH_R], int m,
-                                           const CholShared& sh, int tx, int ty, int t, double* __restrict__ piv,
-                                           double* __restrict__ Rinv) {
+                                           const CholShared& sh, CholState& st, int tx, int ty, int t,
+                                           double* __restrict__ piv, double* __restrict__ Rinv) {
   const int j0 = BJ * CH_G, j1 = min(m, j0 + CH_G);
   for (int jj = j0; jj < j1; ++jj) {
     double* col = sh.cb + (jj & 1) * JACOBI_MAX;
@@ -529,25 +534,31 @@ __device__ __forceinline__ void chol_block(double (&A)[CH_R][CH_R], double (&B)[
       for (int b = 0; b <= BJ; ++b) row[tx + CH_G * b] = B[BJ][b];
     }
     __syncthreads();
+    // every LDS read of the step first, so that their latency overlaps the pivot's reciprocal square root;
+    // no branch before the products (the pivot bookkeeping is thread 0's, in registers)
     double d = col[jj];
-    if (!(d > 0.0)) {
-      if (t == 0 && *sh.bad == 0) *sh.bad = jj + 1;
-      d = 1e-300;
-    }
-    const double isd = rsqrt_refined(d);
-    const double sd = d * isd;
-    if (t == 0) {
-      if (piv) piv[jj] = sd;
-      *sh.pmin = fmin(*sh.pmin, sd);
-    }
     double li[CH_R], lj[CH_R], xr[CH_R];
 #pragma unroll
     for (int a = BJ; a < CH_R; ++a) {
-      li[a] = col[ty + CH_G * a] * isd;
-      lj[a] = col[tx + CH_G * a] * isd;
+      li[a] = col[ty + CH_G * a];
+      lj[a] = col[tx + CH_G * a];
     }
 #pragma unroll
-    for (int b = 0; b <= BJ; ++b) xr[b] = row[tx + CH_G * b] * isd;   // row jj of B, final
+    for (int b = 0; b <= BJ; ++b) xr[b] = row[tx + CH_G * b];
+    const bool ok = d > 0.0;
+    st.bad = (!ok && st.bad == 0) ? jj + 1 : st.bad;
+    d = ok ? d : 1e-300;
+    const double isd = rsqrt_refined(d);
+    const double sd = d * isd;
+    st.pmin = fmin(st.pmin, sd);
+#pragma unroll
+    for (int a = BJ; a < CH_R; ++a) {
+      li[a] *= isd;
+      lj[a] *= isd;
+    }
+#pragma unroll
+    for (int b = 0; b <= BJ; ++b) xr[b] *= isd;   // row jj of B, final
+    if (t == 0 && piv) piv[jj] = sd;
     // trailing lower blocks BJ <= b <= a of the Schur complement
 #pragma unroll
     for (int a = BJ; a < CH_R; ++a)
@@ -584,10 +595,10 @@ __device__ __forceinline__ void chol_block(double (&A)[CH_R][CH_R], double (&B)[
 // the column blocks BJ, BJ + 1, ..., CH_R - 1 in turn
 template <int BJ>
 __device__ __forceinline__ void chol_blocks(double (&A)[CH_R][CH_R], double (&B)[CH_R][CH_R], int m,
-                                            const CholShared& sh, int tx, int ty, int t, double* __restrict__ piv,
-                                            double* __restrict__ Rinv) {
-  chol_block<BJ>(A, B, m, sh, tx, ty, t, piv, Rinv);
-  if constexpr (BJ + 1 < CH_R) chol_blocks<BJ + 1>(A, B, m, sh, tx, ty, t, piv, Rinv);
+                                            const CholShared& sh, CholState& st, int tx, int ty, int t,
+                                            double* __restrict__ piv, double* __restrict__ Rinv) {
+  chol_block<BJ>(A, B, m, sh, st, tx, ty, t, piv, Rinv);
+  if constexpr (BJ + 1 < CH_R) chol_blocks<BJ + 1>(A, B, m, sh, st, tx, ty, t, piv, Rinv);
 }
 
 __global__ __launch_bounds__(CH_G * CH_G) void chol_inv_kernel(const double* __restrict__ G, int m,
@@ -595,9 +606,8 @@ __global__ __launch_bounds__(CH_G * CH_G) void chol_inv_kernel(const double* __r
                                                                int* info, double* __restrict__ piv,
                                                                double* __restrict__ pmin_out) {
   __shared__ double bufs[4 * JACOBI_MAX];
-  __shared__ int bad;
-  __shared__ double pmin_s;
-  CholShared sh{bufs, bufs + 2 * JACOBI_MAX, &bad, &pmin_s};
+  CholShared sh{bufs, bufs + 2 * JACOBI_MAX};
+  CholState st;
   const int t = threadIdx.x, tx = t % CH_G, ty = t / CH_G;
   double A[CH_R][CH_R], B[CH_R][CH_R];
 #pragma unroll
@@ -605,18 +615,16 @@ __global__ __launch_bounds__(CH_G * CH_G) void chol_inv_kernel(const double* __r
 #pragma unroll
     for (int b = 0; b < CH_R; ++b) {
       const int i = ty + CH_G * a, j = tx + CH_G * b;
-      double v = (i == j) ? 1.0 : 0.0;   // padding beyond m: identity, never coupled to the rest
-      if (i < m && j < m) v = G[j + (size_t)m * i] + (i == j ? shift : 0.0);
-      A[a][b] = v;
+      const double g = G[min(j, m - 1) + (size_t)m * min(i, m - 1)];   // branch-free, clamped address
+      // padding beyond m: identity, never coupled to the rest
+      A[a][b] = (i < m && j < m) ? g + (i == j ? shift : 0.0) : (i == j ? 1.0 : 0.0);
       B[a][b] = (i == j) ? 1.0 : 0.0;
     }
   }
-  if (t == 0) { bad = 0; pmin_s = HUGE_VAL; }
-  __syncthreads();
-  chol_blocks<0>(A, B, m, sh, tx, ty, t, piv, Rinv);
+  chol_blocks<0>(A, B, m, sh, st, tx, ty, t, piv, Rinv);
   if (t == 0) {
-    if (info) *info = bad;
-    if (pmin_out) *pmin_out = pmin_s;
+    if (info) *info = st.bad;
+    if (pmin_out) *pmin_out = st.pmin;
   }
 }
 
