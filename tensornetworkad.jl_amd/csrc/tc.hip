@@ -453,19 +453,26 @@ __global__ __launch_bounds__(NT, TILE == 128 ? 1 : (sizeof(T) == 8 ? 3 : 4)) voi
 
 // split-K reduction: C = alpha * sum_s W[s] + beta * C over the strided output (slices summed in
 // order, so the result does not depend on scheduling)
+// The grid's y index walks the output columns n and x the rows m, so no element index is divided (the
+// first version split a 64-bit element index by M per element, and its integer divisions, not the
+// memory traffic, set its time); the column offset is formed once per column, the row offset without a
+// division when the row index is not split (M1 = 1).
 template <typename T>
 __global__ __launch_bounds__(NT) void tc_splitk_reduce(TcDesc d, const T* __restrict__ Wsp, int nsplit,
                                                        T* __restrict__ C) {
-  const long long M = (long long)d.M0 * d.M1, N = (long long)d.N0 * d.N1;
-  const long long MN = M * N;
-  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < MN; e += (long long)gridDim.x * blockDim.x) {
-    const int m = (int)(e % M), n = (int)(e / M);
-    double acc = 0.0;
-    for (int s = 0; s < nsplit; ++s) acc += (double)Wsp[e + (size_t)s * MN];
-    const long long o = off2(m, d.M0, d.cm0, d.cm1) + off2(n, d.N0, d.cn0, d.cn1);
-    double v = d.alpha * acc;
-    if (d.beta != 0.0) v += d.beta * (double)C[o];
-    C[o] = (T)v;
+  const int M = d.M0 * d.M1, N = d.N0 * d.N1;
+  const size_t MN = (size_t)M * N;
+  for (int n = blockIdx.y; n < N; n += gridDim.y) {
+    const long long on = off2(n, d.N0, d.cn0, d.cn1);
+    for (int m = blockIdx.x * NT + threadIdx.x; m < M; m += gridDim.x * NT) {
+      const size_t e = (size_t)m + (size_t)M * n;
+      double acc = 0.0;
+      for (int s = 0; s < nsplit; ++s) acc += (double)Wsp[e + (size_t)s * MN];
+      const long long o = (d.M1 == 1 ? (long long)m * d.cm0 : off2(m, d.M0, d.cm0, d.cm1)) + on;
+      double v = d.alpha * acc;
+      if (d.beta != 0.0) v += d.beta * (double)C[o];
+      C[o] = (T)v;
+    }
   }
 }
 
@@ -929,13 +936,14 @@ void tc_launch(Ctx* c, Prec p, const TcDesc& d, const void* A, const void* B, vo
   }
   TNAD_HIP_TRY(hipGetLastError());
   if (nsplit > 1) {
-    const long long MN = M * N;
-    unsigned nb = (unsigned)std::min<long long>((MN + NT - 1) / NT, 65535LL * 8);
+    if (M * N >= (1LL << 31)) throw Error(TNAD_ERR_ARG, "split-K output too large");
+    const dim3 rg((unsigned)std::min<long long>((M + NT - 1) / NT, 65535LL),
+                  (unsigned)std::min<long long>(N, 65535LL));
     if (p == Prec::F64)
-      hipLaunchKernelGGL(tc_splitk_reduce<double>, dim3(nb), dim3(NT), 0, s, dd, (const double*)W, nsplit,
+      hipLaunchKernelGGL(tc_splitk_reduce<double>, rg, dim3(NT), 0, s, dd, (const double*)W, nsplit,
                          (double*)C);
     else
-      hipLaunchKernelGGL(tc_splitk_reduce<float>, dim3(nb), dim3(NT), 0, s, dd, (const float*)W, nsplit,
+      hipLaunchKernelGGL(tc_splitk_reduce<float>, rg, dim3(NT), 0, s, dd, (const float*)W, nsplit,
                          (float*)C);
     TNAD_HIP_TRY(hipGetLastError());
   }
