@@ -4,10 +4,11 @@
 //    matrix stays in registers in a cyclic distribution (thread (ty, tx) holds the 6 x 12 elements
 //    (ty + 32 a, tx + 16 b)), both triangles, so the product A v of each step is a per-thread partial
 //    sum over its twelve columns finished by DPP butterflies across the 16 lanes of the same ty (one
-//    row of a wavefront), and the symmetric rank-two update is local.  Per step: the owner
-//    lanes publish column j through LDS, wavefront 0 forms the reflector from it and publishes v and
-//    tau (the other wavefronts wait at the barrier instead of repeating its reductions, square root
-//    and division), then p = tau A v through LDS, then the update; three barriers per step; column
+//    row of a wavefront), and the symmetric rank-two update is local.  Per step: p = tau A v through
+//    LDS, while the owner lanes of column j + 1 publish it as it stands; after the second barrier every
+//    thread updates its elements, and wavefront 0 forms column j + 1 after the update from the published
+//    column, v, p and p . v alone, then the next reflector from it, and publishes v and tau (the other
+//    wavefronts never repeat its reductions, square root and division); two barriers per step; column
 //    blocks that a step cannot touch are not emitted (template block index), finished row blocks are
 //    skipped.
 //    The reflectors (v with v[j+1] = 1, zeros above) and their tau are written for the back
@@ -108,18 +109,70 @@ __device__ __forceinline__ double wave_sum(double x) {
 }
 
 struct TrdShared {
-  double col[2][JACOBI_MAX];   // column j of the current matrix (double-buffered by step parity)
+  double col[2][JACOBI_MAX];   // column j + 1 as it stands before step j's update (double-buffered by parity)
   double v[2][JACOBI_MAX];     // the reflector v (v[j+1] = 1, zeros elsewhere outside j+1..m-1)
   double p[2][JACOBI_MAX];     // tau A v
+  double colb[JACOBI_MAX];     // the same for the last step of a column block (column j + 1 in the next block)
   double tv[2];                // tau
 };
+
+// the value of x in lane k (k uniform)
+__device__ __forceinline__ double lane_of(double x, int k) {
+  const long long b = __double_as_longlong(x);
+  const int lo = __builtin_amdgcn_readlane((int)b, k), hi = __builtin_amdgcn_readlane((int)(b >> 32), k);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// wavefront 0: the reflector H = I - tau v v^T with v[j+1] = 1 that maps column j's rows j+1.. (c: rows lane,
+// lane + 64, lane + 128 of the lane) to (beta, 0, ..., 0); publishes v and tau (buffer j & 1) and writes v to
+// Vh, tau and beta to the outputs
+__device__ __forceinline__ void trd_reflect(const double (&c)[3], int j, int m, int lane, TrdShared& sh,
+                                            double* __restrict__ Vh, double* __restrict__ tauo,
+                                            double* __restrict__ eo) {
+  double sg = 0.0;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const int r = lane + 64 * k;
+    sg += (r > j + 1 && r < m) ? c[k] * c[k] : 0.0;
+  }
+  sg = wave_sum(sg);
+  // alpha = c at row j + 1: read from its lane in each of the three slots, then the slot chosen (a select of
+  // values, not of an index into c, which would send c to scratch memory)
+  const int jr = j + 1, sl = jr >> 6;
+  const double a0 = lane_of(c[0], jr & 63), a1 = lane_of(c[1], jr & 63), a2 = lane_of(c[2], jr & 63);
+  const double al = sl == 0 ? a0 : (sl == 1 ? a1 : a2);
+  double be = al, tv = 0.0, sc = 0.0;
+  if (sg > 0.0) {
+    const double nr = sqrt(al * al + sg);
+    be = al >= 0.0 ? -nr : nr;
+    const double d = al - be;             // |d| >= |al|: no cancellation
+    const double inv = 1.0 / (be * d);    // tau = (beta - alpha) / beta = -d^2 inv, 1 / d = beta inv
+    tv = -d * d * inv;
+    sc = be * inv;
+  }
+  double* vv = sh.v[j & 1];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const int r = lane + 64 * k;
+    const double v = r == j + 1 ? 1.0 : ((r > j + 1 && r < m) ? c[k] * sc : 0.0);
+    vv[r] = v;
+    if (r < m) Vh[r + (size_t)j * m] = v;
+  }
+  if (lane == 0) {
+    sh.tv[j & 1] = tv;
+    tauo[j] = tv;
+    eo[j] = be;
+  }
+}
 
 // Householder steps j = TD_GX BJ .. min(TD_GX (BJ + 1) - 1, m - 3).  Columns below TD_GX BJ are finished (they lie
 // left of every remaining step's column), so only column blocks b >= BJ are touched; row blocks whose
 // rows all lie at or above j are skipped: those finished before the block's first step statically (A0),
-// the others by a uniform test.  Per step: the owner lanes publish column j, wavefront 0 forms the
-// reflector and publishes v and tau, every thread forms its share of p = tau A v, then the update; three
-// barriers.
+// the others by a uniform test.  Step j finds v_j and tau_j published (by wavefront 0, during step j - 1, or
+// before the first step); every thread forms its share of p = tau A v, and the owners of column j + 1
+// publish it as it stands; after the second barrier every thread updates its elements while wavefront 0
+// forms column j + 1 after the update from the published column, v, p and (p . v) alone, then the next
+// reflector from it, so two barriers per step.
 template <int BJ>
 __device__ __forceinline__ void trd_block(double (&A)[TD_RA][TD_RB], int m, TrdShared& sh, int tx, int ty,
                                           int lane, int tid, double* __restrict__ Vh, double* __restrict__ tauo,
@@ -128,46 +181,9 @@ __device__ __forceinline__ void trd_block(double (&A)[TD_RA][TD_RB], int m, TrdS
   const int j0 = BJ * TD_GX, j1 = min(m - 2, j0 + TD_GX);
   for (int j = j0; j < j1; ++j) {
     const int buf = j & 1;
-    double* col = sh.col[buf];
-    double* vv = sh.v[buf];
+    const double* vv = sh.v[buf];
     double* pv = sh.p[buf];
-    if (tx == j - j0) {
-#pragma unroll
-      for (int a = A0; a < TD_RA; ++a) col[ty + TD_GY * a] = A[a][BJ];
-    }
-    __syncthreads();
-    if (tid < 64) {   // the reflector H = I - tau v v^T with v[j+1] = 1 that maps A[j+1:, j] to (beta, 0, ..., 0)
-      double sg = 0.0;
-      for (int r = j + 2 + lane; r < m; r += 64) {
-        const double x = col[r];
-        sg += x * x;
-      }
-      sg = wave_sum(sg);
-      const double al = col[j + 1];
-      double be = al, tv = 0.0, sc = 0.0;
-      if (sg > 0.0) {
-        const double nr = sqrt(al * al + sg);
-        be = al >= 0.0 ? -nr : nr;
-        const double d = al - be;             // |d| >= |al|: no cancellation
-        const double inv = 1.0 / (be * d);    // tau = (beta - alpha) / beta = -d^2 inv, 1 / d = beta inv
-        tv = -d * d * inv;
-        sc = be * inv;
-      }
-      // rows of col at or above j may hold stale values: the selects drop them
-#pragma unroll
-      for (int k = 0; k < JACOBI_MAX / 64; ++k) {
-        const int r = lane + 64 * k;
-        const double x = col[r] * sc;
-        const double v = r == j + 1 ? 1.0 : ((r > j + 1 && r < m) ? x : 0.0);
-        vv[r] = v;
-        if (r < m) Vh[r + (size_t)j * m] = v;
-      }
-      if (lane == 0) {
-        sh.tv[buf] = tv;
-        tauo[j] = tv;
-        eo[j] = be;
-      }
-    }
+    const bool next = j + 1 < m - 2;   // uniform: a step j + 1 follows
     __syncthreads();
     const double tv = sh.tv[buf];
     // p = tau A v on the rows below j (finished row blocks are skipped and never stored); column by column,
@@ -203,12 +219,39 @@ __device__ __forceinline__ void trd_block(double (&A)[TD_RA][TD_RB], int m, TrdS
       const int r = ty + TD_GY * a;
       if (tx == (TD_GX == 32 ? 16 : 0)) pv[r] = (r > j && r < m) ? tv * ps[a] : 0.0;   // a lane that holds the row total
     }
+    // column j + 1 before this step's update (its rows in finished blocks are never read)
+    if (next) {
+      double* cn = sh.col[buf];
+      if (j + 1 < j0 + TD_GX) {
+        if (tx == j + 1 - j0) {
+#pragma unroll
+          for (int a = A0; a < TD_RA; ++a) cn[ty + TD_GY * a] = A[a][BJ];
+        }
+      } else if constexpr (BJ + 1 < TD_RB) {   // a buffer of its own: one store merged from the two
+        if (tx == 0) {                          // branches would index A dynamically (A in scratch memory)
+#pragma unroll
+          for (int a = A0; a < TD_RA; ++a) sh.colb[ty + TD_GY * a] = A[a][BJ + 1];
+        }
+      }
+    }
     __syncthreads();
     // w = p - (tau / 2) (p . v) v ;  A -= v w^T + w v^T
     double pw = 0.0;
     for (int r = j + 1 + lane; r < m; r += 64) pw += pv[r] * vv[r];
     pw = wave_sum(pw);
     const double kc = 0.5 * tv * pw;
+    if (next && tid < 64) {   // column j + 1 after the update (v[j+1] = 1), then the next reflector
+      const double* cn = j + 1 < j0 + TD_GX ? sh.col[buf] : sh.colb;
+      const double wj1 = pv[j + 1] - kc;
+      double c[3];
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        const int r = lane + 64 * k;
+        const double x = cn[r] - vv[r] * wj1 - (pv[r] - kc * vv[r]);
+        c[k] = r >= j + 2 ? x : 0.0;   // rows above hold stale values
+      }
+      trd_reflect(c, j + 1, m, lane, sh, Vh, tauo, eo);
+    }
     // the thread's rows of v and w, then the update column by column (p is written this step only for
     // rows in unskipped blocks: columns at or left of j take w = 0)
     double vr[TD_RA], wr[TD_RA];
@@ -259,6 +302,19 @@ __global__ __launch_bounds__(TD_NT) void trd_kernel(const double* __restrict__ H
       const double x = H[in ? min(r, c) + (size_t)max(r, c) * ldh : 0];   // branch-free, clamped address
       A[a][b] = in ? x : 0.0;
     }
+  if (m > 2) {   // the first reflector, from column 0 (published, then formed by wavefront 0)
+    if (tx == 0) {
+#pragma unroll
+      for (int a = 0; a < TD_RA; ++a) sh.col[1][ty + TD_GY * a] = A[a][0];
+    }
+    __syncthreads();
+    if (tid < 64) {
+      double c[3];
+#pragma unroll
+      for (int k = 0; k < 3; ++k) c[k] = sh.col[1][lane + 64 * k];
+      trd_reflect(c, 0, m, lane, sh, Vh, tauo, eout);
+    }
+  }
   trd_blocks<0>(A, m, sh, tx, ty, lane, tid, Vh, tauo, eout);
 #pragma unroll
   for (int a = 0; a < TD_RA; ++a)
@@ -1072,13 +1128,6 @@ __global__ __launch_bounds__(64) void trd_gram_kernel(const double* __restrict__
   }
 }
 
-// lane k of a double held by lanes 0..5
-__device__ __forceinline__ double lane_d(double x, int k) {
-  const long long b = __double_as_longlong(x);
-  const int lo = __builtin_amdgcn_readlane((int)b, k), hi = __builtin_amdgcn_readlane((int)(b >> 32), k);
-  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
-}
-
 __device__ __forceinline__ void back_load(const double* __restrict__ Vh, const double* __restrict__ tau,
                                           const double* __restrict__ gram, int m, int q0, int lane,
                                           double (&u)[BK_Q][3], double (&t)[BK_Q], double (&g)[BK_Q / 4]) {
@@ -1117,8 +1166,8 @@ __device__ __forceinline__ void back_apply(double (&z)[3], const double (&uraw)[
     for (int q = 0; q < 4; ++q) d[q] = u[b + q][0] * z[0] + u[b + q][1] * z[1] + u[b + q][2] * z[2];
     wave_sums<4>(d);
     const double gb = g[b / 4];
-    const double g10 = lane_d(gb, 0), g20 = lane_d(gb, 1), g21 = lane_d(gb, 2), g30 = lane_d(gb, 3),
-                 g31 = lane_d(gb, 4), g32 = lane_d(gb, 5);
+    const double g10 = lane_of(gb, 0), g20 = lane_of(gb, 1), g21 = lane_of(gb, 2), g30 = lane_of(gb, 3),
+                 g31 = lane_of(gb, 4), g32 = lane_of(gb, 5);
     const double s0 = t[b] * d[0];
     const double s1 = t[b + 1] * (d[1] - s0 * g10);
     const double s2 = t[b + 2] * (d[2] - s0 * g20 - s1 * g21);
