@@ -958,10 +958,20 @@ __global__ __launch_bounds__(DC_NT) void dc_kernel(const double* __restrict__ d,
   const bool odd = depth > 0 && ((depth - 1) & 1);
   double* Qin = odd ? Qa : Qb;
   double* Qout = odd ? Qb : Qa;
-  for (int idx = tid; idx < m * m; idx += DC_NT) {
-    const int r = idx % m, c = idx / m;
-    Qin[idx] = r == c ? 1.0 : 0.0;
-    Qout[idx] = 0.0;
+  // Qin = I, Qout = 0; the row and column of each element advance without a division per element
+  {
+    const int rstep = DC_NT % m, cstep = DC_NT / m;
+    int r = tid % m, c = tid / m;
+    for (int idx = tid; idx < m * m; idx += DC_NT) {
+      Qin[idx] = r == c ? 1.0 : 0.0;
+      Qout[idx] = 0.0;
+      r += rstep;
+      c += cstep;
+      if (r >= m) {
+        r -= m;
+        ++c;
+      }
+    }
   }
   // leaves: every boundary is a tear, d_i -= |e_(i-1)| + |e_i|
   if (tid < m) {
