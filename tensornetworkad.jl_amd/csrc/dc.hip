@@ -999,7 +999,7 @@ __global__ __launch_bounds__(DC_NT) void dc_kernel(const double* __restrict__ d,
 
 // the top merge's secular roots, one wavefront per root (all 64 lanes share its sums): root j's pole and
 // offset, and its eigenvalue
-__global__ __launch_bounds__(64) void dc_sec_kernel(DcTop top, int m, double* __restrict__ lam_out,
+__global__ __launch_bounds__(64) void dc_sec_kernel(DcTop top, double* __restrict__ lam_out,
                                                     const int* __restrict__ gate) {
   if (gate && *gate == 0) return;
   const int j = blockIdx.x, lane = threadIdx.x;
@@ -1022,7 +1022,7 @@ __global__ __launch_bounds__(64) void dc_sec_kernel(DcTop top, int m, double* __
 }
 
 // the top merge's z recomputed from its roots (Lowner's formula), one wavefront per entry
-__global__ __launch_bounds__(64) void dc_zh_kernel(DcTop top, int m, const int* __restrict__ gate) {
+__global__ __launch_bounds__(64) void dc_zh_kernel(DcTop top, const int* __restrict__ gate) {
   if (gate && *gate == 0) return;
   const int i = blockIdx.x, lane = threadIdx.x;
   const int K = *top.K;
@@ -1259,9 +1259,9 @@ void dc_solve(const DcWork& w, int m, const int* gate, hipStream_t s) {
   hipLaunchKernelGGL(dc_kernel, dim3(1), dim3(DC_NT), 0, s, w.dv, w.ev, m, w.Qa, w.Qb, w.U, w.top, w.lam, gate);
   TNAD_HIP_TRY(hipGetLastError());
   if (m > 1) {
-    hipLaunchKernelGGL(dc_sec_kernel, dim3(m), dim3(64), 0, s, w.top, m, w.lam, gate);
+    hipLaunchKernelGGL(dc_sec_kernel, dim3(m), dim3(64), 0, s, w.top, w.lam, gate);
     TNAD_HIP_TRY(hipGetLastError());
-    hipLaunchKernelGGL(dc_zh_kernel, dim3(m), dim3(64), 0, s, w.top, m, gate);
+    hipLaunchKernelGGL(dc_zh_kernel, dim3(m), dim3(64), 0, s, w.top, gate);
     TNAD_HIP_TRY(hipGetLastError());
   }
 }
