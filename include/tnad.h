@@ -231,7 +231,8 @@ int tnad_time_jacobi(tnad_ctx* ctx, int m, int sweeps, int reps, double* avg_ms,
 /* Roofline timing of the divide-and-conquer symmetric eigensolver (no counterpart in the reference): its
  * stages on a random symmetric m x m matrix (3 <= m <= 192), averaged over reps runs after one warm-up, HIP
  * events on the context's stream: stage_ms[0] Householder tridiagonalisation (one workgroup), stage_ms[1]
- * divide and conquer on the tridiagonal matrix (one workgroup), stage_ms[2] back-transformation and sort. */
+ * divide and conquer on the tridiagonal matrix (one workgroup, then the top merge's roots and z over the
+ * chip), stage_ms[2] back-transformation, the top merge's product and the sort. */
 int tnad_time_syev(tnad_ctx* ctx, int m, int reps, double* stage_ms);
 
 /* ---------------------------------------------------------------------------------------------
